@@ -1,0 +1,348 @@
+// dtb_abi.cpp -- extern "C" boundary of libdtb.so (declared in include/dtb_abi.h).
+#include "dtb_abi.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "dtb_engine.hpp"
+#include "dtb_gen.hpp"
+#include "dtb_kernels.hpp"
+
+struct dtb_engine_s {
+    std::unique_ptr<dtb::Engine> eng;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F f)
+{
+    try {
+        g_last_error.clear();
+        f();
+        return DTB_OK;
+    } catch (const dtb::Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return DTB_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return DTB_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return DTB_INTERNAL;
+    }
+}
+
+dtb::Engine &engine(dtb_engine_t h)
+{
+    if (h == nullptr || !h->eng)
+        throw dtb::Error(dtb::kInvalidParameter, "null engine handle");
+    return *h->eng;
+}
+
+void need(bool ok, const char *what)
+{
+    if (!ok)
+        throw dtb::Error(dtb::kInvalidParameter, what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dtb_version(void)
+{
+    return DTB_ABI_VERSION;
+}
+
+const char *dtb_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+int dtb_device_count(int *count)
+{
+    return guarded([&] {
+        need(count != nullptr, "count is NULL");
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess)
+            c = 0;
+        *count = c;
+    });
+}
+
+int dtb_engine_create(int device, dtb_engine_t *out)
+{
+    return guarded([&] {
+        need(out != nullptr, "out is NULL");
+        auto h = std::make_unique<dtb_engine_s>();
+        h->eng = std::make_unique<dtb::Engine>(device);
+        *out = h.release();
+    });
+}
+
+int dtb_engine_destroy(dtb_engine_t h)
+{
+    return guarded([&] {
+        if (h == nullptr)
+            return;
+        if (h->eng)
+            h->eng->sync();
+        delete h;
+    });
+}
+
+int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t seed, int *inst_id)
+{
+    return guarded([&] {
+        need(net != nullptr && inst_id != nullptr, "network and inst_id are required");
+        dtb::Engine &e = engine(h);
+        DTB_HIP(hipSetDevice(e.device()));
+        dtb::NetworkHost nh;
+        nh.n = net->n;
+        nh.props = net->props;
+        nh.nsyn = net->nsyn;
+        nh.rowptr = net->rowptr;
+        nh.col = net->colinds;
+        nh.w = net->weights;
+        nh.kind_bits = net->connkinds;
+        e.sync();
+        auto inst = std::make_unique<dtb::Instance>(nh, dt, seed, e.stream());
+        *inst_id = e.add(std::move(inst));
+    });
+}
+
+int dtb_instance_remove(dtb_engine_t h, int inst)
+{
+    return guarded([&] { engine(h).remove(inst); });
+}
+
+int dtb_instance_info(dtb_engine_t h, int inst, int64_t *n, int64_t *nsyn, int32_t *npop, uint32_t *pop_ids,
+                      uint32_t *pop_begin)
+{
+    return guarded([&] {
+        dtb::Instance *I = engine(h).get(inst);
+        if (n)
+            *n = I->n();
+        if (nsyn)
+            *nsyn = I->nsyn();
+        if (npop)
+            *npop = (int32_t)I->npop();
+        if (pop_ids)
+            std::memcpy(pop_ids, I->pop_ids().data(), sizeof(uint32_t) * I->npop());
+        if (pop_begin)
+            std::memcpy(pop_begin, I->pop_begin().data(), sizeof(uint32_t) * (I->npop() + 1));
+    });
+}
+
+int dtb_set_ou(dtb_engine_t h, int inst, const uint32_t *brain_ids, const float *mean, const float *deviation,
+               const float *correlation_time, int32_t n)
+{
+    return guarded([&] {
+        need(n >= 0, "negative count");
+        need(n == 0 || (brain_ids && mean && deviation && correlation_time), "OU arrays are required");
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->set_ou(brain_ids, mean, deviation, correlation_time, (size_t)n, e.stream());
+        e.mark_dirty();
+    });
+}
+
+int dtb_set_samples(dtb_engine_t h, int inst, const uint32_t *ids, int32_t n)
+{
+    return guarded([&] {
+        need(n >= 0 && (n == 0 || ids), "sample ids are required");
+        dtb::Engine &e = engine(h);
+        e.get(inst)->set_samples(ids, (size_t)n);
+        e.mark_dirty();
+    });
+}
+
+int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const uint32_t *prop_ids,
+                     const float *vals, int32_t n)
+{
+    return guarded([&] {
+        need(n >= 0 && (n == 0 || (neuron_ids && prop_ids && vals)), "property arrays are required");
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->update_props(neuron_ids, prop_ids, vals, (size_t)n, e.stream());
+    });
+}
+
+int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, const uint32_t *brain_ids,
+                         const float *vals, int32_t n, int32_t assign)
+{
+    return guarded([&] {
+        need(n >= 0 && (n == 0 || (prop_ids && brain_ids && vals)), "property arrays are required");
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->update_prop_cols(prop_ids, brain_ids, vals, (size_t)n, assign != 0, e.stream());
+    });
+}
+
+int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, const dtb_run_outputs *out)
+{
+    return guarded([&] {
+        // dist_simulator.cpp:3225-3234 parameter checks.
+        need(iters > 0, "iter must be positive");
+        dtb::Engine &e = engine(h);
+        dtb::Instance *I = e.get(inst);
+        const bool use_ou = (flags & DTB_RUN_USE_OU) != 0;
+        const bool want_samples =
+            (flags & (DTB_RUN_SAMPLE_SPIKE | DTB_RUN_SAMPLE_VMEMB | DTB_RUN_SAMPLE_ISYN | DTB_RUN_SAMPLE_IOU)) != 0;
+        if ((flags & DTB_RUN_SAMPLE_IOU) && !use_ou)
+            throw dtb::Error(dtb::kInvalidParameter, "i_ou samples need the OU background");
+        if (want_samples && I->samples().empty())
+            throw dtb::Error(dtb::kUnsetSampleFailure, "samples are not set");
+        e.sync();
+        I->configure_run(t_steps, use_ou, e.stream());
+        const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
+        const bool samples = want_samples && out &&
+                             (out->sample_spike || out->sample_vmemb || out->sample_isynaptic || out->sample_iou);
+        I->begin_record(iters, freq, samples, e.stream());
+        e.mark_dirty();
+        e.run(iters, (flags & DTB_RUN_GRAPH) ? 64 : 0);
+        e.sync();
+        if (out)
+            I->fetch_record(freq ? out->freq : nullptr, (flags & DTB_RUN_SAMPLE_SPIKE) ? out->sample_spike : nullptr,
+                            (flags & DTB_RUN_SAMPLE_VMEMB) ? out->sample_vmemb : nullptr,
+                            (flags & DTB_RUN_SAMPLE_ISYN) ? out->sample_isynaptic : nullptr,
+                            (flags & DTB_RUN_SAMPLE_IOU) ? out->sample_iou : nullptr, e.stream());
+        I->end_record();
+        e.mark_dirty();
+    });
+}
+
+int dtb_engine_run(dtb_engine_t h, int64_t steps, int32_t graph_chunk)
+{
+    return guarded([&] {
+        need(steps >= 0, "negative step count");
+        engine(h).run(steps, graph_chunk);
+    });
+}
+
+int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        e.sync();
+        if (e.get(inst)->configure_run(t_steps, use_ou != 0, e.stream()))
+            e.mark_dirty();
+    });
+}
+
+int dtb_engine_sync(dtb_engine_t h)
+{
+    return guarded([&] { engine(h).sync(); });
+}
+
+int dtb_engine_set_prop_blocks(dtb_engine_t h, uint32_t nblocks)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        if (nblocks == 0) {
+            int cus = 0;
+            DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device()));
+            nblocks = (uint32_t)(cus > 0 ? cus : 1) * 8u;
+        }
+        e.set_prop_blocks(nblocks);
+        e.mark_dirty();
+    });
+}
+
+int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_neuron, float *ms_prop)
+{
+    return guarded([&] {
+        need(steps >= 0, "negative step count");
+        engine(h).run_timed(steps, ms_neuron, ms_prop);
+    });
+}
+
+int dtb_read_state(dtb_engine_t h, int inst, float *v, int32_t *t_ref_count, uint8_t *f, uint8_t *emitted,
+                   float *i_ou, float *j, float *i_syn)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->read_state(v, t_ref_count, f, emitted, i_ou, j, i_syn, e.stream());
+    });
+}
+
+int dtb_write_state(dtb_engine_t h, int inst, const float *v, const int32_t *t_ref_count, const uint8_t *f,
+                    const float *i_ou, const float *j)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->write_state(v, t_ref_count, f, i_ou, j, e.stream());
+    });
+}
+
+int dtb_set_step(dtb_engine_t h, int inst, int64_t step)
+{
+    return guarded([&] {
+        need(step >= 0, "negative step");
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->set_step(step, e.stream());
+    });
+}
+
+int dtb_counters(dtb_engine_t h, int inst, int64_t *step, uint64_t *events, uint64_t *spikes)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        e.sync();
+        e.get(inst)->counters(step, events, spikes, e.stream());
+    });
+}
+
+int dtb_ou_normals(int device, uint64_t seed, uint32_t neuron0, uint64_t substep, uint32_t count, float *out)
+{
+    return guarded([&] {
+        need(out != nullptr || count == 0, "out is NULL");
+        int nd = 0;
+        DTB_HIP(hipGetDeviceCount(&nd));
+        if (device < 0 || device >= nd)
+            throw dtb::Error(dtb::kNoDevice, "device index out of range");
+        DTB_HIP(hipSetDevice(device));
+        if (count == 0)
+            return;
+        float *d = nullptr;
+        DTB_HIP(hipMalloc(reinterpret_cast<void **>(&d), sizeof(float) * count));
+        hipError_t err = dtb::launch_normals(seed, neuron0, substep, count, d, nullptr);
+        if (err == hipSuccess)
+            err = hipMemcpy(out, d, sizeof(float) * count, hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        DTB_HIP(err);
+    });
+}
+
+int dtb_gen_uniform(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint32_t sub_base, float *props,
+                    int64_t *rowptr, uint32_t *colinds, float *weights, uint32_t *connkinds)
+{
+    return guarded([&] {
+        need(g_ui && props && rowptr && colinds && weights && connkinds, "output arrays are required");
+        dtb::gen_uniform(n, k, seed, g_ui, sub_base, props, rowptr, colinds, weights, connkinds);
+    });
+}
+
+int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
+                  const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
+                  uint32_t *connkinds, uint32_t *vox_sizes)
+{
+    return guarded([&] {
+        need(g_ui && props && rowptr && colinds && weights && connkinds, "output arrays are required");
+        dtb::gen_voxel(n, nvox, k, intra_frac, density, seed, g_ui, props, rowptr, colinds, weights, connkinds,
+                       vox_sizes);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,776 @@
+// dtb_engine.cpp -- host runtime of the DTB step (see dtb_engine.hpp).
+//
+// Host arithmetic that produces kernel inputs (1/C, refractory steps,
+// exp(-dt/tau), OU factors, V0) is done here once per run configuration, in
+// the order the CPU oracle uses, with contraction disabled, so that the GPU
+// and the oracle start from bit-identical parameters.
+#include "dtb_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "dtb_kernels.hpp"
+
+namespace dtb {
+
+void hip_check(hipError_t e, const char *what, const char *file, int line)
+{
+    if (e == hipSuccess)
+        return;
+    std::string msg = std::string(hipGetErrorString(e)) + " in " + what + " (" + file + ":" + std::to_string(line) + ")";
+    if (e == hipErrorOutOfMemory)
+        throw Error(kOutOfMemory, msg);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+        throw Error(kNoDevice, msg);
+    throw Error(kHipError, msg);
+}
+
+int prop_type_column(int prop)
+{
+    // dist_blockwrapper.py:32-33 maps columns to PropType with
+    // [19, -1, 0, -1, 1, 2, ..., 18]; this is the inverse.
+    static const int kCol[kNumPropTypes] = {kColIExt,  kColC,      kColTRef,   kColGLeak, kColVLeak,
+                                            kColVTh,   kColVReset, kColGU + 0, kColGU + 1, kColGU + 2,
+                                            kColGU + 3, kColVU + 0, kColVU + 1, kColVU + 2, kColVU + 3,
+                                            kColTauU + 0, kColTauU + 1, kColTauU + 2, kColTauU + 3, kColNoiseRate};
+    if (prop < 0 || prop >= kNumPropTypes)
+        return -1;
+    return kCol[prop];
+}
+
+size_t data_size(DataType t)
+{
+    switch (t) {
+    case DOUBLE:
+        return 8;
+    case FLOAT:
+    case INT32:
+        return 4;
+    case FLOAT16:
+        return 2;
+    case INT8:
+        return 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side parameter derivation.  Mirrors the oracle exactly.
+
+static inline float host_expf(float x)
+{
+    return (float)std::exp((double)x);
+}
+
+static inline int32_t refractory_substeps(float t_ref, float dt, int32_t t_steps)
+{
+    // property.cpp:45-53 (t_ref / dt at init, block.cpp:150) and
+    // membrane_voltage.cpp:177 ((int)t_ref * t_steps).
+    const float steps = t_ref / dt;
+    return (int32_t)steps * t_steps;
+}
+
+static inline void ou_factors(float dt_sub, float tau, float sigma, float *f1, float *f2s)
+{
+    // extern_stimuli.cpp:87-91
+    const float factor = (-dt_sub) / tau;
+    const float a = 1.0f - host_expf(factor);
+    const float b = std::sqrt(1.0f - host_expf(2.0f * factor));
+    *f1 = a;
+    *f2s = b * sigma;
+}
+
+static inline float bits_as_float(int32_t v)
+{
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+
+Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t s)
+{
+    if (net.n <= 0 || net.n > (int64_t)kColMask)
+        throw Error(kInvalidNetwork, "neuron count out of range");
+    if (net.props == nullptr || net.rowptr == nullptr)
+        throw Error(kInvalidParameter, "props and rowptr are required");
+    if (net.nsyn < 0 || (net.nsyn > 0 && (net.col == nullptr || net.w == nullptr || net.kind_bits == nullptr)))
+        throw Error(kInvalidParameter, "synapse arrays are required");
+    if (!(dt > 0.0f))
+        throw Error(kInvalidParameter, "dt must be positive");
+    n_ = (uint32_t)net.n;
+    nsyn_ = net.nsyn;
+    dt_ = dt;
+    seed_ = seed;
+
+    // CSR validation (rows ascending, columns in range).
+    if (net.rowptr[0] != 0 || net.rowptr[net.n] != net.nsyn)
+        throw Error(kInvalidNetwork, "rowptr must start at 0 and end at nsyn");
+    for (int64_t i = 0; i < net.n; ++i)
+        if (net.rowptr[i + 1] < net.rowptr[i] || net.rowptr[i + 1] - net.rowptr[i] > (int64_t)0xffffffffu)
+            throw Error(kInvalidNetwork, "rowptr must be non-decreasing");
+    for (int64_t j = 0; j < net.nsyn; ++j)
+        if (net.col[j] >= n_)
+            throw Error(kInvalidNetwork, "column index out of range");
+
+    props_.assign(net.props, net.props + (size_t)net.n * kNumCols);
+
+    // Populations: contiguous runs of the sub-block column
+    // (configuration.cpp:1902-1930 requires each sub-block to be contiguous).
+    std::unordered_set<uint32_t> seen;
+    for (uint32_t i = 0; i < n_; ++i) {
+        const float v = props_[(size_t)i * kNumCols + kColSubIdx];
+        if (!(v >= 0.0f) || v > 4294967040.0f)
+            throw Error(kInvalidNetwork, "sub-block id must be a non-negative integer");
+        const uint32_t id = (uint32_t)v;
+        if (i == 0 || id != pop_ids_.back()) {
+            if (!seen.insert(id).second)
+                throw Error(kInvalidNetwork, "sub-block ids must form contiguous ranges");
+            pop_ids_.push_back(id);
+            pop_begin_.push_back(i);
+        }
+    }
+    pop_begin_.push_back(n_);
+    const size_t np = pop_ids_.size();
+    ou_on_.assign(np, 0);
+    ou_mean_.assign(np, 0.0f);
+    ou_sigma_.assign(np, 0.0f);
+    ou_tau_.assign(np, 1.0f);
+
+    pA_.resize(n_);
+    pB_.resize(n_);
+    gU_.resize(n_);
+    vU_.resize(n_);
+    dec_.resize(n_);
+    J_.resize(n_);
+    blocked_.resize(n_);
+    flag_.resize(n_);
+    V_.resize(n_);
+    iou_.resize(n_);
+    tcnt_.resize(n_);
+    dq_.resize((size_t)n_ * 4);
+    rowptr_.resize((size_t)n_ + 1);
+    col_.resize((size_t)std::max<int64_t>(nsyn_, 1));
+    w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
+    pops_.resize(np);
+    step_.resize(1);
+    counters_.resize(2);
+
+    derive_params(s);
+    init_state(s);
+
+    rowptr_.upload(net.rowptr, (size_t)n_ + 1, s);
+    // Pack the channel bit into bit 31 of the column index so that one
+    // 4-byte load per synapse carries both.
+    const int64_t chunk = (int64_t)1 << 24;
+    std::vector<uint32_t> stage((size_t)std::min<int64_t>(std::max<int64_t>(nsyn_, 1), chunk));
+    for (int64_t base = 0; base < nsyn_; base += chunk) {
+        const int64_t m = std::min<int64_t>(chunk, nsyn_ - base);
+        for (int64_t j = 0; j < m; ++j) {
+            const int64_t g = base + j;
+            const uint32_t kind = (net.kind_bits[g >> 5] >> (g & 31)) & 1u;
+            stage[(size_t)j] = net.col[g] | (kind ? kKindBit : 0u);
+        }
+        DTB_HIP(hipMemcpyAsync(col_.mutable_gpu_data() + base, stage.data(), (size_t)m * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        DTB_HIP(hipStreamSynchronize(s));
+    }
+    if (nsyn_ > 0)
+        w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+Instance::~Instance() = default;
+
+void Instance::derive_params(hipStream_t s)
+{
+    const float dt_sub = dt_ / (float)t_steps_;
+    float4 *pa = pA_.mutable_cpu_data();
+    float4 *pb = pB_.mutable_cpu_data();
+    float4 *gu = gU_.mutable_cpu_data();
+    float4 *vu = vU_.mutable_cpu_data();
+    float4 *dc = dec_.mutable_cpu_data();
+    uint8_t *bl = blocked_.mutable_cpu_data();
+    for (uint32_t i = 0; i < n_; ++i) {
+        const float *p = &props_[(size_t)i * kNumCols];
+        pa[i] = make_float4(p[kColVReset], p[kColVTh], 1.0f / p[kColC], p[kColVLeak]);
+        pb[i] = make_float4(p[kColGLeak], p[kColIExt], p[kColNoiseRate],
+                            bits_as_float(refractory_substeps(p[kColTRef], dt_, t_steps_)));
+        gu[i] = make_float4(p[kColGU + 0], p[kColGU + 1], p[kColGU + 2], p[kColGU + 3]);
+        vu[i] = make_float4(p[kColVU + 0], p[kColVU + 1], p[kColVU + 2], p[kColVU + 3]);
+        // property.cpp:19-29: exp(-dt / tau)
+        dc[i] = make_float4(host_expf((-dt_sub) / p[kColTauU + 0]), host_expf((-dt_sub) / p[kColTauU + 1]),
+                            host_expf((-dt_sub) / p[kColTauU + 2]), host_expf((-dt_sub) / p[kColTauU + 3]));
+        bl[i] = p[kColBlocked] != 0.0f;
+    }
+    PopDev *pd = pops_.mutable_cpu_data();
+    for (size_t q = 0; q < pop_ids_.size(); ++q) {
+        PopDev r{};
+        r.begin = pop_begin_[q];
+        r.end = pop_begin_[q + 1];
+        r.ou_on = ou_on_[q];
+        if (ou_on_[q]) {
+            ou_factors(dt_sub, ou_tau_[q], ou_sigma_[q], &r.f1, &r.f2s);
+            r.mu = ou_mean_[q];
+        }
+        pd[q] = r;
+    }
+    pA_.to_gpu(s);
+    pB_.to_gpu(s);
+    gU_.to_gpu(s);
+    vU_.to_gpu(s);
+    dec_.to_gpu(s);
+    blocked_.to_gpu(s);
+    pops_.to_gpu(s);
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+void Instance::init_state(hipStream_t s)
+{
+    // block.cpp:136-179 and membrane_voltage.cpp:18.
+    float *V = V_.mutable_cpu_data();
+    for (uint32_t i = 0; i < n_; ++i) {
+        const float *p = &props_[(size_t)i * kNumCols];
+        V[i] = (p[kColVTh] + p[kColVReset]) / 2.0f;
+    }
+    V_.to_gpu(s);
+    V_.release_cpu();
+    float *io = iou_.mutable_cpu_data();
+    std::fill(io, io + n_, 0.0f);
+    iou_.to_gpu(s);
+    DTB_HIP(hipMemsetAsync(tcnt_.mutable_gpu_data(), 0, tcnt_.size(), s));
+    DTB_HIP(hipMemsetAsync(flag_.mutable_gpu_data(), 0, flag_.size(), s));
+    DTB_HIP(hipMemsetAsync(J_.mutable_gpu_data(), 0, J_.size(), s));
+    DTB_HIP(hipMemsetAsync(dq_.mutable_gpu_data(), 0, dq_.size(), s));
+    DTB_HIP(hipMemsetAsync(step_.mutable_gpu_data(), 0, step_.size(), s));
+    DTB_HIP(hipMemsetAsync(counters_.mutable_gpu_data(), 0, counters_.size(), s));
+    DTB_HIP(hipStreamSynchronize(s));
+    iou_.release_cpu();
+}
+
+uint32_t Instance::find_pop(uint32_t brain_id) const
+{
+    for (size_t q = 0; q < pop_ids_.size(); ++q)
+        if (pop_ids_[q] == brain_id)
+            return (uint32_t)q;
+    return UINT32_MAX;
+}
+
+void Instance::set_ou(const uint32_t *brain_ids, const float *mean, const float *sigma, const float *tau,
+                      size_t n, hipStream_t s)
+{
+    // block.cpp:222-281.  A population seen for the first time has its OU
+    // current reset to the mean (reset_partial_gpu); a known one only has its
+    // parameters replaced.  Ids that are not in this block are skipped.
+    for (size_t k = 0; k < n; ++k) {
+        if (!(tau[k] > 0.0f))
+            throw Error(kInvalidParameter, "OU correlation time must be positive");
+    }
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t q = find_pop(brain_ids[k]);
+        if (q == UINT32_MAX)
+            continue;
+        const bool first = !ou_on_[q];
+        ou_on_[q] = 1;
+        ou_mean_[q] = mean[k];
+        ou_sigma_[q] = sigma[k];
+        ou_tau_[q] = tau[k];
+        if (first) {
+            const uint32_t b = pop_begin_[q], e = pop_begin_[q + 1];
+            std::vector<float> fill(e - b, mean[k]);
+            DTB_HIP(hipMemcpyAsync(iou_.mutable_gpu_data() + b, fill.data(), fill.size() * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+            DTB_HIP(hipStreamSynchronize(s));
+        }
+    }
+    derive_params(s);
+}
+
+void Instance::update_props(const uint32_t *neuron_ids, const uint32_t *prop_ids, const float *vals, size_t n,
+                            hipStream_t s)
+{
+    // property.cpp:57-183 sets one property of one neuron; the columns of the
+    // host property matrix are updated and the kernel inputs re-derived.
+    for (size_t k = 0; k < n; ++k) {
+        if (prop_type_column((int)prop_ids[k]) < 0)
+            throw Error(kInvalidParameter, "unknown property id");
+    }
+    for (size_t k = 0; k < n; ++k) {
+        if (neuron_ids[k] >= n_)
+            continue;   // property.cpp:73 ignores out-of-range neurons
+        props_[(size_t)neuron_ids[k] * kNumCols + prop_type_column((int)prop_ids[k])] = vals[k];
+    }
+    derive_params(s);
+}
+
+void Instance::update_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_ids, const float *vals, size_t n,
+                                bool assign, hipStream_t s)
+{
+    // property.cpp:185-466: per-population assign (=) or multiply (*=).
+    for (size_t k = 0; k < n; ++k) {
+        if (prop_type_column((int)prop_ids[k]) < 0)
+            throw Error(kInvalidParameter, "unknown property id");
+    }
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t q = find_pop(brain_ids[k]);
+        if (q == UINT32_MAX)
+            continue;
+        const int c = prop_type_column((int)prop_ids[k]);
+        for (uint32_t i = pop_begin_[q]; i < pop_begin_[q + 1]; ++i) {
+            float &x = props_[(size_t)i * kNumCols + c];
+            x = assign ? vals[k] : x * vals[k];
+        }
+    }
+    derive_params(s);
+}
+
+void Instance::set_samples(const uint32_t *ids, size_t n)
+{
+    for (size_t k = 0; k < n; ++k)
+        if (ids[k] >= n_)
+            throw Error(kInvalidParameter, "sample id out of range");
+    samples_.assign(ids, ids + n);
+}
+
+bool Instance::configure_run(int32_t t_steps, bool use_ou, hipStream_t s)
+{
+    if (t_steps < 1)
+        throw Error(kInvalidParameter, "t_steps must be >= 1");
+    if (t_steps > 1 && !use_ou)
+        throw Error(kInvalidParameter, "t_steps > 1 requires the OU background");   // dist_simulator.cpp:3225-3234
+    const bool changed = t_steps != t_steps_ || use_ou != use_ou_;
+    use_ou_ = use_ou;
+    if (t_steps != t_steps_) {
+        t_steps_ = t_steps;
+        derive_params(s);
+    }
+    return changed;
+}
+
+void Instance::begin_record(int64_t steps, bool freq, bool samples, hipStream_t s)
+{
+    if (samples && samples_.empty())
+        throw Error(kUnsetSampleFailure, "samples requested but not set");
+    rec_step0_ = step(s);
+    rec_len_ = steps;
+    rec_freq_ = freq;
+    rec_samples_ = samples;
+    if (freq) {
+        rec_counts_.resize((size_t)steps * npop());
+        DTB_HIP(hipMemsetAsync(rec_counts_.mutable_gpu_data(), 0, rec_counts_.size(), s));
+    }
+    if (samples) {
+        const size_t m = (size_t)steps * samples_.size();
+        rec_spike_.resize(m);
+        rec_v_.resize(m);
+        rec_isyn_.resize(m);
+        rec_iou_.resize(m);
+        rec_spike_.mutable_gpu_data();
+        rec_v_.mutable_gpu_data();
+        rec_isyn_.mutable_gpu_data();
+        rec_iou_.mutable_gpu_data();
+    }
+}
+
+void Instance::end_record()
+{
+    rec_freq_ = rec_samples_ = false;
+    rec_len_ = 0;
+}
+
+void Instance::fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, hipStream_t s)
+{
+    if (freq && rec_freq_)
+        rec_counts_.download(freq, rec_counts_.count(), s);
+    if (rec_samples_) {
+        if (spike)
+            rec_spike_.download(spike, rec_spike_.count(), s);
+        if (v)
+            rec_v_.download(v, rec_v_.count(), s);
+        if (isyn)
+            rec_isyn_.download(isyn, rec_isyn_.count(), s);
+        if (iou)
+            rec_iou_.download(iou, rec_iou_.count(), s);
+    }
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+int64_t Instance::step(hipStream_t s)
+{
+    int64_t v = 0;
+    step_.download(&v, 1, s);
+    DTB_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+void Instance::set_step(int64_t step, hipStream_t s)
+{
+    step_.upload(&step, 1, s);
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+void Instance::counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s)
+{
+    unsigned long long c[2] = {0, 0};
+    int64_t st = 0;
+    counters_.download(c, 2, s);
+    step_.download(&st, 1, s);
+    DTB_HIP(hipStreamSynchronize(s));
+    if (step)
+        *step = st;
+    if (events)
+        *events = c[0];
+    if (spikes)
+        *spikes = c[1];
+}
+
+static inline float q40_to_f32_host(long long q)
+{
+    return (float)((double)q * 0x1p-40);
+}
+
+void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, float *iou, float *J, float *isyn,
+                          hipStream_t s)
+{
+    std::vector<float> v(n_);
+    std::vector<uint8_t> fl(n_);
+    V_.download(v.data(), n_, s);
+    flag_.download(fl.data(), n_, s);
+    if (t)
+        tcnt_.download(t, n_, s);
+    if (iou)
+        iou_.download(iou, n_, s);
+    std::vector<float4> j, dcv;
+    std::vector<long long> q;
+    if (J || isyn) {
+        j.resize(n_);
+        q.resize((size_t)n_ * 4);
+        J_.download(j.data(), n_, s);
+        dq_.download(q.data(), (size_t)n_ * 4, s);
+    }
+    DTB_HIP(hipStreamSynchronize(s));
+    if (V)
+        std::memcpy(V, v.data(), sizeof(float) * n_);
+    for (uint32_t i = 0; i < n_; ++i) {
+        if (f)
+            f[i] = fl[i] & kFlagSpike;
+        if (emitted)
+            emitted[i] = (fl[i] & kFlagEmit) ? 1 : 0;
+    }
+    if (J || isyn) {
+        // Fold the pending delta exactly as the next neuron step will.
+        const float4 *dc = dec_.cpu_data();
+        const float4 *g = gU_.cpu_data();
+        const float4 *vu = vU_.cpu_data();
+        for (uint32_t i = 0; i < n_; ++i) {
+            const long long *qq = &q[(size_t)i * 4];
+            float4 r;
+            r.x = std::fma(j[i].x, dc[i].x, q40_to_f32_host(qq[0]));
+            r.y = std::fma(j[i].y, dc[i].y, q40_to_f32_host(qq[1]));
+            r.z = std::fma(j[i].z, dc[i].z, q40_to_f32_host(qq[2]));
+            r.w = std::fma(j[i].w, dc[i].w, q40_to_f32_host(qq[3]));
+            if (J) {
+                J[(size_t)i * 4 + 0] = r.x;
+                J[(size_t)i * 4 + 1] = r.y;
+                J[(size_t)i * 4 + 2] = r.z;
+                J[(size_t)i * 4 + 3] = r.w;
+            }
+            if (isyn) {
+                const float a = g[i].x * (vu[i].x - v[i]) * r.x;
+                const float b = g[i].y * (vu[i].y - v[i]) * r.y;
+                const float c = g[i].z * (vu[i].z - v[i]) * r.z;
+                const float d = g[i].w * (vu[i].w - v[i]) * r.w;
+                isyn[i] = a + b + c + d;
+            }
+        }
+    }
+}
+
+void Instance::write_state(const float *V, const int32_t *t, const uint8_t *f, const float *iou, const float *J,
+                           hipStream_t s)
+{
+    // Checkpoint restore: J is the reference-visible J (delta folded), so the
+    // pending delta is cleared.
+    if (V)
+        V_.upload(V, n_, s);
+    if (t)
+        tcnt_.upload(t, n_, s);
+    if (f) {
+        std::vector<uint8_t> fl(f, f + n_);
+        for (auto &x : fl)
+            x = x ? kFlagSpike : 0;
+        flag_.upload(fl.data(), n_, s);
+        DTB_HIP(hipStreamSynchronize(s));
+    }
+    if (iou)
+        iou_.upload(iou, n_, s);
+    if (J) {
+        J_.upload(reinterpret_cast<const float4 *>(J), n_, s);
+        DTB_HIP(hipMemsetAsync(dq_.mutable_gpu_data(), 0, dq_.size(), s));
+    }
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+InstDev Instance::dev()
+{
+    InstDev d{};
+    d.pA = pA_.gpu_data();
+    d.pB = pB_.gpu_data();
+    d.gU = gU_.gpu_data();
+    d.vU = vU_.gpu_data();
+    d.dec = dec_.gpu_data();
+    d.blocked = blocked_.gpu_data();
+    d.V = V_.mutable_gpu_data();
+    d.tcnt = tcnt_.mutable_gpu_data();
+    d.flag = flag_.mutable_gpu_data();
+    d.iou = iou_.mutable_gpu_data();
+    d.J = J_.mutable_gpu_data();
+    d.dq = dq_.mutable_gpu_data();
+    d.rowptr = rowptr_.gpu_data();
+    d.col = col_.gpu_data();
+    d.w = w_.gpu_data();
+    d.pops = pops_.gpu_data();
+    d.rec_counts = (rec_freq_ && rec_len_ > 0) ? rec_counts_.mutable_gpu_data() : nullptr;
+    d.events = counters_.mutable_gpu_data();
+    d.spikes = counters_.mutable_gpu_data() + 1;
+    d.step = step_.mutable_gpu_data();
+    d.seed = seed_;
+    d.rec_step0 = rec_step0_;
+    d.n = n_;
+    d.npop = npop();
+    d.rec_len = (uint32_t)rec_len_;
+    d.t_steps = t_steps_;
+    d.use_ou = use_ou_ ? 1 : 0;
+    d.dt_sub = dt_ / (float)t_steps_;
+    return d;
+}
+
+SampleRecDev Instance::sample_rec()
+{
+    SampleRecDev r{};
+    if (rec_samples_ && rec_len_ > 0) {
+        r.spike = rec_spike_.mutable_gpu_data();
+        r.v = rec_v_.mutable_gpu_data();
+        r.isyn = rec_isyn_.mutable_gpu_data();
+        r.iou = rec_iou_.mutable_gpu_data();
+        r.nsample = (uint32_t)samples_.size();
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+
+Engine::Engine(int device) : device_(device)
+{
+    int count = 0;
+    DTB_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        throw Error(kNoDevice, "device index out of range");
+    DTB_HIP(hipSetDevice(device));
+    DTB_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    d_ctl_.resize(1);
+    d_ctl_.mutable_gpu_data();
+    int cus = 0;
+    DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    prop_blocks_ = (uint32_t)std::max(1, cus) * 8u;
+}
+
+Engine::~Engine()
+{
+    drop_graph();
+    insts_.clear();
+    if (stream_)
+        (void)hipStreamDestroy(stream_);
+}
+
+int Engine::add(std::unique_ptr<Instance> inst)
+{
+    for (size_t i = 0; i < insts_.size(); ++i)
+        if (!insts_[i]) {
+            insts_[i] = std::move(inst);
+            dirty_ = true;
+            return (int)i;
+        }
+    insts_.push_back(std::move(inst));
+    dirty_ = true;
+    return (int)insts_.size() - 1;
+}
+
+std::unique_ptr<Instance> Engine::remove(int id)
+{
+    if (id < 0 || (size_t)id >= insts_.size() || !insts_[id])
+        throw Error(kInvalidParameter, "no such instance");
+    sync();
+    dirty_ = true;
+    return std::move(insts_[id]);
+}
+
+Instance *Engine::get(int id)
+{
+    if (id < 0 || (size_t)id >= insts_.size() || !insts_[id])
+        throw Error(kInvalidParameter, "no such instance");
+    return insts_[id].get();
+}
+
+size_t Engine::active() const
+{
+    size_t k = 0;
+    for (auto &p : insts_)
+        k += p ? 1 : 0;
+    return k;
+}
+
+void Engine::drop_graph()
+{
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+        graph_steps_ = 0;
+    }
+}
+
+void Engine::rebuild()
+{
+    drop_graph();
+    std::vector<InstDev> table;
+    std::vector<WgDesc> wgs;
+    std::vector<SampleDev> samples;
+    std::vector<SampleRecDev> recs;
+    uint64_t total = 0;
+    for (auto &p : insts_) {
+        if (!p)
+            continue;
+        const uint32_t idx = (uint32_t)table.size();
+        table.push_back(p->dev());
+        recs.push_back(p->sample_rec());
+        const auto &pb = p->pop_begin();
+        uint32_t q = 0;
+        for (uint32_t b = 0; b < p->n(); b += kBlock) {
+            while (pb[q + 1] <= b)
+                ++q;
+            WgDesc d;
+            d.inst = idx;
+            d.begin = b;
+            d.count = std::min<uint32_t>(kBlock, p->n() - b);
+            d.pop0 = q;
+            wgs.push_back(d);
+        }
+        if (recs.back().nsample > 0) {
+            const auto &ids = p->samples();
+            for (uint32_t k = 0; k < ids.size(); ++k)
+                samples.push_back(SampleDev{idx, ids[k], k, 0});
+        }
+        total += p->n();
+    }
+    if (total > 0xffffffffull)
+        throw Error(kInvalidParameter, "too many neurons in one engine");
+    ninst_ = (uint32_t)table.size();
+    nwg_ = (uint32_t)wgs.size();
+    nsamples_ = (uint32_t)samples.size();
+    fired_cap_ = (uint32_t)std::max<uint64_t>(total, 1);
+    d_insts_.resize(std::max<size_t>(table.size(), 1));
+    d_wgs_.resize(std::max<size_t>(wgs.size(), 1));
+    d_samples_.resize(std::max<size_t>(samples.size(), 1));
+    d_recs_.resize(std::max<size_t>(recs.size(), 1));
+    d_fired_.resize((size_t)fired_cap_ * 2);
+    d_fired_.mutable_gpu_data();
+    d_insts_.upload(table.data(), table.size(), stream_);
+    d_wgs_.upload(wgs.data(), wgs.size(), stream_);
+    d_samples_.upload(samples.data(), samples.size(), stream_);
+    d_recs_.upload(recs.data(), recs.size(), stream_);
+    DTB_HIP(hipStreamSynchronize(stream_));
+    dirty_ = false;
+}
+
+void Engine::launch_step()
+{
+    DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
+                               fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
+    DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
+                             d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
+    if (nsamples_ > 0)
+        DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(), stream_));
+}
+
+void Engine::run(int64_t steps, int graph_chunk)
+{
+    if (steps < 0)
+        throw Error(kInvalidParameter, "negative step count");
+    DTB_HIP(hipSetDevice(device_));
+    if (dirty_)
+        rebuild();
+    if (ninst_ == 0 || steps == 0)
+        return;
+    if (graph_chunk > 1 && steps >= graph_chunk) {
+        if (!graph_ || graph_steps_ != graph_chunk) {
+            drop_graph();
+            hipGraph_t g = nullptr;
+            DTB_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < graph_chunk; ++k)
+                launch_step();
+            DTB_HIP(hipStreamEndCapture(stream_, &g));
+            DTB_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+            DTB_HIP(hipGraphDestroy(g));
+            graph_steps_ = graph_chunk;
+        }
+        int64_t done = 0;
+        while (steps - done >= graph_chunk) {
+            DTB_HIP(hipGraphLaunch(graph_, stream_));
+            done += graph_chunk;
+        }
+        for (; done < steps; ++done)
+            launch_step();
+    } else {
+        for (int64_t k = 0; k < steps; ++k)
+            launch_step();
+    }
+}
+
+void Engine::run_timed(int64_t steps, float *ms_neuron, float *ms_prop)
+{
+    DTB_HIP(hipSetDevice(device_));
+    if (dirty_)
+        rebuild();
+    hipEvent_t e0, e1, e2;
+    DTB_HIP(hipEventCreate(&e0));
+    DTB_HIP(hipEventCreate(&e1));
+    DTB_HIP(hipEventCreate(&e2));
+    float tn = 0.0f, tp = 0.0f;
+    for (int64_t k = 0; k < steps; ++k) {
+        DTB_HIP(hipEventRecord(e0, stream_));
+        DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
+                                   fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
+        DTB_HIP(hipEventRecord(e1, stream_));
+        DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
+                                 d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
+        DTB_HIP(hipEventRecord(e2, stream_));
+        if (nsamples_ > 0)
+            DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(),
+                                  stream_));
+        DTB_HIP(hipEventSynchronize(e2));
+        float a = 0.0f, b = 0.0f;
+        DTB_HIP(hipEventElapsedTime(&a, e0, e1));
+        DTB_HIP(hipEventElapsedTime(&b, e1, e2));
+        tn += a;
+        tp += b;
+    }
+    DTB_HIP(hipEventDestroy(e0));
+    DTB_HIP(hipEventDestroy(e1));
+    DTB_HIP(hipEventDestroy(e2));
+    if (ms_neuron)
+        *ms_neuron = tn;
+    if (ms_prop)
+        *ms_prop = tp;
+}
+
+void Engine::sync()
+{
+    DTB_HIP(hipSetDevice(device_));
+    DTB_HIP(hipStreamSynchronize(stream_));
+}
+
+}  // namespace dtb

@@ -1,0 +1,333 @@
+// dtb_engine.hpp -- C++ host runtime: device buffers, instances and the
+// step engine that drives the gfx950 kernels.
+//
+// Vocabulary follows the reference host library:
+//   DataAllocator<T>  include/data_allocator.hpp:8-38 (device buffer + host
+//                     mirror, zero-filled, explicit copies)
+//   Weights/DataType  include/weights.hpp:14-94 (two values per synapse)
+//   Instance          one BrainBlock<float,float2> (include/block.hpp:21-889)
+//   Engine            the per-GPU step loop of snn_run
+//                     (tools/dist_simulator.cpp:2072-2315), here able to step
+//                     many instances in one launch pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dtb_types.hpp"
+
+namespace dtb {
+
+// Status codes: 0-5 are protos/snn.proto:44-51 SnnStatus.
+enum StatusCode : int {
+    kOk = 0,
+    kInvalidParameter = 1,
+    kUnsetSampleFailure = 2,
+    kUnsupported = 3,
+    kNotEnableMetric = 4,
+    kInvalidNetwork = 5,
+    kHipError = 100,
+    kOutOfMemory = 101,
+    kNoDevice = 102,
+};
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what, const char *file, int line);
+#define DTB_HIP(x) ::dtb::hip_check((x), #x, __FILE__, __LINE__)
+
+// Property-matrix columns (src/configuration.cpp:1778-2156).
+enum Col : int {
+    kColNoiseRate = 0,
+    kColBlocked = 1,
+    kColIExt = 2,
+    kColSubIdx = 3,
+    kColC = 4,
+    kColTRef = 5,
+    kColGLeak = 6,
+    kColVLeak = 7,
+    kColVTh = 8,
+    kColVReset = 9,
+    kColGU = 10,
+    kColVU = 14,
+    kColTauU = 18,
+    kNumCols = 22,
+};
+
+// include/stage.hpp:320-341 == protos/snn.proto:94-115.
+enum PropType : int {
+    EXT_STIMULI_I = 0,
+    MEMBRANE_C = 1,
+    REF_T = 2,
+    LEAKAGE_G = 3,
+    LEAKAGE_V = 4,
+    THRESHOLD_V = 5,
+    RESET_V = 6,
+    CONDUCT_G_AMPA = 7,
+    CONDUCT_G_NMDA = 8,
+    CONDUCT_G_GABAa = 9,
+    CONDUCT_G_GABAb = 10,
+    MEMBRANE_V_AMPA = 11,
+    MEMBRANE_V_NMDA = 12,
+    MEMBRANE_V_GABAa = 13,
+    MEMBRANE_V_GABAb = 14,
+    TAO_AMPA = 15,
+    TAO_NMDA = 16,
+    TAO_GABAa = 17,
+    TAO_GABAb = 18,
+    NOISE_RATE = 19,
+    kNumPropTypes = 20,
+};
+
+int prop_type_column(int prop);   // PropType -> property column, -1 if unknown
+
+// include/weights.hpp:14-21
+enum DataType : int { DOUBLE = 0, FLOAT = 1, FLOAT16 = 2, INT32 = 3, INT8 = 4 };
+size_t data_size(DataType t);
+
+// Device buffer with a lazily created, zero-filled host mirror.
+template <typename T>
+class DataAllocator {
+public:
+    explicit DataAllocator(size_t count = 0) { resize(count); }
+    ~DataAllocator() { release(); }
+    DataAllocator(const DataAllocator &) = delete;
+    DataAllocator &operator=(const DataAllocator &) = delete;
+
+    void resize(size_t count)
+    {
+        if (count == count_)
+            return;
+        release();
+        count_ = count;
+    }
+    size_t count() const { return count_; }
+    size_t size() const { return count_ * sizeof(T); }
+
+    const T *gpu_data() { return mutable_gpu_data(); }
+    T *mutable_gpu_data()
+    {
+        if (gpu_ == nullptr && count_ > 0) {
+            DTB_HIP(hipMalloc(reinterpret_cast<void **>(&gpu_), size()));
+            DTB_HIP(hipMemset(gpu_, 0, size()));
+        }
+        return gpu_;
+    }
+    const T *cpu_data() { return mutable_cpu_data(); }
+    T *mutable_cpu_data()
+    {
+        if (cpu_.size() != count_)
+            cpu_.assign(count_, T{});
+        return cpu_.data();
+    }
+    void to_gpu(hipStream_t s)
+    {
+        if (count_ == 0)
+            return;
+        DTB_HIP(hipMemcpyAsync(mutable_gpu_data(), mutable_cpu_data(), size(), hipMemcpyHostToDevice, s));
+    }
+    void to_cpu(hipStream_t s)
+    {
+        if (count_ == 0)
+            return;
+        DTB_HIP(hipMemcpyAsync(mutable_cpu_data(), mutable_gpu_data(), size(), hipMemcpyDeviceToHost, s));
+    }
+    void upload(const T *src, size_t n, hipStream_t s)
+    {
+        if (n > count_)
+            throw Error(kInvalidParameter, "upload larger than buffer");
+        if (n == 0)
+            return;
+        DTB_HIP(hipMemcpyAsync(mutable_gpu_data(), src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T *dst, size_t n, hipStream_t s)
+    {
+        if (n > count_)
+            throw Error(kInvalidParameter, "download larger than buffer");
+        if (n == 0)
+            return;
+        DTB_HIP(hipMemcpyAsync(dst, mutable_gpu_data(), n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    void release_cpu() { std::vector<T>().swap(cpu_); }
+
+private:
+    void release()
+    {
+        if (gpu_ != nullptr)
+            (void)hipFree(gpu_);
+        gpu_ = nullptr;
+        std::vector<T>().swap(cpu_);
+        count_ = 0;
+    }
+    size_t count_ = 0;
+    T *gpu_ = nullptr;
+    std::vector<T> cpu_;
+};
+
+// Host description of a network in the reference layout: property matrix
+// [n][22] and a presynaptic-major CSR whose synapses carry a weight pair and a
+// channel bit (0: AMPA/NMDA pair, 1: GABA_A/GABA_B pair), bit-packed in
+// 32-bit words as in presynaptic_voltage.cpp:152.
+struct NetworkHost {
+    int64_t n = 0;
+    const float *props = nullptr;
+    int64_t nsyn = 0;
+    const int64_t *rowptr = nullptr;
+    const uint32_t *col = nullptr;
+    const float *w = nullptr;           // [nsyn][2]
+    const uint32_t *kind_bits = nullptr;  // [ceil(nsyn/32)]
+};
+
+// Per-run options (protos/snn.proto:63-77 RunRequest).
+struct RunOptions {
+    int32_t t_steps = 1;
+    bool use_ou = true;
+    bool record_freq = false;
+    bool record_samples = false;
+};
+
+class Engine;
+
+// One simulated block.
+class Instance {
+public:
+    Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t stream);
+    ~Instance();
+
+    uint32_t n() const { return n_; }
+    int64_t nsyn() const { return nsyn_; }
+    uint32_t npop() const { return (uint32_t)pop_ids_.size(); }
+    const std::vector<uint32_t> &pop_ids() const { return pop_ids_; }
+    const std::vector<uint32_t> &pop_begin() const { return pop_begin_; }
+    float dt() const { return dt_; }
+    uint64_t seed() const { return seed_; }
+    int32_t t_steps() const { return t_steps_; }
+    bool use_ou() const { return use_ou_; }
+
+    // extern_stimuli.cpp / block.cpp:222-281 (set_I_ou_current_param).
+    void set_ou(const uint32_t *brain_ids, const float *mean, const float *sigma, const float *tau, size_t n,
+                hipStream_t s);
+    // property.cpp:57-466 (update_props / assign_prop_cols / update_prop_cols).
+    void update_props(const uint32_t *neuron_ids, const uint32_t *prop_ids, const float *vals, size_t n,
+                      hipStream_t s);
+    void update_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_ids, const float *vals, size_t n,
+                          bool assign, hipStream_t s);
+    void set_samples(const uint32_t *ids, size_t n);
+    const std::vector<uint32_t> &samples() const { return samples_; }
+
+    // Configure t_steps / OU mode for the next run (recomputes substep decays).
+    bool configure_run(int32_t t_steps, bool use_ou, hipStream_t s);
+    // Recording window of `steps` rows starting at the current step.
+    void begin_record(int64_t steps, bool freq, bool samples, hipStream_t s);
+    void end_record();
+    void fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, hipStream_t s);
+
+    // Reference-visible state after the last completed step: V, refractory
+    // counter, spike flag, emitted flag, i_ou, J[4] (pending delta folded
+    // in), I_syn (without i_ou).
+    void read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, float *iou, float *J, float *isyn,
+                    hipStream_t s);
+    void write_state(const float *V, const int32_t *t, const uint8_t *f, const float *iou, const float *J,
+                     hipStream_t s);
+    int64_t step(hipStream_t s);
+    void set_step(int64_t step, hipStream_t s);
+    void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
+
+    InstDev dev();
+    SampleRecDev sample_rec();
+
+private:
+    void derive_params(hipStream_t s);
+    void init_state(hipStream_t s);
+    uint32_t find_pop(uint32_t brain_id) const;
+
+    uint32_t n_ = 0;
+    int64_t nsyn_ = 0;
+    float dt_ = 0.1f;
+    uint64_t seed_ = 0;
+    int32_t t_steps_ = 1;
+    bool use_ou_ = true;
+
+    std::vector<float> props_;           // host property matrix [n][22]
+    std::vector<uint32_t> pop_ids_;      // brain id per population
+    std::vector<uint32_t> pop_begin_;    // [npop+1]
+    std::vector<uint8_t> ou_on_;
+    std::vector<float> ou_mean_, ou_sigma_, ou_tau_;
+    std::vector<uint32_t> samples_;
+
+    DataAllocator<float4> pA_, pB_, gU_, vU_, dec_, J_;
+    DataAllocator<uint8_t> blocked_, flag_;
+    DataAllocator<float> V_, iou_;
+    DataAllocator<int32_t> tcnt_;
+    DataAllocator<long long> dq_;
+    DataAllocator<int64_t> rowptr_;
+    DataAllocator<uint32_t> col_;
+    DataAllocator<float2> w_;
+    DataAllocator<PopDev> pops_;
+    DataAllocator<int64_t> step_;
+    DataAllocator<unsigned long long> counters_;   // events, spikes
+    DataAllocator<uint32_t> rec_counts_;
+    DataAllocator<uint8_t> rec_spike_;
+    DataAllocator<float> rec_v_, rec_isyn_, rec_iou_;
+    int64_t rec_step0_ = 0;
+    int64_t rec_len_ = 0;
+    bool rec_freq_ = false, rec_samples_ = false;
+};
+
+// Steps a set of instances together.  One neuron-step launch covers every
+// neuron of every active instance; one propagation launch covers every fired
+// row.  The step loop can be captured into a HIP graph.
+class Engine {
+public:
+    explicit Engine(int device);
+    ~Engine();
+
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+
+    int add(std::unique_ptr<Instance> inst);
+    std::unique_ptr<Instance> remove(int id);
+    Instance *get(int id);
+    size_t active() const;
+
+    // Runs `steps` steps of every active instance.  graph_chunk > 0 captures
+    // that many steps into one HIP graph and replays it.
+    void run(int64_t steps, int graph_chunk);
+    void set_prop_blocks(uint32_t nb) { prop_blocks_ = nb; }
+    uint32_t prop_blocks() const { return prop_blocks_; }
+    void sync();
+    void mark_dirty() { dirty_ = true; }
+
+    // Kernel-time measurement: events recorded around the step kernels.
+    void run_timed(int64_t steps, float *ms_neuron, float *ms_prop);
+
+private:
+    void rebuild();
+    void launch_step();
+    void drop_graph();
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::vector<std::unique_ptr<Instance>> insts_;
+    bool dirty_ = true;
+    uint32_t nwg_ = 0, ninst_ = 0, fired_cap_ = 0, nsamples_ = 0;
+    uint32_t prop_blocks_ = 0;
+    DataAllocator<InstDev> d_insts_;
+    DataAllocator<WgDesc> d_wgs_;
+    DataAllocator<FiredDev> d_fired_;
+    DataAllocator<StepCtl> d_ctl_;
+    DataAllocator<SampleDev> d_samples_;
+    DataAllocator<SampleRecDev> d_recs_;
+    hipGraphExec_t graph_ = nullptr;
+    int graph_steps_ = 0;
+};
+
+}  // namespace dtb

@@ -1,0 +1,23 @@
+// dtb_gen.hpp -- seeded synthetic networks (host, multi-threaded).
+#pragma once
+
+#include <cstdint>
+
+namespace dtb {
+
+// n neurons, 80% excitatory (population sub_base) then 20% inhibitory
+// (sub_base + 1); every neuron has exactly K in-edges from uniformly drawn
+// presynaptic neurons (no self-connections).  Outputs: props [n][22],
+// rowptr [n+1], col [n K], w [n K][2], kind_bits [ceil(n K / 32)].
+void gen_uniform(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint32_t sub_base, float *props,
+                 int64_t *rowptr, uint32_t *col, float *w, uint32_t *kind_bits);
+
+// nvox voxels with sizes proportional to lognormal(0, 0.5), each an E/I pair
+// (80/20, populations 2v and 2v+1).  K in-edges per neuron: round(K
+// intra_frac) from the neuron's own voxel, the rest from a source voxel drawn
+// from a sparse lognormal voxel-to-voxel matrix of the given density.
+void gen_voxel(int64_t n, uint32_t nvox, uint32_t K, float intra_frac, float density, uint64_t seed,
+               const float g_ui[4], float *props, int64_t *rowptr, uint32_t *col, float *w, uint32_t *kind_bits,
+               uint32_t *vox_sizes);
+
+}  // namespace dtb

@@ -1,0 +1,116 @@
+// dtb_types.hpp -- device-visible data layout of a simulation instance.
+//
+// One instance is one reference "block" (BrainBlock<float,float2>,
+// include/block.hpp:782-885).  Its per-neuron state and parameters are
+// structure-of-arrays in HBM; parameters that the step reads together are
+// packed into float4 records so that one 64-lane wave issues 16-byte loads
+// over a contiguous 1 KiB.  Many instances can be stepped by one launch: the
+// kernels find their instance through a small descriptor table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dtb {
+
+constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
+constexpr int kWave = 64;
+constexpr uint32_t kKindBit = 0x80000000u;   // channel-pair bit packed into the column index
+constexpr uint32_t kColMask = 0x7fffffffu;
+
+// Flag byte per neuron: bit 0 = membrane spike flag f (reference
+// f_inner_actives_), bit 1 = spike emitted to the synapses in the last step
+// (f_inner_actives_ or f_recorded_inner_actives_, dist_simulator.cpp:2118-2131).
+constexpr uint8_t kFlagSpike = 1u;
+constexpr uint8_t kFlagEmit = 2u;
+
+// Per-population record (reference sub-block, sub_bcounts_ ranges and the
+// OU_Background_Current_Param map, include/stage.hpp:14-19).
+struct alignas(16) PopDev {
+    uint32_t begin;     // first neuron (instance-local)
+    uint32_t end;       // one past the last neuron
+    float mu;           // OU mean
+    float f1;           // 1 - exp(-dt/tau)
+    float f2s;          // sqrt(1 - exp(-2 dt/tau)) * sigma
+    uint32_t ou_on;     // OU configured for this population
+    uint32_t pad0, pad1;
+};
+
+// Fired-list entry written by the neuron step and consumed by propagation.
+struct alignas(16) FiredDev {
+    uint32_t inst;      // instance index
+    uint32_t len;       // out-degree of the fired row
+    int64_t row_begin;  // first synapse of the row
+};
+
+// Everything a kernel needs about one instance.
+struct alignas(16) InstDev {
+    // parameters, float4 per neuron:
+    //   pA = (v_reset, v_th, 1/C, v_leak)
+    //   pB = (g_leak, i_ext, noise_rate, bits(refractory substeps))
+    //   gU, vU, dec = (AMPA, NMDA, GABA_A, GABA_B)
+    const float4 *pA, *pB, *gU, *vU, *dec;
+    const uint8_t *blocked;     // excluded from population statistics
+    // state
+    float *V;
+    int32_t *tcnt;              // refractory counter (reference t_actives_)
+    uint8_t *flag;              // kFlagSpike | kFlagEmit
+    float *iou;                 // OU current (reference i_ou_background_stimuli_)
+    float4 *J;                  // J_u after the previous step's decay (pending delta below)
+    long long *dq;              // [n][4] Q23.40 delta accumulators
+    // connectivity: presynaptic-major CSR, 64-bit row offsets
+    const int64_t *rowptr;      // [n+1]
+    const uint32_t *col;        // [S] postsynaptic id | kKindBit
+    const float2 *w;            // [S] weight pair
+    const PopDev *pops;         // [npop]
+    // recording (device buffers, row = step - rec_step0)
+    uint32_t *rec_counts;       // [rec_len][npop] spikes per population
+    // counters
+    unsigned long long *events; // synaptic events delivered (sum of fired out-degrees)
+    unsigned long long *spikes; // spikes emitted
+    int64_t *step;              // steps completed (device side)
+    // scalars
+    uint64_t seed;
+    int64_t rec_step0;
+    uint32_t n;
+    uint32_t npop;
+    uint32_t rec_len;
+    int32_t t_steps;
+    int32_t use_ou;
+    float dt_sub;
+};
+
+// One workgroup of the neuron step: 256 consecutive neurons of one instance.
+struct alignas(16) WgDesc {
+    uint32_t inst;
+    uint32_t begin;     // first instance-local neuron
+    uint32_t count;     // neurons in this workgroup (<= 256)
+    uint32_t pop0;      // population of the first neuron
+};
+
+// Sample request (reference stat_samples_kernel, stat_result.cpp:393-433).
+struct alignas(16) SampleDev {
+    uint32_t inst;
+    uint32_t neuron;
+    uint32_t slot;      // column in the instance's sample record
+    uint32_t pad;
+};
+
+// Per-instance sample records, row = step - rec_step0.
+struct alignas(16) SampleRecDev {
+    uint8_t *spike;
+    float *v;
+    float *isyn;
+    float *iou;
+    uint32_t nsample;
+    uint32_t pad[3];
+};
+
+// Shared per-launch control block in device memory.
+struct alignas(16) StepCtl {
+    uint32_t fired_count[2];    // entries in fired list [parity]
+    uint32_t done_blocks;       // last-block detection in propagation
+    uint32_t tick;              // launches completed (parity source)
+};
+
+}  // namespace dtb

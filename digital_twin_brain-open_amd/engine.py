@@ -1,0 +1,342 @@
+"""Python host mirror of the reference block API over the C ABI (libdtb.so).
+
+``Block`` keeps the method vocabulary of the reference client
+(``python/dist_blockwrapper.py:31-766``: ``run``, ``update_property``,
+``mul_property_by_subblk``, ``assign_property_by_subblk``, ``set_samples``,
+``update_ou_background_stimuli``) for one block resident on one GPU, calling
+the native engine in-process instead of the gRPC/MPI service.
+``Engine`` exposes the multi-instance engine (batched stepping).
+"""
+import ctypes
+from ctypes import byref, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+
+import numpy as np
+
+from . import _lib
+
+NCOL = 22
+# dist_blockwrapper.py:32-33: property column -> PropType (-1: not updatable).
+COLUMN_TO_PROP = np.array([19, -1, 0, -1, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18],
+                          dtype=np.int32)
+DEFAULT_G_UI = (0.01, 0.0005, 0.1, 0.005)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+class NetworkArrays:
+    """A block in the reference layout (src/configuration.cpp:1079-1667):
+    property matrix [n][22] float32, presynaptic-major CSR (int64 rowptr,
+    uint32 postsynaptic ids), weight pair per synapse, channel bits packed
+    32 per uint32 word."""
+
+    def __init__(self, props, rowptr, col, w, kinds):
+        self.props = np.ascontiguousarray(props, dtype=np.float32)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1, 2)
+        self.kinds = np.ascontiguousarray(kinds, dtype=np.uint32)
+        assert self.props.ndim == 2 and self.props.shape[1] == NCOL
+        assert self.rowptr.shape[0] == self.n + 1
+        assert self.col.shape[0] == self.nsyn and self.w.shape[0] == self.nsyn
+        assert self.kinds.shape[0] >= (self.nsyn + 31) // 32
+
+    @property
+    def n(self):
+        return int(self.props.shape[0])
+
+    @property
+    def nsyn(self):
+        return int(self.col.shape[0])
+
+    def kind_per_synapse(self):
+        """Channel bit of every synapse as uint8 (0: AMPA/NMDA, 1: GABA_A/GABA_B)."""
+        bits = np.unpackbits(self.kinds.view(np.uint8), bitorder="little")
+        return bits[: self.nsyn].astype(np.uint8)
+
+    def populations(self):
+        """(brain ids, begin offsets [npop+1]) from property column 3."""
+        sub = self.props[:, 3].astype(np.int64)
+        starts = np.flatnonzero(np.r_[True, sub[1:] != sub[:-1]])
+        return sub[starts].astype(np.uint32), np.r_[starts, self.n].astype(np.int64)
+
+    def struct(self):
+        return _lib.Network(self.n, _ptr(self.props), self.nsyn, _ptr(self.rowptr), _ptr(self.col),
+                            _ptr(self.w), _ptr(self.kinds))
+
+    @staticmethod
+    def from_edges(props, pre, post, w, kind):
+        """Builds the CSR from an edge list (presynaptic id, postsynaptic id,
+        weight pair, channel bit); synapses stay in input order within a row."""
+        props = np.asarray(props, dtype=np.float32)
+        n = props.shape[0]
+        pre = np.asarray(pre, dtype=np.int64)
+        order = np.argsort(pre, kind="stable")
+        counts = np.bincount(pre, minlength=n)
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(counts, out=rowptr[1:])
+        kind_s = np.asarray(kind, dtype=np.uint8)[order]
+        nw = (len(pre) + 31) // 32
+        padded = np.zeros(nw * 32, dtype=np.uint8)
+        padded[: len(pre)] = kind_s
+        kinds = np.packbits(padded, bitorder="little").view(np.uint32)
+        return NetworkArrays(props, rowptr, np.asarray(post, dtype=np.uint32)[order],
+                             np.asarray(w, dtype=np.float32).reshape(-1, 2)[order], kinds)
+
+
+def gen_uniform(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
+    """n neurons (80% E, 20% I), exactly k uniformly drawn in-edges each."""
+    lib = _lib.load()
+    nsyn = int(n) * int(k)
+    props = np.empty((n, NCOL), dtype=np.float32)
+    rowptr = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(nsyn, dtype=np.uint32)
+    w = np.empty((nsyn, 2), dtype=np.float32)
+    kinds = np.empty((nsyn + 31) // 32, dtype=np.uint32)
+    g = np.asarray(g_ui, dtype=np.float32)
+    _lib.check(lib.dtb_gen_uniform(n, k, seed, _ptr(g), sub_base, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w),
+                                   _ptr(kinds)))
+    return NetworkArrays(props, rowptr, col, w, kinds)
+
+
+def gen_voxel(n, nvox, k, seed, intra_frac=0.7, density=0.05, g_ui=DEFAULT_G_UI):
+    """nvox lognormal-sized E/I voxels, k in-edges per neuron (intra_frac
+    within the voxel, the rest from a sparse lognormal voxel matrix)."""
+    lib = _lib.load()
+    nsyn = int(n) * int(k)
+    props = np.empty((n, NCOL), dtype=np.float32)
+    rowptr = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(nsyn, dtype=np.uint32)
+    w = np.empty((nsyn, 2), dtype=np.float32)
+    kinds = np.empty((nsyn + 31) // 32, dtype=np.uint32)
+    sizes = np.empty(nvox, dtype=np.uint32)
+    g = np.asarray(g_ui, dtype=np.float32)
+    _lib.check(lib.dtb_gen_voxel(n, nvox, k, intra_frac, density, seed, _ptr(g), _ptr(props), _ptr(rowptr),
+                                 _ptr(col), _ptr(w), _ptr(kinds), _ptr(sizes)))
+    net = NetworkArrays(props, rowptr, col, w, kinds)
+    net.voxel_sizes = sizes
+    return net
+
+
+def device_count():
+    c = c_int(0)
+    _lib.check(_lib.load().dtb_device_count(byref(c)))
+    return c.value
+
+
+def ou_normals(seed, neuron0, substep, count, device=0):
+    out = np.empty(count, dtype=np.float32)
+    _lib.check(_lib.load().dtb_ou_normals(device, seed, neuron0, substep, count, _ptr(out)))
+    return out
+
+
+class Engine:
+    """One GPU, one HIP stream, any number of resident instances."""
+
+    def __init__(self, device=0):
+        self._lib = _lib.load()
+        h = c_void_p()
+        _lib.check(self._lib.dtb_engine_create(device, byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            _lib.check(self._lib.dtb_engine_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def add(self, net, dt=0.1, seed=0):
+        s = net.struct()
+        iid = c_int(-1)
+        _lib.check(self._lib.dtb_instance_add(self._h, byref(s), dt, seed, byref(iid)))
+        return iid.value
+
+    def remove(self, inst):
+        _lib.check(self._lib.dtb_instance_remove(self._h, inst))
+
+    def configure(self, inst, t_steps=1, use_ou=True):
+        _lib.check(self._lib.dtb_configure(self._h, inst, t_steps, int(bool(use_ou))))
+
+    def run(self, steps, graph_chunk=0):
+        _lib.check(self._lib.dtb_engine_run(self._h, steps, graph_chunk))
+
+    def sync(self):
+        _lib.check(self._lib.dtb_engine_sync(self._h))
+
+    def set_prop_blocks(self, nblocks):
+        _lib.check(self._lib.dtb_engine_set_prop_blocks(self._h, nblocks))
+
+    def time_kernels(self, steps):
+        a, b = c_float(0), c_float(0)
+        _lib.check(self._lib.dtb_engine_time_kernels(self._h, steps, byref(a), byref(b)))
+        return a.value, b.value
+
+    def set_step(self, inst, step):
+        _lib.check(self._lib.dtb_set_step(self._h, inst, step))
+
+    def counters(self, inst):
+        st, ev, sp = c_int64(0), c_uint64(0), c_uint64(0)
+        _lib.check(self._lib.dtb_counters(self._h, inst, byref(st), byref(ev), byref(sp)))
+        return st.value, ev.value, sp.value
+
+    def info(self, inst):
+        n, ns, npop = c_int64(0), c_int64(0), c_int32(0)
+        _lib.check(self._lib.dtb_instance_info(self._h, inst, byref(n), byref(ns), byref(npop), None, None))
+        ids = np.empty(npop.value, dtype=np.uint32)
+        beg = np.empty(npop.value + 1, dtype=np.uint32)
+        _lib.check(self._lib.dtb_instance_info(self._h, inst, None, None, None, _ptr(ids), _ptr(beg)))
+        return {"n": n.value, "nsyn": ns.value, "pop_ids": ids, "pop_begin": beg}
+
+    def set_ou(self, inst, brain_ids, mean, deviation, correlation_time):
+        ids = np.ascontiguousarray(brain_ids, dtype=np.uint32)
+        m = np.broadcast_to(np.asarray(mean, dtype=np.float32), ids.shape).copy()
+        d = np.broadcast_to(np.asarray(deviation, dtype=np.float32), ids.shape).copy()
+        t = np.broadcast_to(np.asarray(correlation_time, dtype=np.float32), ids.shape).copy()
+        _lib.check(self._lib.dtb_set_ou(self._h, inst, _ptr(ids), _ptr(m), _ptr(d), _ptr(t), ids.size))
+
+    def set_samples(self, inst, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        _lib.check(self._lib.dtb_set_samples(self._h, inst, _ptr(ids), ids.size))
+
+    def update_props(self, inst, neuron_ids, prop_ids, vals):
+        nid = np.ascontiguousarray(neuron_ids, dtype=np.uint32)
+        pid = np.broadcast_to(np.asarray(prop_ids, dtype=np.uint32), nid.shape).copy()
+        v = np.broadcast_to(np.asarray(vals, dtype=np.float32), nid.shape).copy()
+        _lib.check(self._lib.dtb_update_props(self._h, inst, _ptr(nid), _ptr(pid), _ptr(v), nid.size))
+
+    def update_prop_cols(self, inst, prop_ids, brain_ids, vals, assign):
+        bid = np.ascontiguousarray(brain_ids, dtype=np.uint32)
+        pid = np.broadcast_to(np.asarray(prop_ids, dtype=np.uint32), bid.shape).copy()
+        v = np.broadcast_to(np.asarray(vals, dtype=np.float32), bid.shape).copy()
+        _lib.check(self._lib.dtb_update_prop_cols(self._h, inst, _ptr(pid), _ptr(bid), _ptr(v), bid.size,
+                                                  int(bool(assign))))
+
+    def run_block(self, inst, iters, t_steps=1, use_ou=True, freqs=True, sample_spike=False, sample_vmemb=False,
+                  sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0):
+        flags = (_lib.RUN_USE_OU if use_ou else 0) | (_lib.RUN_GRAPH if graph else 0)
+        out = {}
+        o = _lib.RunOutputs()
+        if freqs:
+            flags |= _lib.RUN_FREQ
+            out["freq"] = np.zeros((iters, npop), dtype=np.uint32)
+            o.freq = _ptr(out["freq"])
+        for key, flag, dt, field in (("spike", _lib.RUN_SAMPLE_SPIKE, np.uint8, "sample_spike"),
+                                     ("vmemb", _lib.RUN_SAMPLE_VMEMB, np.float32, "sample_vmemb"),
+                                     ("isynaptic", _lib.RUN_SAMPLE_ISYN, np.float32, "sample_isynaptic"),
+                                     ("iou", _lib.RUN_SAMPLE_IOU, np.float32, "sample_iou")):
+            if {"spike": sample_spike, "vmemb": sample_vmemb, "isynaptic": sample_isynaptic,
+                    "iou": sample_iou}[key]:
+                flags |= flag
+                out[key] = np.zeros((iters, nsample), dtype=dt)
+                setattr(o, field, _ptr(out[key]))
+        _lib.check(self._lib.dtb_run(self._h, inst, iters, t_steps, flags, byref(o)))
+        return out
+
+    def read_state(self, inst, n):
+        st = {
+            "V": np.empty(n, dtype=np.float32),
+            "t": np.empty(n, dtype=np.int32),
+            "f": np.empty(n, dtype=np.uint8),
+            "emitted": np.empty(n, dtype=np.uint8),
+            "i_ou": np.empty(n, dtype=np.float32),
+            "J": np.empty((n, 4), dtype=np.float32),
+            "I_syn": np.empty(n, dtype=np.float32),
+        }
+        _lib.check(self._lib.dtb_read_state(self._h, inst, _ptr(st["V"]), _ptr(st["t"]), _ptr(st["f"]),
+                                            _ptr(st["emitted"]), _ptr(st["i_ou"]), _ptr(st["J"]),
+                                            _ptr(st["I_syn"])))
+        return st
+
+    def write_state(self, inst, V=None, t=None, f=None, i_ou=None, J=None):
+        conv = lambda a, dt: None if a is None else np.ascontiguousarray(a, dtype=dt)
+        V, t, f, i_ou, J = (conv(V, np.float32), conv(t, np.int32), conv(f, np.uint8), conv(i_ou, np.float32),
+                            conv(J, np.float32))
+        _lib.check(self._lib.dtb_write_state(self._h, inst, _ptr(V), _ptr(t), _ptr(f), _ptr(i_ou), _ptr(J)))
+
+
+class Block:
+    """One block on one GPU with the reference client vocabulary."""
+
+    def __init__(self, net, delta_t=0.1, seed=0, device=0, engine=None):
+        self.engine = engine if engine is not None else Engine(device)
+        self.inst = self.engine.add(net, delta_t, seed)
+        self.delta_t = delta_t
+        info = self.engine.info(self.inst)
+        self._n = info["n"]
+        self.pop_ids = info["pop_ids"]
+        self.pop_begin = info["pop_begin"]
+        self._nsample = 0
+
+    # dist_blockwrapper.py:128-151
+    @property
+    def total_neurons(self):
+        return self._n
+
+    @property
+    def subblk_id(self):
+        return self.pop_ids
+
+    @property
+    def neurons_per_subblk(self):
+        return np.diff(self.pop_begin.astype(np.int64))
+
+    def update_ou_background_stimuli(self, population_id, correlation_time, mean, deviation):
+        """dist_blockwrapper.py:583-612 (argument order kept)."""
+        self.engine.set_ou(self.inst, population_id, mean, deviation, correlation_time)
+
+    def set_samples(self, sample_idx):
+        """dist_blockwrapper.py:556-581."""
+        sample_idx = np.asarray(sample_idx, dtype=np.uint32)
+        self.engine.set_samples(self.inst, sample_idx)
+        self._nsample = int(sample_idx.size)
+
+    def update_property(self, neuron_idx, property_idx, property_weight):
+        """Per-neuron set (Snn.Updateprop).  property_idx are PropType ids."""
+        self.engine.update_props(self.inst, neuron_idx, property_idx, property_weight)
+
+    def mul_property_by_subblk(self, population_id, property_idx, hyper_parameter):
+        """dist_blockwrapper.py:502-515, multiplies per population."""
+        self.engine.update_prop_cols(self.inst, property_idx, population_id, hyper_parameter, assign=False)
+
+    def assign_property_by_subblk(self, population_id, property_idx, hyper_parameter):
+        """dist_blockwrapper.py:517-527, assigns per population."""
+        self.engine.update_prop_cols(self.inst, property_idx, population_id, hyper_parameter, assign=True)
+
+    def run(self, iterations, freqs=True, output_sample_spike=False, output_sample_vmemb=False,
+            output_sample_iou=False, output_sample_isynaptic=False, iou=True, t_steps=1, graph=False):
+        """dist_blockwrapper.py:179-350; returns a dict of per-step arrays."""
+        return self.engine.run_block(self.inst, iterations, t_steps=t_steps, use_ou=iou, freqs=freqs,
+                                     sample_spike=output_sample_spike, sample_vmemb=output_sample_vmemb,
+                                     sample_isynaptic=output_sample_isynaptic, sample_iou=output_sample_iou,
+                                     graph=graph, nsample=self._nsample, npop=len(self.pop_ids))
+
+    def state(self):
+        st = self.engine.read_state(self.inst, self._n)
+        st["step"] = self.engine.counters(self.inst)[0]
+        return st
+
+    def load_state(self, st):
+        """Restores a checkpoint taken with state() (state and step index)."""
+        self.engine.write_state(self.inst, st.get("V"), st.get("t"), st.get("f"), st.get("i_ou"), st.get("J"))
+        if "step" in st:
+            self.engine.set_step(self.inst, int(st["step"]))
+
+    def counters(self):
+        return self.engine.counters(self.inst)

@@ -1,0 +1,187 @@
+/*
+ * dtb_abi.h -- C ABI of the MI355X (gfx950) DTB single-block step library
+ * (libdtb.so).
+ *
+ * Plain C types only: opaque handles, pointers and sizes.  Every function
+ * returns an int status: 0..5 are the reference's SnnStatus values
+ * (protos/snn.proto:44-51), DTB_HIP_ERROR and above are runtime failures.
+ * No exception crosses the ABI; dtb_last_error() returns the message of the
+ * calling thread's last failure.  One driver thread per engine handle.
+ *
+ * Citations name the reference interface each entry point replaces, relative
+ * to the reference repository (DTB-consortium/Digital_twin_brain-open).
+ */
+#ifndef DTB_ABI_H
+#define DTB_ABI_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTB_ABI_VERSION 1
+
+/* protos/snn.proto:44-51 SnnStatus, then runtime failures. */
+enum dtb_status {
+    DTB_OK = 0,
+    DTB_INVALID_PARAMETER = 1,
+    DTB_UNSET_SAMPLE_FAILURE = 2,
+    DTB_UNSUPPORTED = 3,
+    DTB_NOT_ENABLE_METRIC = 4,
+    DTB_INVALID_NETWORK = 5,
+    DTB_HIP_ERROR = 100,
+    DTB_OUT_OF_MEMORY = 101,
+    DTB_NO_DEVICE = 102,
+    DTB_INTERNAL = 103
+};
+
+/* Run flags (protos/snn.proto:63-77 RunRequest fields). */
+enum dtb_run_flags {
+    DTB_RUN_USE_OU = 1u << 0,          /* use_ou_background */
+    DTB_RUN_FREQ = 1u << 1,            /* output_freq (u32 counts per population) */
+    DTB_RUN_SAMPLE_SPIKE = 1u << 2,    /* output_sample_spike */
+    DTB_RUN_SAMPLE_VMEMB = 1u << 3,    /* output_sample_vmemb */
+    DTB_RUN_SAMPLE_ISYN = 1u << 4,     /* output_sample_isynaptic */
+    DTB_RUN_SAMPLE_IOU = 1u << 5,      /* output_sample_iou */
+    DTB_RUN_GRAPH = 1u << 8            /* replay the step loop from a HIP graph */
+};
+
+/* PropType, include/stage.hpp:320-341 == protos/snn.proto:94-115. */
+enum dtb_prop_type {
+    DTB_EXT_STIMULI_I = 0, DTB_MEMBRANE_C = 1, DTB_REF_T = 2, DTB_LEAKAGE_G = 3,
+    DTB_LEAKAGE_V = 4, DTB_THRESHOLD_V = 5, DTB_RESET_V = 6,
+    DTB_CONDUCT_G_AMPA = 7, DTB_CONDUCT_G_NMDA = 8, DTB_CONDUCT_G_GABAa = 9, DTB_CONDUCT_G_GABAb = 10,
+    DTB_MEMBRANE_V_AMPA = 11, DTB_MEMBRANE_V_NMDA = 12, DTB_MEMBRANE_V_GABAa = 13, DTB_MEMBRANE_V_GABAb = 14,
+    DTB_TAO_AMPA = 15, DTB_TAO_NMDA = 16, DTB_TAO_GABAa = 17, DTB_TAO_GABAb = 18,
+    DTB_NOISE_RATE = 19
+};
+
+typedef struct dtb_engine_s *dtb_engine_t;
+
+/* A network in the reference block layout (src/configuration.cpp:1079-1667,
+ * 1778-2156): property matrix [n][22] float32 (columns as python/
+ * make_block.py:441-465), presynaptic-major CSR with 64-bit row offsets,
+ * postsynaptic ids, a weight pair per synapse (include/weights.hpp:30-37,
+ * DataType FLOAT) and the channel-pair bit per synapse packed in 32-bit words
+ * (bit j of word j/32; src/stage/presynaptic_voltage.cpp:152). */
+typedef struct dtb_network {
+    int64_t n;
+    const float *props;         /* [n][22] */
+    int64_t nsyn;
+    const int64_t *rowptr;      /* [n+1] */
+    const uint32_t *colinds;    /* [nsyn] */
+    const float *weights;       /* [nsyn][2] */
+    const uint32_t *connkinds;  /* [ceil(nsyn/32)] */
+} dtb_network;
+
+/* Per-step outputs of dtb_run, caller-allocated, row-major [iters][...];
+ * NULL disables an output (RunResponse fields, protos/snn.proto:79-92). */
+typedef struct dtb_run_outputs {
+    uint32_t *freq;             /* [iters][npop] spikes per population */
+    uint8_t *sample_spike;      /* [iters][nsample] */
+    float *sample_vmemb;        /* [iters][nsample] */
+    float *sample_isynaptic;    /* [iters][nsample] */
+    float *sample_iou;          /* [iters][nsample] */
+} dtb_run_outputs;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* include/version.hpp:1-6; tools/dist_simulator.cpp:3916-3922 (GetVersion). */
+int dtb_version(void);
+/* Message of the calling thread's last failure (empty if none). */
+const char *dtb_last_error(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int dtb_device_count(int *count);
+
+/* ---- engine: one GPU, one stream, any number of instances ---------------- */
+
+/* Replaces the per-GPU worker state of tools/dist_simulator.cpp:513-606
+ * (NodeInfo) and the BrainBlock constructor, src/block.cpp:19-34. */
+int dtb_engine_create(int device, dtb_engine_t *out);
+int dtb_engine_destroy(dtb_engine_t h);
+
+/* Adds a block: BrainBlock::init_connection_table_gpu + init_config_params_gpu
+ * + init_all_stages_gpu (src/block.cpp:36-179).  dt in ms; seed keys the
+ * Philox streams (the reference seeds from /dev/urandom, src/block.cpp:129-134). */
+int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t seed, int *inst_id);
+int dtb_instance_remove(dtb_engine_t h, int inst);
+/* Neuron and population counts; pop_ids [npop] (brain ids, column 3) and
+ * pop_begin [npop+1] may be NULL (sub_bids_/sub_bcounts_, include/block.hpp). */
+int dtb_instance_info(dtb_engine_t h, int inst, int64_t *n, int64_t *nsyn, int32_t *npop, uint32_t *pop_ids,
+                      uint32_t *pop_begin);
+
+/* BrainBlock::set_I_ou_current_param, src/block.cpp:222-281
+ * (Snn.Updateoubackgroundparam, protos/snn.proto:161-167). */
+int dtb_set_ou(dtb_engine_t h, int inst, const uint32_t *brain_ids, const float *mean, const float *deviation,
+               const float *correlation_time, int32_t n);
+
+/* Snn.Updatesample, tools/dist_simulator.cpp:2703-2739. */
+int dtb_set_samples(dtb_engine_t h, int inst, const uint32_t *ids, int32_t n);
+
+/* BrainBlock::update_Props_gpu, src/block.cpp:1281-1331 ->
+ * src/stage/property.cpp:57-183 (Snn.Updateprop). */
+int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const uint32_t *prop_ids,
+                     const float *vals, int32_t n);
+
+/* BrainBlock::update_Prop_Cols_gpu (assign = 0, multiply) and
+ * assign_Prop_Cols_gpu (assign = 1), src/block.cpp:1333-1470 ->
+ * src/stage/property.cpp:185-466 (Snn.Updatehyperpara). */
+int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, const uint32_t *brain_ids,
+                         const float *vals, int32_t n, int32_t assign);
+
+/* The Run loop of tools/dist_simulator.cpp:1820-2332 (snn_run) for one
+ * instance: iters steps of t_steps substeps, per-step outputs into `out`.
+ * Every other instance of the engine advances too. */
+int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, const dtb_run_outputs *out);
+
+/* Batched stepping of every instance of the engine with its last run
+ * configuration (t_steps, OU), no per-step outputs; graph_chunk > 1 replays
+ * chunks of that many steps from a captured HIP graph. */
+int dtb_engine_run(dtb_engine_t h, int64_t steps, int32_t graph_chunk);
+/* Per-instance run configuration for dtb_engine_run. */
+int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou);
+/* Waits for all work of the engine. */
+int dtb_engine_sync(dtb_engine_t h);
+/* Propagation launch width (workgroups), 0 = default. */
+int dtb_engine_set_prop_blocks(dtb_engine_t h, uint32_t nblocks);
+/* Times `steps` steps with HIP events around each kernel (ms totals). */
+int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_neuron, float *ms_prop);
+
+/* Checkpoint: reference-visible state after the last step (v_membranes_,
+ * t_actives_, f_inner_actives_, emitted flags, i_ou_background_stimuli_,
+ * J (AMPA, NMDA, GABA_A, GABA_B) [n][4], i_synaptics_ without i_ou).
+ * Any pointer may be NULL. */
+int dtb_read_state(dtb_engine_t h, int inst, float *v, int32_t *t_ref_count, uint8_t *f, uint8_t *emitted,
+                   float *i_ou, float *j, float *i_syn);
+int dtb_write_state(dtb_engine_t h, int inst, const float *v, const int32_t *t_ref_count, const uint8_t *f,
+                    const float *i_ou, const float *j);
+/* Sets the step index (the Philox counter of the next step); with
+ * dtb_write_state this restores a checkpoint bit-exactly. */
+int dtb_set_step(dtb_engine_t h, int inst, int64_t step);
+/* Steps completed, synaptic events delivered (sum of fired out-degrees) and
+ * spikes emitted since the instance was added. */
+int dtb_counters(dtb_engine_t h, int inst, int64_t *step, uint64_t *events, uint64_t *spikes);
+
+/* ---- verification ------------------------------------------------------- */
+
+/* OU-stream normals of neurons [neuron0, neuron0+count) at one substep,
+ * computed on the device. */
+int dtb_ou_normals(int device, uint64_t seed, uint32_t neuron0, uint64_t substep, uint32_t count, float *out);
+
+/* ---- synthetic networks (host) ----------------------------------------- */
+
+/* Outputs sized by the caller: props [n][22], rowptr [n+1], colinds [n K],
+ * weights [n K][2], connkinds [ceil(n K/32)].  g_ui = 4 conductances
+ * (columns 10-13). */
+int dtb_gen_uniform(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint32_t sub_base, float *props,
+                    int64_t *rowptr, uint32_t *colinds, float *weights, uint32_t *connkinds);
+int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
+                  const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
+                  uint32_t *connkinds, uint32_t *vox_sizes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
