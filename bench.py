@@ -1,0 +1,168 @@
+"""Benchmark: synaptic events/s and realtime factor of the DTB single-block
+step on one GPU.
+
+Workload (one GPU's share of the whole-brain model): 2,000,000 LIF neurons
+in 200 voxels (lognormal sizes, 80/20 E/I each), in-degree 500 (70% within
+the voxel, 30% from a sparse lognormal voxel-to-voxel matrix at 5% density),
+4 receptor channels, fp32 state and weights, dt = 0.1 ms, OU background
+(mean 5.0, sigma 0.1, tau 4 ms, about 10 Hz).  Synthetic, seeded.
+
+A step is one launch pair (neuron step + spike propagation) over the whole
+block with the network resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DT_MS = 0.1
+N_NEURONS = 2_000_000
+N_VOXELS = 200
+IN_DEGREE = 500
+OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
+NET_SEED, SIM_SEED = 2024, 7
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes of the neuron-step kernel per neuron per step:
+# parameters 5 x float4 (80 B) read; J float4 (16 B) r+w; Q23.40 delta
+# accumulators 4 x int64 (32 B) r+w (read, then cleared); V, refractory
+# counter, i_ou 3 x 4 B r+w; flag byte r+w.
+NEURON_BYTES = 80 + 2 * 16 + 2 * 32 + 2 * 12 + 2 * 1      # = 202
+# Propagation per delivered synapse: column index (4 B) + weight pair (8 B);
+# per fired row: the fired-list entry (16 B).
+SYN_BYTES = 12
+ROW_BYTES = 16
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dtb, net, blk, steps):
+    """The CPU oracle (a C port of the reference step, OpenMP) on this host,
+    started from the GPU's state after warm-up so that it runs the same
+    dynamics; returns events/s over `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    orc = importlib.import_module("oracle")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    on = orc.OracleNet.from_network(net, dt=DT_MS, seed=SIM_SEED)
+    on.set_ou(on.pop_ids, OU_MEAN, OU_SIGMA, OU_TAU)
+    sim = orc.Sim(on, 32, nthreads=threads)
+    st = blk.state()
+    sim.load_state(st["V"], st["t"], st["f"], st["i_ou"], st["J"], st["I_syn"], st["step"])
+    deg = np.diff(net.rowptr)
+    t0 = time.perf_counter()
+    out = sim.run(steps, raster=True)
+    el = time.perf_counter() - t0
+    events = int((out["raster"].astype(np.int64) * deg[None, :]).sum())
+    return {"value": events / el, "unit": "synaptic events/s", "cores": threads, "kind": "port",
+            "sample": "%d steps of the same 2M-neuron block from the GPU state after warm-up "
+                      "(%d events, %.1f s, s/step %.4f, RTF %.1f)" % (steps, events, el, el / steps,
+                                                                     el / (steps * DT_MS * 1e-3))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=100, help="CPU oracle sample length (0 = skip)")
+    ap.add_argument("--neurons", type=int, default=N_NEURONS)
+    ap.add_argument("--prop-blocks", type=int, default=0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build steps one GPU's block; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    dtb = importlib.import_module("digital_twin_brain-open_amd")
+    if dtb.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.perf_counter()
+    net = dtb.gen_voxel(args.neurons, N_VOXELS, IN_DEGREE, seed=NET_SEED)
+    log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    eng = dtb.Engine(0)
+    if args.prop_blocks:
+        eng.set_prop_blocks(args.prop_blocks)
+    blk = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED, engine=eng)
+    blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, OU_MEAN, OU_SIGMA)
+    eng.configure(blk.inst, 1, True)
+    log("uploaded in %.1f s" % (time.perf_counter() - t0))
+
+    eng.run(args.warmup, graph_chunk=0)
+    eng.sync()
+    step0, ev0, sp0 = eng.counters(blk.inst)
+
+    # Timed region: K steps, HIP events around every kernel on the engine
+    # stream.
+    t0 = time.perf_counter()
+    ms_neuron, ms_prop = eng.time_kernels(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    step1, ev1, sp1 = eng.counters(blk.inst)
+    assert step1 - step0 == args.steps
+    events = ev1 - ev0
+    spikes = sp1 - sp0
+    value = events / wall
+    rtf = wall / (args.steps * DT_MS * 1e-3)
+    rate_hz = spikes / (net.n * args.steps * DT_MS * 1e-3)
+
+    neuron_us = ms_neuron * 1e3 / args.steps
+    prop_us = ms_prop * 1e3 / args.steps
+    neuron_bytes = NEURON_BYTES * net.n
+    prop_bytes = SYN_BYTES * events / args.steps + ROW_BYTES * spikes / args.steps
+    neuron_gbs = neuron_bytes / (neuron_us * 1e-6) / 1e9
+    prop_gbs = prop_bytes / (prop_us * 1e-6) / 1e9
+    dominant = "k_neuron_step" if neuron_us >= prop_us else "k_propagate"
+    achieved = neuron_gbs if dominant == "k_neuron_step" else prop_gbs
+    traffic_env = os.environ.get("DTB_BENCH_TRAFFIC_BYTES")
+
+    res = {
+        "metric": "synaptic events/s (2M-neuron block)",
+        "value": value,
+        "unit": "synaptic events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded voxel network, random weights k*2^-24)",
+        "config": {"workload": "one GPU's block: %d neurons, %d voxels, in-degree %d, dt %.1f ms, OU mu %.1f"
+                               % (net.n, N_VOXELS, IN_DEGREE, DT_MS, OU_MEAN),
+                   "neurons": net.n, "synapses": net.nsyn, "parallelism": "single GPU"},
+        "realtime_factor": rtf,
+        "mean_rate_hz": rate_hz,
+        "events_per_step": events / args.steps,
+        "kernel_us_per_step": {"k_neuron_step": neuron_us, "k_propagate": prop_us},
+        "kernel_gbs": {"k_neuron_step": neuron_gbs, "k_propagate": prop_gbs},
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": float(traffic_env) if traffic_env else None},
+    }
+    if args.cpu_steps > 0:
+        t0 = time.perf_counter()
+        res["cpu_baseline"] = cpu_baseline(dtb, net, blk, args.cpu_steps)
+        log("cpu baseline %.1f s" % (time.perf_counter() - t0))
+    else:
+        res["cpu_baseline"] = None
+    eng.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

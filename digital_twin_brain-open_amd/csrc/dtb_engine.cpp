@@ -238,7 +238,6 @@ void Instance::init_state(hipStream_t s)
         V[i] = (p[kColVTh] + p[kColVReset]) / 2.0f;
     }
     V_.to_gpu(s);
-    V_.release_cpu();
     float *io = iou_.mutable_cpu_data();
     std::fill(io, io + n_, 0.0f);
     iou_.to_gpu(s);
@@ -249,6 +248,7 @@ void Instance::init_state(hipStream_t s)
     DTB_HIP(hipMemsetAsync(step_.mutable_gpu_data(), 0, step_.size(), s));
     DTB_HIP(hipMemsetAsync(counters_.mutable_gpu_data(), 0, counters_.size(), s));
     DTB_HIP(hipStreamSynchronize(s));
+    V_.release_cpu();
     iou_.release_cpu();
 }
 
@@ -468,11 +468,13 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
         const float4 *vu = vU_.cpu_data();
         for (uint32_t i = 0; i < n_; ++i) {
             const long long *qq = &q[(size_t)i * 4];
-            float4 r;
-            r.x = std::fma(j[i].x, dc[i].x, q40_to_f32_host(qq[0]));
-            r.y = std::fma(j[i].y, dc[i].y, q40_to_f32_host(qq[1]));
-            r.z = std::fma(j[i].z, dc[i].z, q40_to_f32_host(qq[2]));
-            r.w = std::fma(j[i].w, dc[i].w, q40_to_f32_host(qq[3]));
+            float4 r = j[i];
+            if (!(fl[i] & kFlagFolded)) {
+                r.x = std::fma(j[i].x, dc[i].x, q40_to_f32_host(qq[0]));
+                r.y = std::fma(j[i].y, dc[i].y, q40_to_f32_host(qq[1]));
+                r.z = std::fma(j[i].z, dc[i].z, q40_to_f32_host(qq[2]));
+                r.w = std::fma(j[i].w, dc[i].w, q40_to_f32_host(qq[3]));
+            }
             if (J) {
                 J[(size_t)i * 4 + 0] = r.x;
                 J[(size_t)i * 4 + 1] = r.y;
@@ -493,21 +495,30 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
 void Instance::write_state(const float *V, const int32_t *t, const uint8_t *f, const float *iou, const float *J,
                            hipStream_t s)
 {
-    // Checkpoint restore: J is the reference-visible J (delta folded), so the
-    // pending delta is cleared.
+    // Checkpoint restore.  J is the reference-visible J of the last step, so
+    // the pending delta is cleared and the flag byte marks J as folded: the
+    // next neuron step then uses it as is.
     if (V)
         V_.upload(V, n_, s);
     if (t)
         tcnt_.upload(t, n_, s);
-    if (f) {
-        std::vector<uint8_t> fl(f, f + n_);
-        for (auto &x : fl)
-            x = x ? kFlagSpike : 0;
+    if (iou)
+        iou_.upload(iou, n_, s);
+    if (f || J) {
+        std::vector<uint8_t> fl(n_);
+        flag_.download(fl.data(), n_, s);
+        DTB_HIP(hipStreamSynchronize(s));
+        for (uint32_t i = 0; i < n_; ++i) {
+            uint8_t x = fl[i];
+            if (f)
+                x = (uint8_t)((x & ~kFlagSpike) | (f[i] ? kFlagSpike : 0));
+            if (J)
+                x = (uint8_t)(x | kFlagFolded);
+            fl[i] = x;
+        }
         flag_.upload(fl.data(), n_, s);
         DTB_HIP(hipStreamSynchronize(s));
     }
-    if (iou)
-        iou_.upload(iou, n_, s);
     if (J) {
         J_.upload(reinterpret_cast<const float4 *>(J), n_, s);
         DTB_HIP(hipMemsetAsync(dq_.mutable_gpu_data(), 0, dq_.size(), s));
@@ -735,36 +746,42 @@ void Engine::run_timed(int64_t steps, float *ms_neuron, float *ms_prop)
     DTB_HIP(hipSetDevice(device_));
     if (dirty_)
         rebuild();
-    hipEvent_t e0, e1, e2;
-    DTB_HIP(hipEventCreate(&e0));
-    DTB_HIP(hipEventCreate(&e1));
-    DTB_HIP(hipEventCreate(&e2));
-    float tn = 0.0f, tp = 0.0f;
-    for (int64_t k = 0; k < steps; ++k) {
-        DTB_HIP(hipEventRecord(e0, stream_));
-        DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
-                                   fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
-        DTB_HIP(hipEventRecord(e1, stream_));
-        DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
-                                 d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
-        DTB_HIP(hipEventRecord(e2, stream_));
-        if (nsamples_ > 0)
-            DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(),
-                                  stream_));
-        DTB_HIP(hipEventSynchronize(e2));
-        float a = 0.0f, b = 0.0f;
-        DTB_HIP(hipEventElapsedTime(&a, e0, e1));
-        DTB_HIP(hipEventElapsedTime(&b, e1, e2));
-        tn += a;
-        tp += b;
+    // Events bracket every kernel on the engine stream; they are read back
+    // once per chunk so the launch pipeline is not drained every step.
+    const int64_t chunk = 256;
+    std::vector<hipEvent_t> ev((size_t)(3 * chunk));
+    for (auto &e : ev)
+        DTB_HIP(hipEventCreate(&e));
+    double tn = 0.0, tp = 0.0;
+    for (int64_t base = 0; base < steps; base += chunk) {
+        const int64_t m = std::min<int64_t>(chunk, steps - base);
+        for (int64_t k = 0; k < m; ++k) {
+            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k)], stream_));
+            DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
+                                       fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
+            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k + 1)], stream_));
+            DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
+                                     d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
+            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k + 2)], stream_));
+            if (nsamples_ > 0)
+                DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(),
+                                      stream_));
+        }
+        DTB_HIP(hipEventSynchronize(ev[(size_t)(3 * (m - 1) + 2)]));
+        for (int64_t k = 0; k < m; ++k) {
+            float a = 0.0f, b = 0.0f;
+            DTB_HIP(hipEventElapsedTime(&a, ev[(size_t)(3 * k)], ev[(size_t)(3 * k + 1)]));
+            DTB_HIP(hipEventElapsedTime(&b, ev[(size_t)(3 * k + 1)], ev[(size_t)(3 * k + 2)]));
+            tn += a;
+            tp += b;
+        }
     }
-    DTB_HIP(hipEventDestroy(e0));
-    DTB_HIP(hipEventDestroy(e1));
-    DTB_HIP(hipEventDestroy(e2));
+    for (auto &e : ev)
+        DTB_HIP(hipEventDestroy(e));
     if (ms_neuron)
-        *ms_neuron = tn;
+        *ms_neuron = (float)tn;
     if (ms_prop)
-        *ms_prop = tp;
+        *ms_prop = (float)tp;
 }
 
 void Engine::sync()

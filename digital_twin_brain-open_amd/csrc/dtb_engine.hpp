@@ -117,7 +117,11 @@ public:
     {
         if (gpu_ == nullptr && count_ > 0) {
             DTB_HIP(hipMalloc(reinterpret_cast<void **>(&gpu_), size()));
-            DTB_HIP(hipMemset(gpu_, 0, size()));
+            // The zero fill runs on the null stream, which does not order
+            // with the engine's non-blocking stream: wait for it here so a
+            // later upload on the engine stream cannot be overwritten.
+            DTB_HIP(hipMemsetAsync(gpu_, 0, size(), nullptr));
+            DTB_HIP(hipStreamSynchronize(nullptr));
         }
         return gpu_;
     }

@@ -95,17 +95,20 @@ __global__ __launch_bounds__(kBlock) void k_neuron_step(const WgDesc *__restrict
         const float4 vu = I.vU[li];
         const float4 dc = I.dec[li];
         long long *dq = I.dq + 4 * (size_t)li;
-        float4 J = fold_delta(I.J[li], dc, dq);
+        const uint32_t fl = I.flag[li];
+        float4 J = I.J[li];
+        if (!(fl & kFlagFolded))
+            J = fold_delta(J, dc, dq);
         reinterpret_cast<int4 *>(dq)[0] = make_int4(0, 0, 0, 0);
         reinterpret_cast<int4 *>(dq)[1] = make_int4(0, 0, 0, 0);
 
         float V = I.V[li];
         int32_t t = I.tcnt[li];
-        uint32_t f = I.flag[li] & kFlagSpike;
+        uint32_t f = fl & kFlagSpike;
         float iou = I.iou[li];
         float isyn = syn_current(g, vu, J, V);
 
-        while (li >= I.pops[p].end)
+        while (p + 1 < I.npop && li >= I.pops[p].end)
             ++p;
         const PopDev pop = I.pops[p];
         const bool ou = I.use_ou && pop.ou_on;
@@ -222,6 +225,8 @@ __global__ __launch_bounds__(kBlock) void k_propagate(const InstDev *__restrict_
 
     for (uint32_t r = wave; r < F; r += nwaves) {
         const FiredDev e = list[r];
+        if (e.inst >= ninst)
+            continue;   // never written by the neuron step; kept as a bound
         const InstDev &I = insts[e.inst];
         const uint32_t *__restrict__ col = I.col + e.row_begin;
         const float2 *__restrict__ w = I.w + e.row_begin;
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ i
     if (R.v)
         R.v[o] = V;
     if (R.isyn) {
-        const float4 J = fold_delta(I.J[li], I.dec[li], I.dq + 4 * (size_t)li);
+        const float4 J = fold_delta(I.J[li], I.dec[li], I.dq + 4 * (size_t)li);   // flag bit 2 is clear here
         R.isyn[o] = syn_current(I.gU[li], I.vU[li], J, V);
     }
     if (R.iou)

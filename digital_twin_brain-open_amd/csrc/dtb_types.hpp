@@ -23,6 +23,10 @@ constexpr uint32_t kColMask = 0x7fffffffu;
 // (f_inner_actives_ or f_recorded_inner_actives_, dist_simulator.cpp:2118-2131).
 constexpr uint8_t kFlagSpike = 1u;
 constexpr uint8_t kFlagEmit = 2u;
+// bit 2: J already holds the reference-visible J of the last step (set when a
+// checkpoint is restored); the next neuron step uses it without the
+// decay-and-delta fold.
+constexpr uint8_t kFlagFolded = 4u;
 
 // Per-population record (reference sub-block, sub_bcounts_ ranges and the
 // OU_Background_Current_Param map, include/stage.hpp:14-19).

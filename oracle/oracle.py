@@ -173,6 +173,18 @@ class Sim:
             load().dtbo_init64(ctypes.byref(self._ns), ctypes.byref(s))
         self.st.step = s.step
 
+    def load_state(self, V, t, f, i_ou, J, I_syn, step):
+        """Starts from a given reference-visible state (e.g. one read back
+        from the GPU with dtb_read_state)."""
+        dt = self.st.V.dtype
+        self.st.V[:] = np.asarray(V, dtype=dt)
+        self.st.t[:] = np.asarray(t, dtype=np.int32)
+        self.st.f[:] = np.asarray(f, dtype=np.uint8)
+        self.st.i_ou[:] = np.asarray(i_ou, dtype=dt)
+        self.st.J[:] = np.asarray(J, dtype=dt).reshape(-1, 4)
+        self.st.I_syn[:] = np.asarray(I_syn, dtype=dt)
+        self.st.step = int(step)
+
     def refresh(self):
         """Re-reads network parameters (after OU or property changes)."""
         self._ns = self.net.struct()

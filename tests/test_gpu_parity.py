@@ -185,14 +185,17 @@ def test_property_updates_match_oracle(dtb, orc, gpu):
     st = blk.state()
     blk.engine.close()
 
-    props = net.props.copy()
+    # The oracle initialises its state from the original properties (as the
+    # block did at load time), then runs with the updated ones: property
+    # updates change parameters, not the state (property.cpp:57-466).
+    on = orc.OracleNet(net.props.copy(), net.rowptr, net.col, net.w, net.kind_per_synapse(), dt=DT, seed=seed)
+    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
+    sim = orc.Sim(on, 32)
+    props = on.props
     p0 = props[:, 3] == 0
     props[p0, 10] *= np.float32(1.3)
     props[props[:, 3] == 1, 8] = -51.0
     props[[5, 17, 4999], 2] = [0.3, 0.4, 0.5]
-    on = orc.OracleNet(props, net.rowptr, net.col, net.w, net.kind_per_synapse(), dt=DT, seed=seed)
-    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
-    sim = orc.Sim(on, 32)
     ref = sim.run(steps, pop_counts=True)
     assert np.array_equal(out["freq"], ref["freq"])
     assert_state_equal(st, sim)
