@@ -30,15 +30,15 @@ OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
 NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes of the neuron-step kernel per neuron per step:
-# parameters 5 x float4 (80 B) read; J float4 (16 B) r+w; Q23.40 delta
-# accumulators 4 x int64 (32 B) r+w (read, then cleared); V, refractory
-# counter, i_ou 3 x 4 B r+w; flag byte r+w.
-NEURON_BYTES = 80 + 2 * 16 + 2 * 32 + 2 * 12 + 2 * 1      # = 202
-# Propagation per delivered synapse: column index (4 B) + weight pair (8 B);
-# per fired row: the fired-list entry (16 B).
-SYN_BYTES = 12
-ROW_BYTES = 16
+# Algorithmic HBM bytes per step:
+# k_step, per neuron: parameters 5 x float4 (80 B) read; J float4 (16 B) read
+# + write; V, refractory counter, i_ou 3 x 4 B read + write; flag byte read +
+# write (= 138 B); per delivered synapse: column index + weight pair (12 B).
+STEP_NEURON_BYTES = 80 + 2 * 16 + 2 * 12 + 2 * 1
+STEP_SYN_BYTES = 12
+# k_bin: fired-list entry (16 B) per fired row, column index (4 B) per
+# delivered synapse.
+BIN_ROW_BYTES, BIN_SYN_BYTES = 16, 4
 
 
 def log(msg):
@@ -108,9 +108,10 @@ def main():
     # Timed region: K steps, HIP events around every kernel on the engine
     # stream.
     t0 = time.perf_counter()
-    ms_neuron, ms_prop = eng.time_kernels(args.steps)
+    ms_step, ms_bin = eng.time_kernels(args.steps)
     eng.sync()
     wall = time.perf_counter() - t0
+    eng.check()
     step1, ev1, sp1 = eng.counters(blk.inst)
     assert step1 - step0 == args.steps
     events = ev1 - ev0
@@ -119,14 +120,13 @@ def main():
     rtf = wall / (args.steps * DT_MS * 1e-3)
     rate_hz = spikes / (net.n * args.steps * DT_MS * 1e-3)
 
-    neuron_us = ms_neuron * 1e3 / args.steps
-    prop_us = ms_prop * 1e3 / args.steps
-    neuron_bytes = NEURON_BYTES * net.n
-    prop_bytes = SYN_BYTES * events / args.steps + ROW_BYTES * spikes / args.steps
-    neuron_gbs = neuron_bytes / (neuron_us * 1e-6) / 1e9
-    prop_gbs = prop_bytes / (prop_us * 1e-6) / 1e9
-    dominant = "k_neuron_step" if neuron_us >= prop_us else "k_propagate"
-    achieved = neuron_gbs if dominant == "k_neuron_step" else prop_gbs
+    ev_step, sp_step = events / args.steps, spikes / args.steps
+    us = {"k_step": ms_step * 1e3 / args.steps, "k_bin": ms_bin * 1e3 / args.steps}
+    algo_bytes = {"k_step": STEP_NEURON_BYTES * net.n + STEP_SYN_BYTES * ev_step,
+                  "k_bin": BIN_ROW_BYTES * sp_step + BIN_SYN_BYTES * ev_step}
+    gbs = {k: algo_bytes[k] / (us[k] * 1e-6) / 1e9 for k in us}
+    dominant = max(us, key=us.get)
+    achieved = gbs[dominant]
     traffic_env = os.environ.get("DTB_BENCH_TRAFFIC_BYTES")
 
     res = {
@@ -148,8 +148,9 @@ def main():
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
-        "kernel_us_per_step": {"k_neuron_step": neuron_us, "k_propagate": prop_us},
-        "kernel_gbs": {"k_neuron_step": neuron_gbs, "k_propagate": prop_gbs},
+        "kernel_us_per_step": us,
+        "kernel_algo_bytes_per_step": algo_bytes,
+        "kernel_gbs": gbs,
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": float(traffic_env) if traffic_env else None},

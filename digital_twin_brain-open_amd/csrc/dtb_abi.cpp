@@ -149,7 +149,8 @@ int dtb_set_ou(dtb_engine_t h, int inst, const uint32_t *brain_ids, const float 
         need(n >= 0, "negative count");
         need(n == 0 || (brain_ids && mean && deviation && correlation_time), "OU arrays are required");
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();
         e.get(inst)->set_ou(brain_ids, mean, deviation, correlation_time, (size_t)n, e.stream());
         e.mark_dirty();
     });
@@ -171,7 +172,8 @@ int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const
     return guarded([&] {
         need(n >= 0 && (n == 0 || (neuron_ids && prop_ids && vals)), "property arrays are required");
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();
         e.get(inst)->update_props(neuron_ids, prop_ids, vals, (size_t)n, e.stream());
     });
 }
@@ -182,7 +184,8 @@ int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, con
     return guarded([&] {
         need(n >= 0 && (n == 0 || (prop_ids && brain_ids && vals)), "property arrays are required");
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();
         e.get(inst)->update_prop_cols(prop_ids, brain_ids, vals, (size_t)n, assign != 0, e.stream());
     });
 }
@@ -201,7 +204,7 @@ int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t f
             throw dtb::Error(dtb::kInvalidParameter, "i_ou samples need the OU background");
         if (want_samples && I->samples().empty())
             throw dtb::Error(dtb::kUnsetSampleFailure, "samples are not set");
-        e.sync();
+        e.settle();
         I->configure_run(t_steps, use_ou, e.stream());
         const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
         const bool samples = want_samples && out &&
@@ -232,7 +235,8 @@ int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou)
 {
     return guarded([&] {
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();
         if (e.get(inst)->configure_run(t_steps, use_ou != 0, e.stream()))
             e.mark_dirty();
     });
@@ -250,18 +254,28 @@ int dtb_engine_set_prop_blocks(dtb_engine_t h, uint32_t nblocks)
         if (nblocks == 0) {
             int cus = 0;
             DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device()));
-            nblocks = (uint32_t)(cus > 0 ? cus : 1) * 8u;
+            nblocks = (uint32_t)(cus > 0 ? cus : 1);
         }
         e.set_prop_blocks(nblocks);
         e.mark_dirty();
     });
 }
 
-int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_neuron, float *ms_prop)
+int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step, float *ms_bin)
 {
     return guarded([&] {
         need(steps >= 0, "negative step count");
-        engine(h).run_timed(steps, ms_neuron, ms_prop);
+        engine(h).run_timed(steps, ms_step, ms_bin);
+    });
+}
+
+int dtb_engine_check(dtb_engine_t h)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        e.sync();
+        if (e.queue_overflow())
+            throw dtb::Error(dtb::kInvalidNetwork, "a propagation queue bound was exceeded");
     });
 }
 
@@ -270,7 +284,8 @@ int dtb_read_state(dtb_engine_t h, int inst, float *v, int32_t *t_ref_count, uin
 {
     return guarded([&] {
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();   // the pending delta of the last step is folded into J
         e.get(inst)->read_state(v, t_ref_count, f, emitted, i_ou, j, i_syn, e.stream());
     });
 }
@@ -280,8 +295,11 @@ int dtb_write_state(dtb_engine_t h, int inst, const float *v, const int32_t *t_r
 {
     return guarded([&] {
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();
         e.get(inst)->write_state(v, t_ref_count, f, i_ou, j, e.stream());
+        if (j)
+            e.clear_queues(inst);   // J already includes the delta of the restored step
     });
 }
 

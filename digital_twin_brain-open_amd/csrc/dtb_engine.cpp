@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -110,9 +111,11 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     // CSR validation (rows ascending, columns in range).
     if (net.rowptr[0] != 0 || net.rowptr[net.n] != net.nsyn)
         throw Error(kInvalidNetwork, "rowptr must start at 0 and end at nsyn");
+    if (net.nsyn >= kMaxSynapses)
+        throw Error(kInvalidNetwork, "too many synapses for one block");
     for (int64_t i = 0; i < net.n; ++i)
-        if (net.rowptr[i + 1] < net.rowptr[i] || net.rowptr[i + 1] - net.rowptr[i] > (int64_t)0xffffffffu)
-            throw Error(kInvalidNetwork, "rowptr must be non-decreasing");
+        if (net.rowptr[i + 1] < net.rowptr[i] || net.rowptr[i + 1] - net.rowptr[i] >= kMaxRowLength)
+            throw Error(kInvalidNetwork, "rowptr must be non-decreasing with rows below 2^24 synapses");
     for (int64_t j = 0; j < net.nsyn; ++j)
         if (net.col[j] >= n_)
             throw Error(kInvalidNetwork, "column index out of range");
@@ -152,7 +155,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     V_.resize(n_);
     iou_.resize(n_);
     tcnt_.resize(n_);
-    dq_.resize((size_t)n_ * 4);
+    dlt_.resize(n_);
     rowptr_.resize((size_t)n_ + 1);
     col_.resize((size_t)std::max<int64_t>(nsyn_, 1));
     w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
@@ -181,10 +184,50 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     }
     if (nsyn_ > 0)
         w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
+    count_fine_runs(net);
     DTB_HIP(hipStreamSynchronize(s));
 }
 
 Instance::~Instance() = default;
+
+void Instance::count_fine_runs(const NetworkHost &net)
+{
+    // One run = maximal sequence of consecutive synapses of a row whose
+    // targets fall in the same tile.  The binning emits exactly one queue item
+    // per run, so these counts are the exact queue capacities whatever the
+    // order of synapses within a row.
+    const uint32_t nfine = (n_ + (1u << kTileBits) - 1) >> kTileBits;
+    fine_runs_.assign(nfine, 0u);
+    unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw ? hw : 4u, 16u), nsyn_ / (1 << 20)));
+    std::vector<std::vector<uint32_t>> part(nt, std::vector<uint32_t>(nfine, 0u));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            const int64_t a = (int64_t)n_ * t / nt, b = (int64_t)n_ * (t + 1) / nt;
+            uint32_t *c = part[t].data();
+            for (int64_t r = a; r < b; ++r) {
+                uint32_t prev = UINT32_MAX;
+                for (int64_t j = net.rowptr[r]; j < net.rowptr[r + 1]; ++j) {
+                    const uint32_t tile = net.col[j] >> kTileBits;
+                    if (tile != prev)
+                        ++c[tile];
+                    prev = tile;
+                }
+            }
+        });
+    }
+    for (auto &x : th)
+        x.join();
+    for (int t = 0; t < nt; ++t)
+        for (uint32_t k = 0; k < nfine; ++k)
+            fine_runs_[k] += part[t][k];
+}
+
+uint32_t Instance::tile_runs(uint32_t tile) const
+{
+    return fine_runs_.at(tile);
+}
 
 void Instance::derive_params(hipStream_t s)
 {
@@ -244,7 +287,7 @@ void Instance::init_state(hipStream_t s)
     DTB_HIP(hipMemsetAsync(tcnt_.mutable_gpu_data(), 0, tcnt_.size(), s));
     DTB_HIP(hipMemsetAsync(flag_.mutable_gpu_data(), 0, flag_.size(), s));
     DTB_HIP(hipMemsetAsync(J_.mutable_gpu_data(), 0, J_.size(), s));
-    DTB_HIP(hipMemsetAsync(dq_.mutable_gpu_data(), 0, dq_.size(), s));
+    DTB_HIP(hipMemsetAsync(dlt_.mutable_gpu_data(), 0, dlt_.size(), s));
     DTB_HIP(hipMemsetAsync(step_.mutable_gpu_data(), 0, step_.size(), s));
     DTB_HIP(hipMemsetAsync(counters_.mutable_gpu_data(), 0, counters_.size(), s));
     DTB_HIP(hipStreamSynchronize(s));
@@ -428,11 +471,6 @@ void Instance::counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipSt
         *spikes = c[1];
 }
 
-static inline float q40_to_f32_host(long long q)
-{
-    return (float)((double)q * 0x1p-40);
-}
-
 void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, float *iou, float *J, float *isyn,
                           hipStream_t s)
 {
@@ -444,13 +482,12 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
         tcnt_.download(t, n_, s);
     if (iou)
         iou_.download(iou, n_, s);
-    std::vector<float4> j, dcv;
-    std::vector<long long> q;
+    std::vector<float4> j, dl;
     if (J || isyn) {
         j.resize(n_);
-        q.resize((size_t)n_ * 4);
+        dl.resize(n_);
         J_.download(j.data(), n_, s);
-        dq_.download(q.data(), (size_t)n_ * 4, s);
+        dlt_.download(dl.data(), n_, s);
     }
     DTB_HIP(hipStreamSynchronize(s));
     if (V)
@@ -467,13 +504,12 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
         const float4 *g = gU_.cpu_data();
         const float4 *vu = vU_.cpu_data();
         for (uint32_t i = 0; i < n_; ++i) {
-            const long long *qq = &q[(size_t)i * 4];
             float4 r = j[i];
             if (!(fl[i] & kFlagFolded)) {
-                r.x = std::fma(j[i].x, dc[i].x, q40_to_f32_host(qq[0]));
-                r.y = std::fma(j[i].y, dc[i].y, q40_to_f32_host(qq[1]));
-                r.z = std::fma(j[i].z, dc[i].z, q40_to_f32_host(qq[2]));
-                r.w = std::fma(j[i].w, dc[i].w, q40_to_f32_host(qq[3]));
+                r.x = std::fma(j[i].x, dc[i].x, dl[i].x);
+                r.y = std::fma(j[i].y, dc[i].y, dl[i].y);
+                r.z = std::fma(j[i].z, dc[i].z, dl[i].z);
+                r.w = std::fma(j[i].w, dc[i].w, dl[i].w);
             }
             if (J) {
                 J[(size_t)i * 4 + 0] = r.x;
@@ -521,12 +557,12 @@ void Instance::write_state(const float *V, const int32_t *t, const uint8_t *f, c
     }
     if (J) {
         J_.upload(reinterpret_cast<const float4 *>(J), n_, s);
-        DTB_HIP(hipMemsetAsync(dq_.mutable_gpu_data(), 0, dq_.size(), s));
+        DTB_HIP(hipMemsetAsync(dlt_.mutable_gpu_data(), 0, dlt_.size(), s));
     }
     DTB_HIP(hipStreamSynchronize(s));
 }
 
-InstDev Instance::dev()
+InstDev Instance::dev(uint32_t tile_base)
 {
     InstDev d{};
     d.pA = pA_.gpu_data();
@@ -540,7 +576,7 @@ InstDev Instance::dev()
     d.flag = flag_.mutable_gpu_data();
     d.iou = iou_.mutable_gpu_data();
     d.J = J_.mutable_gpu_data();
-    d.dq = dq_.mutable_gpu_data();
+    d.dlt = dlt_.mutable_gpu_data();
     d.rowptr = rowptr_.gpu_data();
     d.col = col_.gpu_data();
     d.w = w_.gpu_data();
@@ -557,6 +593,7 @@ InstDev Instance::dev()
     d.t_steps = t_steps_;
     d.use_ou = use_ou_ ? 1 : 0;
     d.dt_sub = dt_ / (float)t_steps_;
+    d.tile_base = tile_base;
     return d;
 }
 
@@ -574,220 +611,5 @@ SampleRecDev Instance::sample_rec()
 }
 
 // ---------------------------------------------------------------------------
-
-Engine::Engine(int device) : device_(device)
-{
-    int count = 0;
-    DTB_HIP(hipGetDeviceCount(&count));
-    if (device < 0 || device >= count)
-        throw Error(kNoDevice, "device index out of range");
-    DTB_HIP(hipSetDevice(device));
-    DTB_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    d_ctl_.resize(1);
-    d_ctl_.mutable_gpu_data();
-    int cus = 0;
-    DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    prop_blocks_ = (uint32_t)std::max(1, cus) * 8u;
-}
-
-Engine::~Engine()
-{
-    drop_graph();
-    insts_.clear();
-    if (stream_)
-        (void)hipStreamDestroy(stream_);
-}
-
-int Engine::add(std::unique_ptr<Instance> inst)
-{
-    for (size_t i = 0; i < insts_.size(); ++i)
-        if (!insts_[i]) {
-            insts_[i] = std::move(inst);
-            dirty_ = true;
-            return (int)i;
-        }
-    insts_.push_back(std::move(inst));
-    dirty_ = true;
-    return (int)insts_.size() - 1;
-}
-
-std::unique_ptr<Instance> Engine::remove(int id)
-{
-    if (id < 0 || (size_t)id >= insts_.size() || !insts_[id])
-        throw Error(kInvalidParameter, "no such instance");
-    sync();
-    dirty_ = true;
-    return std::move(insts_[id]);
-}
-
-Instance *Engine::get(int id)
-{
-    if (id < 0 || (size_t)id >= insts_.size() || !insts_[id])
-        throw Error(kInvalidParameter, "no such instance");
-    return insts_[id].get();
-}
-
-size_t Engine::active() const
-{
-    size_t k = 0;
-    for (auto &p : insts_)
-        k += p ? 1 : 0;
-    return k;
-}
-
-void Engine::drop_graph()
-{
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-        graph_steps_ = 0;
-    }
-}
-
-void Engine::rebuild()
-{
-    drop_graph();
-    std::vector<InstDev> table;
-    std::vector<WgDesc> wgs;
-    std::vector<SampleDev> samples;
-    std::vector<SampleRecDev> recs;
-    uint64_t total = 0;
-    for (auto &p : insts_) {
-        if (!p)
-            continue;
-        const uint32_t idx = (uint32_t)table.size();
-        table.push_back(p->dev());
-        recs.push_back(p->sample_rec());
-        const auto &pb = p->pop_begin();
-        uint32_t q = 0;
-        for (uint32_t b = 0; b < p->n(); b += kBlock) {
-            while (pb[q + 1] <= b)
-                ++q;
-            WgDesc d;
-            d.inst = idx;
-            d.begin = b;
-            d.count = std::min<uint32_t>(kBlock, p->n() - b);
-            d.pop0 = q;
-            wgs.push_back(d);
-        }
-        if (recs.back().nsample > 0) {
-            const auto &ids = p->samples();
-            for (uint32_t k = 0; k < ids.size(); ++k)
-                samples.push_back(SampleDev{idx, ids[k], k, 0});
-        }
-        total += p->n();
-    }
-    if (total > 0xffffffffull)
-        throw Error(kInvalidParameter, "too many neurons in one engine");
-    ninst_ = (uint32_t)table.size();
-    nwg_ = (uint32_t)wgs.size();
-    nsamples_ = (uint32_t)samples.size();
-    fired_cap_ = (uint32_t)std::max<uint64_t>(total, 1);
-    d_insts_.resize(std::max<size_t>(table.size(), 1));
-    d_wgs_.resize(std::max<size_t>(wgs.size(), 1));
-    d_samples_.resize(std::max<size_t>(samples.size(), 1));
-    d_recs_.resize(std::max<size_t>(recs.size(), 1));
-    d_fired_.resize((size_t)fired_cap_ * 2);
-    d_fired_.mutable_gpu_data();
-    d_insts_.upload(table.data(), table.size(), stream_);
-    d_wgs_.upload(wgs.data(), wgs.size(), stream_);
-    d_samples_.upload(samples.data(), samples.size(), stream_);
-    d_recs_.upload(recs.data(), recs.size(), stream_);
-    DTB_HIP(hipStreamSynchronize(stream_));
-    dirty_ = false;
-}
-
-void Engine::launch_step()
-{
-    DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
-                               fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
-    DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
-                             d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
-    if (nsamples_ > 0)
-        DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(), stream_));
-}
-
-void Engine::run(int64_t steps, int graph_chunk)
-{
-    if (steps < 0)
-        throw Error(kInvalidParameter, "negative step count");
-    DTB_HIP(hipSetDevice(device_));
-    if (dirty_)
-        rebuild();
-    if (ninst_ == 0 || steps == 0)
-        return;
-    if (graph_chunk > 1 && steps >= graph_chunk) {
-        if (!graph_ || graph_steps_ != graph_chunk) {
-            drop_graph();
-            hipGraph_t g = nullptr;
-            DTB_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < graph_chunk; ++k)
-                launch_step();
-            DTB_HIP(hipStreamEndCapture(stream_, &g));
-            DTB_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
-            DTB_HIP(hipGraphDestroy(g));
-            graph_steps_ = graph_chunk;
-        }
-        int64_t done = 0;
-        while (steps - done >= graph_chunk) {
-            DTB_HIP(hipGraphLaunch(graph_, stream_));
-            done += graph_chunk;
-        }
-        for (; done < steps; ++done)
-            launch_step();
-    } else {
-        for (int64_t k = 0; k < steps; ++k)
-            launch_step();
-    }
-}
-
-void Engine::run_timed(int64_t steps, float *ms_neuron, float *ms_prop)
-{
-    DTB_HIP(hipSetDevice(device_));
-    if (dirty_)
-        rebuild();
-    // Events bracket every kernel on the engine stream; they are read back
-    // once per chunk so the launch pipeline is not drained every step.
-    const int64_t chunk = 256;
-    std::vector<hipEvent_t> ev((size_t)(3 * chunk));
-    for (auto &e : ev)
-        DTB_HIP(hipEventCreate(&e));
-    double tn = 0.0, tp = 0.0;
-    for (int64_t base = 0; base < steps; base += chunk) {
-        const int64_t m = std::min<int64_t>(chunk, steps - base);
-        for (int64_t k = 0; k < m; ++k) {
-            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k)], stream_));
-            DTB_HIP(launch_neuron_step(d_wgs_.gpu_data(), nwg_, d_insts_.gpu_data(), d_fired_.mutable_gpu_data(),
-                                       fired_cap_, d_ctl_.mutable_gpu_data(), stream_));
-            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k + 1)], stream_));
-            DTB_HIP(launch_propagate(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_,
-                                     d_ctl_.mutable_gpu_data(), prop_blocks_, stream_));
-            DTB_HIP(hipEventRecord(ev[(size_t)(3 * k + 2)], stream_));
-            if (nsamples_ > 0)
-                DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(),
-                                      stream_));
-        }
-        DTB_HIP(hipEventSynchronize(ev[(size_t)(3 * (m - 1) + 2)]));
-        for (int64_t k = 0; k < m; ++k) {
-            float a = 0.0f, b = 0.0f;
-            DTB_HIP(hipEventElapsedTime(&a, ev[(size_t)(3 * k)], ev[(size_t)(3 * k + 1)]));
-            DTB_HIP(hipEventElapsedTime(&b, ev[(size_t)(3 * k + 1)], ev[(size_t)(3 * k + 2)]));
-            tn += a;
-            tp += b;
-        }
-    }
-    for (auto &e : ev)
-        DTB_HIP(hipEventDestroy(e));
-    if (ms_neuron)
-        *ms_neuron = (float)tn;
-    if (ms_prop)
-        *ms_prop = (float)tp;
-}
-
-void Engine::sync()
-{
-    DTB_HIP(hipSetDevice(device_));
-    DTB_HIP(hipStreamSynchronize(stream_));
-}
 
 }  // namespace dtb

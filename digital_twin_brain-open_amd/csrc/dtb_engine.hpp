@@ -245,12 +245,16 @@ public:
     void set_step(int64_t step, hipStream_t s);
     void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
 
-    InstDev dev();
+    InstDev dev(uint32_t tile_base);
     SampleRecDev sample_rec();
+    // Runs of consecutive synapses (of one row) whose targets share a tile of
+    // 2^kTileBits neurons, per tile: the exact queue bound of the binning.
+    uint32_t tile_runs(uint32_t tile) const;
 
 private:
     void derive_params(hipStream_t s);
     void init_state(hipStream_t s);
+    void count_fine_runs(const NetworkHost &net);
     uint32_t find_pop(uint32_t brain_id) const;
 
     uint32_t n_ = 0;
@@ -271,7 +275,7 @@ private:
     DataAllocator<uint8_t> blocked_, flag_;
     DataAllocator<float> V_, iou_;
     DataAllocator<int32_t> tcnt_;
-    DataAllocator<long long> dq_;
+    DataAllocator<float4> dlt_;
     DataAllocator<int64_t> rowptr_;
     DataAllocator<uint32_t> col_;
     DataAllocator<float2> w_;
@@ -284,11 +288,13 @@ private:
     int64_t rec_step0_ = 0;
     int64_t rec_len_ = 0;
     bool rec_freq_ = false, rec_samples_ = false;
+    std::vector<uint32_t> fine_runs_;
 };
 
 // Steps a set of instances together.  One neuron-step launch covers every
-// neuron of every active instance; one propagation launch covers every fired
-// row.  The step loop can be captured into a HIP graph.
+// neuron of every active instance, one binning launch every fired row and one
+// accumulation launch every target tile.  The step loop can be captured into a
+// HIP graph.
 class Engine {
 public:
     explicit Engine(int device);
@@ -307,29 +313,47 @@ public:
     void run(int64_t steps, int graph_chunk);
     void set_prop_blocks(uint32_t nb) { prop_blocks_ = nb; }
     uint32_t prop_blocks() const { return prop_blocks_; }
+    // True if a binning queue bound was ever exceeded (a bug, never expected).
+    bool queue_overflow();
     void sync();
     void mark_dirty() { dirty_ = true; }
+    // Writes the pending delta of every instance into its dlt buffer (the
+    // queues are left for the next step), then waits.
+    void materialize();
+    // Drops the pending delta of one instance (checkpoint restore).
+    void clear_queues(int id);
+    // Folds every pending delta into J with the current tables and
+    // parameters.  Must run before parameters, samples, recording or the
+    // instance set change (a table rebuild empties the queues).
+    void settle();
 
-    // Kernel-time measurement: events recorded around the step kernels.
-    void run_timed(int64_t steps, float *ms_neuron, float *ms_prop);
+    // Kernel-time measurement: events recorded around the step kernels
+    // (totals in ms over `steps` steps).
+    void run_timed(int64_t steps, float *ms_step, float *ms_bin);
 
 private:
     void rebuild();
     void launch_step();
+    void launch_step_k();
+    void launch_bin_k();
     void drop_graph();
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     std::vector<std::unique_ptr<Instance>> insts_;
+    std::vector<uint32_t> tile_base_;      // per instance slot, first tile in the engine table
     bool dirty_ = true;
-    uint32_t nwg_ = 0, ninst_ = 0, fired_cap_ = 0, nsamples_ = 0;
+    bool pending_ = false;                 // queues may hold an unfolded delta
+    uint32_t ninst_ = 0, fired_cap_ = 0, nsamples_ = 0, ntiles_ = 0;
     uint32_t prop_blocks_ = 0;
     DataAllocator<InstDev> d_insts_;
-    DataAllocator<WgDesc> d_wgs_;
     DataAllocator<FiredDev> d_fired_;
     DataAllocator<StepCtl> d_ctl_;
     DataAllocator<SampleDev> d_samples_;
     DataAllocator<SampleRecDev> d_recs_;
+    DataAllocator<TileDesc> d_tiles_;
+    DataAllocator<uint32_t> d_qcnt_;
+    DataAllocator<QItem> d_queue_;
     hipGraphExec_t graph_ = nullptr;
     int graph_steps_ = 0;
 };

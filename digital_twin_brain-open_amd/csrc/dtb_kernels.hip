@@ -10,17 +10,22 @@
 //   J = J * decay + delta (presynaptic_voltage.cpp:46-120), I_syn
 //             (synaptic_current.cpp:7-105)
 //
-// Two launches per step here:
-//   k_neuron_step  one pass over the neurons that first finishes the previous
-//                  step (J fold-in of the pending delta, I_syn), then runs
-//                  OU + LIF for every substep and appends the step's fired
-//                  rows to a compact list (wave ballot + lane prefix, one
-//                  global reservation per wave);
-//   k_propagate    walks only the fired rows of the presynaptic-major CSR and
-//                  adds their weights into exact Q23.40 fixed-point
-//                  accumulators with 64-bit integer atomics, so the result is
-//                  independent of scheduling order.
-// A sample kernel gathers per-step sample records when samples are set.
+// Two launches per step here.  Every workgroup of 256 threads owns a tile of
+// 256 consecutive neurons of one instance.
+//   k_bin   one wave per fired row of the previous launch: splits the row
+//           (its synapses are ordered by target) into runs whose targets lie
+//           in one tile and queues each run on that tile (8-byte items);
+//   k_step  per tile: expands the tile's queued runs across the lanes and adds
+//           the weights into exact Q23.40 accumulators in LDS (64-bit integer
+//           LDS atomics, so the sum does not depend on the order), folds the
+//           delta into J (J = J * decay + delta), computes I_syn, runs OU + LIF
+//           for every substep and appends the fired rows to a compact list
+//           (wave ballot + lane prefix, one reservation per wave).
+// The launch order is k_step, k_bin, k_step, k_bin, ...: k_step of step k
+// consumes the runs that k_bin queued from the spikes of step k-1.  No global
+// atomics touch the delta.
+//   k_materialize writes the pending delta of every neuron (without consuming
+//   the queues) for sampling and state read-back; k_sample gathers samples.
 #include "dtb_kernels.hpp"
 #include "dtb_rng.hpp"
 
@@ -36,6 +41,16 @@ __device__ __forceinline__ unsigned long long f32_to_q40(float w)
     return (unsigned long long)(long long)rintf(w * 0x1p40f);
 }
 
+__device__ __forceinline__ int64_t qitem_begin(QItem q)
+{
+    return (int64_t)(q & kQBeginMask);
+}
+
+__device__ __forceinline__ uint32_t qitem_len(QItem q)
+{
+    return (uint32_t)(q >> kQBeginBits);
+}
+
 // synaptic_current.cpp:39-43 in source order, no contraction.
 __device__ __forceinline__ float syn_current(const float4 g, const float4 vu, const float4 J, const float V)
 {
@@ -46,19 +61,14 @@ __device__ __forceinline__ float syn_current(const float4 g, const float4 vu, co
     return a + b + c + d;
 }
 
-__device__ __forceinline__ float4 fold_delta(const float4 J, const float4 dc, const long long *dq)
+// presynaptic_voltage.cpp:62-73: J = J * decay + delta (fused multiply-add).
+__device__ __forceinline__ float4 fold_delta(const float4 J, const float4 dc, const float4 dl)
 {
-    const int4 lo = reinterpret_cast<const int4 *>(dq)[0];
-    const int4 hi = reinterpret_cast<const int4 *>(dq)[1];
-    const long long q0 = (long long)(((unsigned long long)(uint32_t)lo.y << 32) | (uint32_t)lo.x);
-    const long long q1 = (long long)(((unsigned long long)(uint32_t)lo.w << 32) | (uint32_t)lo.z);
-    const long long q2 = (long long)(((unsigned long long)(uint32_t)hi.y << 32) | (uint32_t)hi.x);
-    const long long q3 = (long long)(((unsigned long long)(uint32_t)hi.w << 32) | (uint32_t)hi.z);
     float4 r;
-    r.x = fmaf(J.x, dc.x, q40_to_f32(q0));
-    r.y = fmaf(J.y, dc.y, q40_to_f32(q1));
-    r.z = fmaf(J.z, dc.z, q40_to_f32(q2));
-    r.w = fmaf(J.w, dc.w, q40_to_f32(q3));
+    r.x = fmaf(J.x, dc.x, dl.x);
+    r.y = fmaf(J.y, dc.y, dl.y);
+    r.z = fmaf(J.z, dc.z, dl.z);
+    r.w = fmaf(J.w, dc.w, dl.w);
     return r;
 }
 
@@ -70,42 +80,136 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_neuron_step(const WgDesc *__restrict__ wgs,
-                                                        const InstDev *__restrict__ insts,
-                                                        FiredDev *__restrict__ fired,
-                                                        uint32_t fired_cap,
-                                                        StepCtl *__restrict__ ctl)
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, uint32_t lane)
 {
-    const WgDesc d = wgs[blockIdx.x];
-    const InstDev &I = insts[d.inst];
-    const uint32_t parity = ctl->tick & 1u;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        ctl->fired_count[parity ^ 1u] = 0u;   // consumed by the previous launch pair
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, kWave);
+        if (lane >= (uint32_t)off)
+            v += u;
+    }
+    return v;
+}
 
-    const uint32_t li = d.begin + threadIdx.x;
-    const bool active = threadIdx.x < d.count;
+// Adds the queued runs of one tile into acc[count][4] (Q23.40, LDS).  Each
+// wave takes 64 runs at a time, scans their lengths and lets every lane find
+// its synapse by binary search over the wave's 64 run offsets, so short runs
+// still use all lanes.
+__device__ __forceinline__ void accumulate_tile(unsigned long long *acc, int64_t *run_begin, uint32_t *run_off,
+                                                const QItem *__restrict__ q, uint32_t cnt, uint32_t tile_begin,
+                                                uint32_t count, const uint32_t *__restrict__ col,
+                                                const float2 *__restrict__ wt, uint32_t lane, uint32_t w)
+{
+    for (uint32_t b = w * kWave; b < cnt; b += kBlock) {
+        const uint32_t k = b + lane;
+        int64_t beg = 0;
+        uint32_t len = 0;
+        if (k < cnt) {
+            const QItem it = q[k];
+            beg = qitem_begin(it);
+            len = qitem_len(it);
+        }
+        const uint32_t incl = wave_incl_scan_u32(len, lane);
+        const uint32_t total = __shfl(incl, 63, kWave);
+        run_begin[lane] = beg;
+        run_off[lane] = incl - len;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t v = lane; v < total; v += kWave) {
+            // largest r with run_off[r] <= v (runs past cnt have offset total)
+            uint32_t lo = 0, hi = 63;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (run_off[mid] <= v)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int64_t s = run_begin[lo] + (int64_t)(v - run_off[lo]);
+            const uint32_t c = col[s];
+            const float2 x = wt[s];
+            const uint32_t loc = (c & kColMask) - tile_begin;
+            if (loc < count) {
+                const uint32_t ch = (c >> 31) << 1;
+                atomicAdd(&acc[loc * 4u + ch], f32_to_q40(x.x));
+                atomicAdd(&acc[loc * 4u + ch + 1u], f32_to_q40(x.y));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, uint32_t i)
+{
+    float4 r;
+    r.x = q40_to_f32((long long)acc[i * 4u + 0u]);
+    r.y = q40_to_f32((long long)acc[i * 4u + 1u]);
+    r.z = q40_to_f32((long long)acc[i * 4u + 2u]);
+    r.w = q40_to_f32((long long)acc[i * 4u + 3u]);
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ tiles,
+                                                 const InstDev *__restrict__ insts,
+                                                 const QItem *__restrict__ queue,
+                                                 uint32_t *__restrict__ qcnt,
+                                                 FiredDev *__restrict__ fired,
+                                                 uint32_t fired_cap,
+                                                 StepCtl *__restrict__ ctl)
+{
+    __shared__ unsigned long long acc[kBlock * 4];
+    __shared__ int64_t run_begin[kBlock / kWave][kWave];
+    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+
+    const TileDesc d = tiles[blockIdx.x];
+    const InstDev &I = insts[d.inst];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = tid / kWave;
+    const uint32_t parity = ctl->tick & 1u;
+    if (blockIdx.x == 0 && tid == 0) {
+        ctl->fired_count[parity ^ 1u] = 0u;   // consumed by the previous binning
+        ctl->cur = parity;                    // read by this step's binning
+    }
+
+    const uint32_t li = d.begin + tid;
+    const bool active = tid < d.count;
     const int64_t step = *I.step;
+
+    // Per-neuron loads first: they are independent of the accumulation.
+    float4 pa, pb, g, vu, dc, J;
+    float V = 0.0f, iou = 0.0f;
+    int32_t t = 0;
+    uint32_t fl = 0;
+    if (active) {
+        pa = I.pA[li];
+        pb = I.pB[li];
+        g = I.gU[li];
+        vu = I.vU[li];
+        dc = I.dec[li];
+        J = I.J[li];
+        V = I.V[li];
+        t = I.tcnt[li];
+        fl = I.flag[li];
+        iou = I.iou[li];
+    }
+
+    acc[tid * 4 + 0] = 0ull;
+    acc[tid * 4 + 1] = 0ull;
+    acc[tid * 4 + 2] = 0ull;
+    acc[tid * 4 + 3] = 0ull;
+    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    __syncthreads();
+    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    __syncthreads();
+    if (tid == 0)
+        qcnt[blockIdx.x] = 0u;   // this launch consumed the queue
+
     bool emit = false;
     uint32_t p = d.pop0;
-
     if (active) {
-        const float4 pa = I.pA[li];
-        const float4 pb = I.pB[li];
-        const float4 g = I.gU[li];
-        const float4 vu = I.vU[li];
-        const float4 dc = I.dec[li];
-        long long *dq = I.dq + 4 * (size_t)li;
-        const uint32_t fl = I.flag[li];
-        float4 J = I.J[li];
         if (!(fl & kFlagFolded))
-            J = fold_delta(J, dc, dq);
-        reinterpret_cast<int4 *>(dq)[0] = make_int4(0, 0, 0, 0);
-        reinterpret_cast<int4 *>(dq)[1] = make_int4(0, 0, 0, 0);
-
-        float V = I.V[li];
-        int32_t t = I.tcnt[li];
+            J = fold_delta(J, dc, acc_to_delta(acc, tid));
         uint32_t f = fl & kFlagSpike;
-        float iou = I.iou[li];
         float isyn = syn_current(g, vu, J, V);
 
         while (p + 1 < I.npop && li >= I.pops[p].end)
@@ -165,8 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_neuron_step(const WgDesc *__restrict
     // Fired-list compaction: one ballot per wave, lane prefix by popcount.
     const unsigned long long m = __ballot(emit);
     if (m != 0ull) {
-        const uint32_t lane = __lane_id();
-        const uint32_t cnt = (uint32_t)__popcll(m);
+        const uint32_t cnt_w = (uint32_t)__popcll(m);
         const uint32_t pre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         const int leader = __ffsll((long long)m) - 1;
         int64_t rb = 0;
@@ -178,9 +281,9 @@ __global__ __launch_bounds__(kBlock) void k_neuron_step(const WgDesc *__restrict
         const unsigned long long ev = wave_sum_u64((unsigned long long)len);
         uint32_t base = 0;
         if ((int)lane == leader) {
-            base = atomicAdd(&ctl->fired_count[parity], cnt);
+            base = atomicAdd(&ctl->fired_count[parity], cnt_w);
             atomicAdd(I.events, ev);
-            atomicAdd(I.spikes, (unsigned long long)cnt);
+            atomicAdd(I.spikes, (unsigned long long)cnt_w);
         }
         base = __shfl(base, leader, kWave);
         if (emit) {
@@ -198,71 +301,178 @@ __global__ __launch_bounds__(kBlock) void k_neuron_step(const WgDesc *__restrict
     }
 }
 
-__device__ __forceinline__ void scatter_one(unsigned long long *__restrict__ dq, uint32_t c, float2 w)
+// One wave per fired row (grid-stride).  Lane l holds synapses
+// c0 + 8l .. c0 + 8l + 7 of the row (512 per pass).  A run ends where the next
+// synapse's tile differs; the end's begin is the last run start at or before
+// it (within the lane, else from a wave max-scan of the lanes' last starts,
+// else carried from the previous pass).  Each end queues (first synapse,
+// length) on its tile.
+__global__ __launch_bounds__(kBlock) void k_bin(const InstDev *__restrict__ insts,
+                                                uint32_t ninst,
+                                                const FiredDev *__restrict__ fired,
+                                                uint32_t fired_cap,
+                                                const TileDesc *__restrict__ tiles,
+                                                uint32_t *__restrict__ qcnt,
+                                                QItem *__restrict__ queue,
+                                                StepCtl *__restrict__ ctl)
 {
-    const uint32_t post = c & kColMask;
-    const uint32_t ch = (c >> 31) << 1;
-    unsigned long long *a = dq + 4 * (size_t)post + ch;
-    atomicAdd(a, f32_to_q40(w.x));
-    atomicAdd(a + 1, f32_to_q40(w.y));
-}
-
-// Wave-per-row over the fired list (grid-stride), 4 synapses per lane in
-// flight.  The last workgroup to finish advances the launch tick and every
-// instance's step counter.
-__global__ __launch_bounds__(kBlock) void k_propagate(const InstDev *__restrict__ insts,
-                                                      uint32_t ninst,
-                                                      const FiredDev *__restrict__ fired,
-                                                      uint32_t fired_cap,
-                                                      StepCtl *__restrict__ ctl)
-{
-    const uint32_t parity = ctl->tick & 1u;
+    constexpr uint32_t K = 8;
+    constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
+    const uint32_t parity = ctl->cur;
     const uint32_t F = ctl->fired_count[parity];
     const FiredDev *list = fired + (size_t)parity * fired_cap;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t wave = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
     const uint32_t nwaves = gridDim.x * (kBlock / kWave);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // The step is complete once this launch runs: only this thread writes
+        // the tick and the step counters (no kernel of this launch reads them).
+        ctl->tick = ctl->tick + 1u;
+        for (uint32_t i = 0; i < ninst; ++i)
+            *insts[i].step = *insts[i].step + 1;
+    }
 
     for (uint32_t r = wave; r < F; r += nwaves) {
         const FiredDev e = list[r];
         if (e.inst >= ninst)
-            continue;   // never written by the neuron step; kept as a bound
+            continue;
         const InstDev &I = insts[e.inst];
         const uint32_t *__restrict__ col = I.col + e.row_begin;
-        const float2 *__restrict__ w = I.w + e.row_begin;
-        unsigned long long *__restrict__ dq = reinterpret_cast<unsigned long long *>(I.dq);
         const uint32_t len = e.len;
-        uint32_t j = lane;
-        for (; j + 3 * kWave < len; j += 4 * kWave) {
-            const uint32_t c0 = col[j], c1 = col[j + kWave], c2 = col[j + 2 * kWave], c3 = col[j + 3 * kWave];
-            const float2 w0 = w[j], w1 = w[j + kWave], w2 = w[j + 2 * kWave], w3 = w[j + 3 * kWave];
-            scatter_one(dq, c0, w0);
-            scatter_one(dq, c1, w1);
-            scatter_one(dq, c2, w2);
-            scatter_one(dq, c3, w3);
-        }
-        for (; j < len; j += kWave)
-            scatter_one(dq, col[j], w[j]);
-    }
-
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t prev = atomicAdd(&ctl->done_blocks, 1u);
-        if (prev == gridDim.x - 1) {
-            ctl->done_blocks = 0u;
-            ctl->tick = ctl->tick + 1u;
-            for (uint32_t i = 0; i < ninst; ++i)
-                *insts[i].step = *insts[i].step + 1;
-            __threadfence();
+        const uint32_t tile_base = I.tile_base;
+        uint32_t carry_tile = 0xfffffffcu;   // tile of the synapse before the pass
+        uint32_t carry_begin = 0;            // start of the run open at the pass boundary
+        for (uint32_t c0 = 0; c0 < len; c0 += kWave * K) {
+            const uint32_t j0 = c0 + lane * K;
+            uint32_t tl[K];
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k)
+                tl[k] = (j0 + k < len) ? ((col[j0 + k] & kColMask) >> kTileBits) : (0xfffffff0u + k);
+            // neighbours across lanes
+            uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
+            uint32_t nextK = __shfl_down(tl[0], 1, kWave);
+            if (lane == 0)
+                prev0 = carry_tile;
+            if (lane == 63) {
+                const uint32_t jn = c0 + kWave * K;
+                nextK = (jn < len) ? ((col[jn] & kColMask) >> kTileBits) : 0xffffffe0u;
+            }
+            // last start within this lane (absolute position), or none
+            uint32_t last_start = kNone;
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k) {
+                const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+                if (j0 + k < len && tl[k] != pv)
+                    last_start = j0 + k;
+            }
+            // exclusive max-scan over lanes of last_start (kNone counts as -1)
+            uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const uint32_t u = __shfl_up(sc, off, kWave);
+                if (lane >= (uint32_t)off)
+                    sc = max(sc, u);
+            }
+            uint32_t before = __shfl_up(sc, 1, kWave);
+            if (lane == 0)
+                before = 0u;
+            uint32_t open = (before != 0u) ? before - 1u : carry_begin;   // run open at the lane's first synapse
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k) {
+                const uint32_t j = j0 + k;
+                if (j < len) {
+                    const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+                    const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
+                    if (tl[k] != pv)
+                        open = j;
+                    if (tl[k] != nx) {
+                        const uint32_t tg = tile_base + tl[k];
+                        const uint32_t pos = atomicAdd(&qcnt[tg], 1u);
+                        const TileDesc td = tiles[tg];
+                        if (pos < td.qcap)
+                            queue[td.qoff + pos] = ((QItem)(j + 1u - open) << kQBeginBits) |
+                                                   (QItem)(e.row_begin + (int64_t)open);
+                        else
+                            ctl->overflow = 1u;
+                    }
+                }
+            }
+            const uint32_t wmax = __shfl(sc, 63, kWave);
+            if (wmax != 0u)
+                carry_begin = wmax - 1u;
+            carry_tile = __shfl(tl[K - 1], 63, kWave);
         }
     }
 }
 
+// Writes the pending delta (the runs queued for the next k_step) of every
+// neuron of every tile without consuming the queues: used before sampling
+// I_syn and before reading the state back.
+__global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restrict__ tiles,
+                                                        const InstDev *__restrict__ insts,
+                                                        const QItem *__restrict__ queue,
+                                                        const uint32_t *__restrict__ qcnt)
+{
+    __shared__ unsigned long long acc[kBlock * 4];
+    __shared__ int64_t run_begin[kBlock / kWave][kWave];
+    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+    const TileDesc d = tiles[blockIdx.x];
+    const InstDev &I = insts[d.inst];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = tid / kWave;
+    acc[tid * 4 + 0] = 0ull;
+    acc[tid * 4 + 1] = 0ull;
+    acc[tid * 4 + 2] = 0ull;
+    acc[tid * 4 + 3] = 0ull;
+    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    __syncthreads();
+    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    __syncthreads();
+    if (tid < d.count)
+        I.dlt[d.begin + tid] = acc_to_delta(acc, tid);
+}
+
+// Folds the pending delta of every tile into J now (J = J * decay + delta,
+// with the decay of the step that produced it), marks J as folded and empties
+// the queues.  Runs before any change of parameters or of the instance set,
+// so no pending spike is lost or folded with a later decay.
+__global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ tiles,
+                                                   const InstDev *__restrict__ insts,
+                                                   const QItem *__restrict__ queue,
+                                                   uint32_t *__restrict__ qcnt)
+{
+    __shared__ unsigned long long acc[kBlock * 4];
+    __shared__ int64_t run_begin[kBlock / kWave][kWave];
+    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+    const TileDesc d = tiles[blockIdx.x];
+    const InstDev &I = insts[d.inst];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = tid / kWave;
+    acc[tid * 4 + 0] = 0ull;
+    acc[tid * 4 + 1] = 0ull;
+    acc[tid * 4 + 2] = 0ull;
+    acc[tid * 4 + 3] = 0ull;
+    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    __syncthreads();
+    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    __syncthreads();
+    if (tid < d.count) {
+        const uint32_t li = d.begin + tid;
+        const uint32_t fl = I.flag[li];
+        if (!(fl & kFlagFolded)) {
+            I.J[li] = fold_delta(I.J[li], I.dec[li], acc_to_delta(acc, tid));
+            I.flag[li] = (uint8_t)(fl | kFlagFolded);
+        }
+    }
+    if (tid == 0)
+        qcnt[blockIdx.x] = 0u;
+}
+
 // stat_result.cpp:393-433 (sampled spike flag, V, I_syn, i_ou).  Runs after
-// k_propagate, so the instance step counter already counts this step and the
-// pending delta of this step is folded in exactly as the next neuron step
-// will fold it.
+// the binning (which advanced the step counter) and k_materialize (dlt holds
+// this step's delta, folded in exactly as the next k_step will fold it).
 __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ insts,
                                                    const SampleDev *__restrict__ samples,
                                                    uint32_t nsamples,
@@ -285,15 +495,15 @@ __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ i
     if (R.v)
         R.v[o] = V;
     if (R.isyn) {
-        const float4 J = fold_delta(I.J[li], I.dec[li], I.dq + 4 * (size_t)li);   // flag bit 2 is clear here
+        const float4 J = fold_delta(I.J[li], I.dec[li], I.dlt[li]);   // flag bit 2 is clear here
         R.isyn[o] = syn_current(I.gU[li], I.vU[li], J, V);
     }
     if (R.iou)
         R.iou[o] = I.iou[li];
 }
 
-// Debug / verification entry: normals of the OU stream for a range of
-// neurons at one substep (used to check CPU/GPU bit identity).
+// Verification entry: normals of the OU stream for a range of neurons at one
+// substep (checks CPU/GPU bit identity of the generator).
 __global__ __launch_bounds__(kBlock) void k_normals(uint64_t seed, uint32_t neuron0, uint64_t substep,
                                                     uint32_t count, float *__restrict__ out)
 {
@@ -304,19 +514,40 @@ __global__ __launch_bounds__(kBlock) void k_normals(uint64_t seed, uint32_t neur
 
 // ---- launchers ----------------------------------------------------------
 
-hipError_t launch_neuron_step(const WgDesc *wgs, uint32_t nwg, const InstDev *insts, FiredDev *fired,
-                              uint32_t fired_cap, StepCtl *ctl, hipStream_t stream)
+hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
+                              uint32_t *qcnt, FiredDev *fired, uint32_t fired_cap, StepCtl *ctl, hipStream_t stream)
 {
-    if (nwg == 0)
+    if (ntiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_neuron_step, dim3(nwg), dim3(kBlock), 0, stream, wgs, insts, fired, fired_cap, ctl);
+    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt, fired, fired_cap,
+                       ctl);
     return hipGetLastError();
 }
 
-hipError_t launch_propagate(const InstDev *insts, uint32_t ninst, const FiredDev *fired, uint32_t fired_cap,
-                            StepCtl *ctl, uint32_t nblocks, hipStream_t stream)
+hipError_t launch_bin(const InstDev *insts, uint32_t ninst, const FiredDev *fired, uint32_t fired_cap,
+                      const TileDesc *tiles, uint32_t *qcnt, QItem *queue, StepCtl *ctl, uint32_t nblocks,
+                      hipStream_t stream)
 {
-    hipLaunchKernelGGL(k_propagate, dim3(nblocks), dim3(kBlock), 0, stream, insts, ninst, fired, fired_cap, ctl);
+    hipLaunchKernelGGL(k_bin, dim3(nblocks), dim3(kBlock), 0, stream, insts, ninst, fired, fired_cap, tiles, qcnt,
+                       queue, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
+                              const uint32_t *qcnt, hipStream_t stream)
+{
+    if (ntiles == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_materialize, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_settle(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
+                         uint32_t *qcnt, hipStream_t stream)
+{
+    if (ntiles == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_settle, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt);
     return hipGetLastError();
 }
 

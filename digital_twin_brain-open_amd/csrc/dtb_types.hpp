@@ -5,7 +5,7 @@
 // structure-of-arrays in HBM; parameters that the step reads together are
 // packed into float4 records so that one 64-lane wave issues 16-byte loads
 // over a contiguous 1 KiB.  Many instances can be stepped by one launch: the
-// kernels find their instance through a small descriptor table.
+// kernels find their instance through small descriptor tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,6 +17,8 @@ constexpr int kBlock = 256;            // threads per workgroup (4 wave64)
 constexpr int kWave = 64;
 constexpr uint32_t kKindBit = 0x80000000u;   // channel-pair bit packed into the column index
 constexpr uint32_t kColMask = 0x7fffffffu;
+constexpr uint32_t kTileBits = 8;            // target tile = one step workgroup = 256 neurons
+static_assert((1 << kTileBits) == kBlock, "a tile is one workgroup of neurons");
 
 // Flag byte per neuron: bit 0 = membrane spike flag f (reference
 // f_inner_actives_), bit 1 = spike emitted to the synapses in the last step
@@ -40,12 +42,21 @@ struct alignas(16) PopDev {
     uint32_t pad0, pad1;
 };
 
-// Fired-list entry written by the neuron step and consumed by propagation.
+// Fired-list entry written by the neuron step and consumed by the binning.
 struct alignas(16) FiredDev {
     uint32_t inst;      // instance index
     uint32_t len;       // out-degree of the fired row
     int64_t row_begin;  // first synapse of the row
 };
+
+// A run of consecutive synapses of one fired row whose targets lie in one
+// tile, queued for that tile: first synapse in bits 0..39, length in bits
+// 40..63.
+using QItem = uint64_t;
+constexpr uint32_t kQBeginBits = 40;
+constexpr uint64_t kQBeginMask = (1ull << kQBeginBits) - 1ull;
+constexpr int64_t kMaxSynapses = (int64_t)1 << kQBeginBits;
+constexpr int64_t kMaxRowLength = (int64_t)1 << (64 - kQBeginBits);
 
 // Everything a kernel needs about one instance.
 struct alignas(16) InstDev {
@@ -58,10 +69,10 @@ struct alignas(16) InstDev {
     // state
     float *V;
     int32_t *tcnt;              // refractory counter (reference t_actives_)
-    uint8_t *flag;              // kFlagSpike | kFlagEmit
+    uint8_t *flag;              // kFlagSpike | kFlagEmit | kFlagFolded
     float *iou;                 // OU current (reference i_ou_background_stimuli_)
-    float4 *J;                  // J_u after the previous step's decay (pending delta below)
-    long long *dq;              // [n][4] Q23.40 delta accumulators
+    float4 *J;                  // J_u after the previous step's decay (delta below pending)
+    float4 *dlt;                // delta of the last step per channel, fp32 of an exact sum
     // connectivity: presynaptic-major CSR, 64-bit row offsets
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
@@ -82,14 +93,20 @@ struct alignas(16) InstDev {
     int32_t t_steps;
     int32_t use_ou;
     float dt_sub;
+    uint32_t tile_base;         // engine-wide index of this instance's first tile
+    uint32_t pad;
 };
 
-// One workgroup of the neuron step: 256 consecutive neurons of one instance.
-struct alignas(16) WgDesc {
+// One tile = one workgroup of the step kernel: up to 256 consecutive neurons
+// of one instance, and the queue of synapse runs that target them.
+struct alignas(16) TileDesc {
     uint32_t inst;
     uint32_t begin;     // first instance-local neuron
-    uint32_t count;     // neurons in this workgroup (<= 256)
+    uint32_t count;     // neurons in the tile (<= 256)
     uint32_t pop0;      // population of the first neuron
+    uint64_t qoff;      // first queue slot
+    uint32_t qcap;      // queue capacity: runs targeting this tile over all rows
+    uint32_t pad;
 };
 
 // Sample request (reference stat_samples_kernel, stat_result.cpp:393-433).
@@ -110,11 +127,13 @@ struct alignas(16) SampleRecDev {
     uint32_t pad[3];
 };
 
-// Shared per-launch control block in device memory.
+// Shared per-engine control block in device memory.
 struct alignas(16) StepCtl {
     uint32_t fired_count[2];    // entries in fired list [parity]
-    uint32_t done_blocks;       // last-block detection in propagation
-    uint32_t tick;              // launches completed (parity source)
+    uint32_t cur;               // parity of the launch in flight (written by the neuron step)
+    uint32_t tick;              // steps completed by the engine (parity source)
+    uint32_t overflow;          // set if a queue bound was ever exceeded (never expected)
+    uint32_t pad[3];
 };
 
 }  // namespace dtb

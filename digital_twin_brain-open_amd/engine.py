@@ -183,7 +183,12 @@ class Engine:
     def set_prop_blocks(self, nblocks):
         _lib.check(self._lib.dtb_engine_set_prop_blocks(self._h, nblocks))
 
+    def check(self):
+        _lib.check(self._lib.dtb_engine_check(self._h))
+
     def time_kernels(self, steps):
+        """Runs `steps` steps; returns total ms of (step kernel, binning)
+        measured with HIP events on the engine stream."""
         a, b = c_float(0), c_float(0)
         _lib.check(self._lib.dtb_engine_time_kernels(self._h, steps, byref(a), byref(b)))
         return a.value, b.value

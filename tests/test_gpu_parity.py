@@ -29,6 +29,7 @@ def run_gpu(dtb, net, steps, seed, mu, sigma=0.1, tau=4.0, t_steps=1, use_ou=Tru
                   output_sample_iou=sample is not None and use_ou, iou=use_ou, t_steps=t_steps, graph=graph)
     st = blk.state()
     cnt = blk.counters()
+    blk.engine.check()
     if engine is None:
         blk.engine.close()
     return out, st, cnt
@@ -169,6 +170,37 @@ def test_checkpoint_resume(dtb, gpu):
     assert np.array_equal(bits(st_a["J"]), bits(st_b["J"]))
     blk.engine.close()
     blk2.engine.close()
+
+
+def test_split_run_equals_single_run(dtb, gpu):
+    """Pending synaptic input survives everything between runs: a run split by
+    no-op property updates, OU re-settings and sample changes equals one
+    uninterrupted run, bit for bit."""
+    net = dtb.gen_uniform(6000, 120, seed=13)
+
+    def fresh():
+        b = dtb.Block(net, delta_t=DT, seed=5)
+        b.update_ou_background_stimuli(b.subblk_id, 4.0, 1.5, 0.1)
+        return b
+
+    a = fresh()
+    ra = a.run(240)
+    sa = a.state()
+    b = fresh()
+    parts = [b.run(60)]
+    b.mul_property_by_subblk([0, 1], [7, 8], [1.0, 1.0])
+    parts.append(b.run(60, graph=True))
+    b.update_ou_background_stimuli(b.subblk_id, 4.0, 1.5, 0.1)
+    b.set_samples(np.arange(0, 6000, 7))
+    parts.append(b.run(60, output_sample_spike=True, output_sample_isynaptic=True))
+    _ = b.state()
+    parts.append(b.run(60))
+    sb = b.state()
+    assert np.array_equal(ra["freq"], np.concatenate([p["freq"] for p in parts]))
+    for k in ("V", "J", "i_ou", "I_syn"):
+        assert np.array_equal(bits(sa[k]), bits(sb[k])), k
+    a.engine.close()
+    b.engine.close()
 
 
 def test_property_updates_match_oracle(dtb, orc, gpu):
