@@ -68,8 +68,30 @@ def cpu_baseline(dtb, net, blk, steps):
                                                                      el / (steps * DT_MS * 1e-3))}
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of `mib` MiB (HIP's copy kernel) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    for _ in range(5):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes_per_copy": n, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run device copies of this size (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=500)
@@ -77,6 +99,9 @@ def main():
     ap.add_argument("--neurons", type=int, default=N_NEURONS)
     ap.add_argument("--prop-blocks", type=int, default=0)
     args = ap.parse_args()
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -21,7 +21,7 @@ Engine::Engine(int device) : device_(device)
     d_ctl_.mutable_gpu_data();
     int cus = 0;
     DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    prop_blocks_ = (uint32_t)std::max(1, cus);
+    prop_blocks_ = (uint32_t)std::max(1, cus) * 4u;   // 16 binning waves per CU
 }
 
 Engine::~Engine()

@@ -91,17 +91,26 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, uint32_t lane
     return v;
 }
 
-// Adds the queued runs of one tile into acc[count][4] (Q23.40, LDS).  Each
-// wave takes 64 runs at a time, scans their lengths and lets every lane find
-// its synapse by binary search over the wave's 64 run offsets, so short runs
-// still use all lanes.
-__device__ __forceinline__ void accumulate_tile(unsigned long long *acc, int64_t *run_begin, uint32_t *run_off,
+// LDS scratch of the run expansion.
+struct RunScratch {
+    int64_t begin[kBlock];
+    uint32_t off[kBlock];
+    uint32_t wave_total[kBlock / kWave];
+};
+
+// Adds the queued runs of one tile into acc[count][4] (Q23.40, LDS).  The
+// workgroup takes 256 runs at a time (one per thread), scans their lengths
+// across the workgroup, and every thread then finds the run of each synapse
+// it handles by binary search over the 256 run offsets, so short runs still
+// keep every lane busy.  Called by all threads with a uniform cnt.
+__device__ __forceinline__ void accumulate_tile(unsigned long long *acc, RunScratch &rs,
                                                 const QItem *__restrict__ q, uint32_t cnt, uint32_t tile_begin,
                                                 uint32_t count, const uint32_t *__restrict__ col,
-                                                const float2 *__restrict__ wt, uint32_t lane, uint32_t w)
+                                                const float2 *__restrict__ wt, uint32_t tid, uint32_t lane,
+                                                uint32_t w)
 {
-    for (uint32_t b = w * kWave; b < cnt; b += kBlock) {
-        const uint32_t k = b + lane;
+    for (uint32_t b = 0; b < cnt; b += kBlock) {
+        const uint32_t k = b + tid;
         int64_t beg = 0;
         uint32_t len = 0;
         if (k < cnt) {
@@ -110,21 +119,30 @@ __device__ __forceinline__ void accumulate_tile(unsigned long long *acc, int64_t
             len = qitem_len(it);
         }
         const uint32_t incl = wave_incl_scan_u32(len, lane);
-        const uint32_t total = __shfl(incl, 63, kWave);
-        run_begin[lane] = beg;
-        run_off[lane] = incl - len;
-        __builtin_amdgcn_wave_barrier();
-        for (uint32_t v = lane; v < total; v += kWave) {
-            // largest r with run_off[r] <= v (runs past cnt have offset total)
-            uint32_t lo = 0, hi = 63;
+        if (lane == kWave - 1)
+            rs.wave_total[w] = incl;
+        __syncthreads();
+        uint32_t base = 0, total = 0;
+#pragma unroll
+        for (uint32_t x = 0; x < kBlock / kWave; ++x) {
+            const uint32_t t = rs.wave_total[x];
+            base += (x < w) ? t : 0u;
+            total += t;
+        }
+        rs.begin[tid] = beg;
+        rs.off[tid] = base + incl - len;
+        __syncthreads();
+        for (uint32_t v = tid; v < total; v += kBlock) {
+            // largest r with off[r] <= v (runs past cnt have offset total)
+            uint32_t lo = 0, hi = kBlock - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) >> 1;
-                if (run_off[mid] <= v)
+                if (rs.off[mid] <= v)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            const int64_t s = run_begin[lo] + (int64_t)(v - run_off[lo]);
+            const int64_t s = rs.begin[lo] + (int64_t)(v - rs.off[lo]);
             const uint32_t c = col[s];
             const float2 x = wt[s];
             const uint32_t loc = (c & kColMask) - tile_begin;
@@ -134,7 +152,7 @@ __device__ __forceinline__ void accumulate_tile(unsigned long long *acc, int64_t
                 atomicAdd(&acc[loc * 4u + ch + 1u], f32_to_q40(x.y));
             }
         }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
     }
 }
 
@@ -157,8 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
                                                  StepCtl *__restrict__ ctl)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ int64_t run_begin[kBlock / kWave][kWave];
-    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+    __shared__ RunScratch rs;
 
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
@@ -199,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
     __syncthreads();
     if (tid == 0)
         qcnt[blockIdx.x] = 0u;   // this launch consumed the queue
@@ -377,25 +394,34 @@ __global__ __launch_bounds__(kBlock) void k_bin(const InstDev *__restrict__ inst
             if (lane == 0)
                 before = 0u;
             uint32_t open = (before != 0u) ? before - 1u : carry_begin;   // run open at the lane's first synapse
+            // Reserve all of this lane's queue slots first (independent
+            // atomics and loads in flight together), then store the items.
+            uint32_t pos[K], cap[K];
+            uint64_t qo[K];
+            QItem item[K];
 #pragma unroll
             for (uint32_t k = 0; k < K; ++k) {
                 const uint32_t j = j0 + k;
-                if (j < len) {
-                    const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
-                    const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
-                    if (tl[k] != pv)
-                        open = j;
-                    if (tl[k] != nx) {
-                        const uint32_t tg = tile_base + tl[k];
-                        const uint32_t pos = atomicAdd(&qcnt[tg], 1u);
-                        const TileDesc td = tiles[tg];
-                        if (pos < td.qcap)
-                            queue[td.qoff + pos] = ((QItem)(j + 1u - open) << kQBeginBits) |
-                                                   (QItem)(e.row_begin + (int64_t)open);
-                        else
-                            ctl->overflow = 1u;
-                    }
+                const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+                const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
+                if (j < len && tl[k] != pv)
+                    open = j;
+                cap[k] = 0u;
+                pos[k] = 1u;
+                if (j < len && tl[k] != nx) {
+                    const uint32_t tg = tile_base + tl[k];
+                    pos[k] = atomicAdd(&qcnt[tg], 1u);
+                    qo[k] = tiles[tg].qoff;
+                    cap[k] = tiles[tg].qcap;
+                    item[k] = ((QItem)(j + 1u - open) << kQBeginBits) | (QItem)(e.row_begin + (int64_t)open);
                 }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < K; ++k) {
+                if (pos[k] < cap[k])
+                    queue[qo[k] + pos[k]] = item[k];
+                else if (cap[k] != 0u)
+                    ctl->overflow = 1u;
             }
             const uint32_t wmax = __shfl(sc, 63, kWave);
             if (wmax != 0u)
@@ -414,8 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
                                                         const uint32_t *__restrict__ qcnt)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ int64_t run_begin[kBlock / kWave][kWave];
-    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+    __shared__ RunScratch rs;
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
@@ -427,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
     __syncthreads();
     if (tid < d.count)
         I.dlt[d.begin + tid] = acc_to_delta(acc, tid);
@@ -443,8 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
                                                    uint32_t *__restrict__ qcnt)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ int64_t run_begin[kBlock / kWave][kWave];
-    __shared__ uint32_t run_off[kBlock / kWave][kWave];
+    __shared__ RunScratch rs;
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
@@ -456,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, run_begin[w], run_off[w], queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, lane, w);
+    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
