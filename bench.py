@@ -7,8 +7,9 @@ the voxel, 30% from a sparse lognormal voxel-to-voxel matrix at 5% density),
 4 receptor channels, fp32 state and weights, dt = 0.1 ms, OU background
 (mean 5.0, sigma 0.1, tau 4 ms, about 10 Hz).  Synthetic, seeded.
 
-A step is one launch pair (neuron step + spike propagation) over the whole
-block with the network resident in HBM.  Prints one JSON line.
+A step is one launch of the step kernel (delta accumulation, neuron update
+and spike binning) over the whole block with the network resident in HBM.
+Prints one JSON line.
 """
 import argparse
 import importlib
@@ -30,15 +31,14 @@ OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
 NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic HBM bytes per step:
-# k_step, per neuron: parameters 5 x float4 (80 B) read; J float4 (16 B) read
-# + write; V, refractory counter, i_ou 3 x 4 B read + write; flag byte read +
-# write (= 138 B); per delivered synapse: column index + weight pair (12 B).
+# Algorithmic HBM bytes of one step (k_step):
+# per neuron: parameters 5 x float4 (80 B) read; J float4 (16 B) read + write;
+# V, refractory counter, i_ou 3 x 4 B read + write; flag byte read + write
+# (= 138 B); per fired row: its two row offsets (16 B); per delivered synapse:
+# column index + weight pair (12 B).
 STEP_NEURON_BYTES = 80 + 2 * 16 + 2 * 12 + 2 * 1
+STEP_ROW_BYTES = 16
 STEP_SYN_BYTES = 12
-# k_bin: fired-list entry (16 B) per fired row, column index (4 B) per
-# delivered synapse.
-BIN_ROW_BYTES, BIN_SYN_BYTES = 16, 4
 
 
 def log(msg):
@@ -97,7 +97,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=100, help="CPU oracle sample length (0 = skip)")
     ap.add_argument("--neurons", type=int, default=N_NEURONS)
-    ap.add_argument("--prop-blocks", type=int, default=0)
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
@@ -119,8 +118,6 @@ def main():
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
-    if args.prop_blocks:
-        eng.set_prop_blocks(args.prop_blocks)
     blk = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED, engine=eng)
     blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, OU_MEAN, OU_SIGMA)
     eng.configure(blk.inst, 1, True)
@@ -133,7 +130,7 @@ def main():
     # Timed region: K steps, HIP events around every kernel on the engine
     # stream.
     t0 = time.perf_counter()
-    ms_step, ms_bin = eng.time_kernels(args.steps)
+    ms_step = eng.time_kernels(args.steps)
     eng.sync()
     wall = time.perf_counter() - t0
     eng.check()
@@ -146,12 +143,9 @@ def main():
     rate_hz = spikes / (net.n * args.steps * DT_MS * 1e-3)
 
     ev_step, sp_step = events / args.steps, spikes / args.steps
-    us = {"k_step": ms_step * 1e3 / args.steps, "k_bin": ms_bin * 1e3 / args.steps}
-    algo_bytes = {"k_step": STEP_NEURON_BYTES * net.n + STEP_SYN_BYTES * ev_step,
-                  "k_bin": BIN_ROW_BYTES * sp_step + BIN_SYN_BYTES * ev_step}
-    gbs = {k: algo_bytes[k] / (us[k] * 1e-6) / 1e9 for k in us}
-    dominant = max(us, key=us.get)
-    achieved = gbs[dominant]
+    us = ms_step * 1e3 / args.steps
+    algo_bytes = STEP_NEURON_BYTES * net.n + STEP_ROW_BYTES * sp_step + STEP_SYN_BYTES * ev_step
+    achieved = algo_bytes / (us * 1e-6) / 1e9
     traffic_env = os.environ.get("DTB_BENCH_TRAFFIC_BYTES")
 
     res = {
@@ -173,10 +167,9 @@ def main():
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
-        "kernel_us_per_step": us,
-        "kernel_algo_bytes_per_step": algo_bytes,
-        "kernel_gbs": gbs,
-        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "kernel_us_per_step": {"k_step": us},
+        "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
+        "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": float(traffic_env) if traffic_env else None},
     }

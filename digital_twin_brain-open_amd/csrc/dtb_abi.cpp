@@ -247,25 +247,11 @@ int dtb_engine_sync(dtb_engine_t h)
     return guarded([&] { engine(h).sync(); });
 }
 
-int dtb_engine_set_prop_blocks(dtb_engine_t h, uint32_t nblocks)
-{
-    return guarded([&] {
-        dtb::Engine &e = engine(h);
-        if (nblocks == 0) {
-            int cus = 0;
-            DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device()));
-            nblocks = (uint32_t)(cus > 0 ? cus : 1);
-        }
-        e.set_prop_blocks(nblocks);
-        e.mark_dirty();
-    });
-}
-
-int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step, float *ms_bin)
+int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step)
 {
     return guarded([&] {
         need(steps >= 0, "negative step count");
-        engine(h).run_timed(steps, ms_step, ms_bin);
+        engine(h).run_timed(steps, ms_step);
     });
 }
 

@@ -160,7 +160,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     col_.resize((size_t)std::max<int64_t>(nsyn_, 1));
     w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
     pops_.resize(np);
-    step_.resize(1);
+    step_.resize(2);
     counters_.resize(2);
 
     derive_params(s);
@@ -444,27 +444,28 @@ void Instance::fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isy
 
 int64_t Instance::step(hipStream_t s)
 {
-    int64_t v = 0;
-    step_.download(&v, 1, s);
+    int64_t v[2] = {0, 0};
+    step_.download(v, 2, s);
     DTB_HIP(hipStreamSynchronize(s));
-    return v;
+    return std::max(v[0], v[1]);
 }
 
 void Instance::set_step(int64_t step, hipStream_t s)
 {
-    step_.upload(&step, 1, s);
+    const int64_t v[2] = {step, step};
+    step_.upload(v, 2, s);
     DTB_HIP(hipStreamSynchronize(s));
 }
 
 void Instance::counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s)
 {
     unsigned long long c[2] = {0, 0};
-    int64_t st = 0;
+    int64_t st[2] = {0, 0};
     counters_.download(c, 2, s);
-    step_.download(&st, 1, s);
+    step_.download(st, 2, s);
     DTB_HIP(hipStreamSynchronize(s));
     if (step)
-        *step = st;
+        *step = std::max(st[0], st[1]);
     if (events)
         *events = c[0];
     if (spikes)

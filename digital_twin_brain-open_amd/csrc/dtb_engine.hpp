@@ -241,6 +241,8 @@ public:
                     hipStream_t s);
     void write_state(const float *V, const int32_t *t, const uint8_t *f, const float *iou, const float *J,
                      hipStream_t s);
+    // The step index is kept in two device slots (read and written by
+    // alternating launch parities); the current step is the larger one.
     int64_t step(hipStream_t s);
     void set_step(int64_t step, hipStream_t s);
     void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
@@ -280,7 +282,7 @@ private:
     DataAllocator<uint32_t> col_;
     DataAllocator<float2> w_;
     DataAllocator<PopDev> pops_;
-    DataAllocator<int64_t> step_;
+    DataAllocator<int64_t> step_;                   // [2]
     DataAllocator<unsigned long long> counters_;   // events, spikes
     DataAllocator<uint32_t> rec_counts_;
     DataAllocator<uint8_t> rec_spike_;
@@ -291,10 +293,9 @@ private:
     std::vector<uint32_t> fine_runs_;
 };
 
-// Steps a set of instances together.  One neuron-step launch covers every
-// neuron of every active instance, one binning launch every fired row and one
-// accumulation launch every target tile.  The step loop can be captured into a
-// HIP graph.
+// Steps a set of instances together.  One launch per step covers every
+// neuron of every active instance (one workgroup per tile of 256 neurons).
+// The step loop can be captured into HIP graphs (one per starting parity).
 class Engine {
 public:
     explicit Engine(int device);
@@ -308,12 +309,10 @@ public:
     Instance *get(int id);
     size_t active() const;
 
-    // Runs `steps` steps of every active instance.  graph_chunk > 0 captures
+    // Runs `steps` steps of every active instance.  graph_chunk > 1 captures
     // that many steps into one HIP graph and replays it.
     void run(int64_t steps, int graph_chunk);
-    void set_prop_blocks(uint32_t nb) { prop_blocks_ = nb; }
-    uint32_t prop_blocks() const { return prop_blocks_; }
-    // True if a binning queue bound was ever exceeded (a bug, never expected).
+    // True if a queue bound was ever exceeded (a bug, never expected).
     bool queue_overflow();
     void sync();
     void mark_dirty() { dirty_ = true; }
@@ -327,16 +326,17 @@ public:
     // instance set change (a table rebuild empties the queues).
     void settle();
 
-    // Kernel-time measurement: events recorded around the step kernels
-    // (totals in ms over `steps` steps).
-    void run_timed(int64_t steps, float *ms_step, float *ms_bin);
+    // Kernel-time measurement: events recorded around every step kernel
+    // (total in ms over `steps` steps).
+    void run_timed(int64_t steps, float *ms_step);
 
 private:
     void rebuild();
-    void launch_step();
+    void launch_step();        // one step at parity par_, then par_ ^= 1
     void launch_step_k();
-    void launch_bin_k();
-    void drop_graph();
+    void launch_aux();         // materialize + sample after a step, if samples are set
+    void drop_graphs();
+    QueueSet queues();
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
@@ -344,18 +344,18 @@ private:
     std::vector<uint32_t> tile_base_;      // per instance slot, first tile in the engine table
     bool dirty_ = true;
     bool pending_ = false;                 // queues may hold an unfolded delta
-    uint32_t ninst_ = 0, fired_cap_ = 0, nsamples_ = 0, ntiles_ = 0;
-    uint32_t prop_blocks_ = 0;
+    uint32_t par_ = 0;                     // parity of the next step launch (= pending queue set)
+    uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;
+    uint64_t qtotal_ = 0;                  // queue slots per parity
     DataAllocator<InstDev> d_insts_;
-    DataAllocator<FiredDev> d_fired_;
     DataAllocator<StepCtl> d_ctl_;
     DataAllocator<SampleDev> d_samples_;
     DataAllocator<SampleRecDev> d_recs_;
     DataAllocator<TileDesc> d_tiles_;
-    DataAllocator<uint32_t> d_qcnt_;
-    DataAllocator<QItem> d_queue_;
-    hipGraphExec_t graph_ = nullptr;
-    int graph_steps_ = 0;
+    DataAllocator<uint32_t> d_qcnt_;       // [2][ntiles]
+    DataAllocator<QItem> d_queue_;         // [2][qtotal]
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
+    int graph_steps_[2] = {0, 0};
 };
 
 }  // namespace dtb

@@ -19,14 +19,11 @@ Engine::Engine(int device) : device_(device)
     DTB_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     d_ctl_.resize(1);
     d_ctl_.mutable_gpu_data();
-    int cus = 0;
-    DTB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-    prop_blocks_ = (uint32_t)std::max(1, cus) * 4u;   // 16 binning waves per CU
 }
 
 Engine::~Engine()
 {
-    drop_graph();
+    drop_graphs();
     insts_.clear();
     if (stream_)
         (void)hipStreamDestroy(stream_);
@@ -78,19 +75,30 @@ bool Engine::queue_overflow()
     return c.overflow != 0;
 }
 
-void Engine::drop_graph()
+void Engine::drop_graphs()
 {
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-        graph_steps_ = 0;
-    }
+    for (int p = 0; p < 2; ++p)
+        if (graph_[p]) {
+            (void)hipGraphExecDestroy(graph_[p]);
+            graph_[p] = nullptr;
+            graph_steps_[p] = 0;
+        }
+}
+
+QueueSet Engine::queues()
+{
+    QueueSet q{};
+    q.items = d_queue_.mutable_gpu_data();
+    q.count = d_qcnt_.mutable_gpu_data();
+    q.qtotal = qtotal_;
+    q.ntiles = ntiles_;
+    return q;
 }
 
 void Engine::rebuild()
 {
     settle();   // the queues are reallocated below; nothing pending may be lost
-    drop_graph();
+    drop_graphs();
     std::vector<InstDev> table;
     std::vector<TileDesc> tiles;
     std::vector<SampleDev> samples;
@@ -101,6 +109,9 @@ void Engine::rebuild()
         Instance *p = insts_[slot].get();
         if (!p)
             continue;
+        // Both step slots hold the current step, whatever parity the engine
+        // is at.
+        p->set_step(p->step(stream_), stream_);
         const uint32_t idx = (uint32_t)table.size();
         tile_base_[slot] = (uint32_t)tiles.size();
         table.push_back(p->dev((uint32_t)tiles.size()));
@@ -132,17 +143,15 @@ void Engine::rebuild()
     ninst_ = (uint32_t)table.size();
     ntiles_ = (uint32_t)tiles.size();
     nsamples_ = (uint32_t)samples.size();
-    fired_cap_ = (uint32_t)std::max<uint64_t>(total, 1);
+    qtotal_ = std::max<uint64_t>(qtotal, 1);
     d_insts_.resize(std::max<size_t>(table.size(), 1));
     d_tiles_.resize(std::max<size_t>(tiles.size(), 1));
     d_samples_.resize(std::max<size_t>(samples.size(), 1));
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
-    d_fired_.resize((size_t)fired_cap_ * 2);
-    d_qcnt_.resize(std::max<size_t>(tiles.size(), 1));
-    d_queue_.resize((size_t)std::max<uint64_t>(qtotal, 1));
+    d_qcnt_.resize(2 * std::max<size_t>(tiles.size(), 1));
+    d_queue_.resize((size_t)(2 * qtotal_));
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
-    d_fired_.mutable_gpu_data();
     d_qcnt_.mutable_gpu_data();
     d_queue_.mutable_gpu_data();
     d_samples_.mutable_gpu_data();
@@ -158,27 +167,26 @@ void Engine::rebuild()
 
 void Engine::launch_step_k()
 {
-    DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), d_queue_.gpu_data(),
-                               d_qcnt_.mutable_gpu_data(), d_fired_.mutable_gpu_data(), fired_cap_,
+    DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
                                d_ctl_.mutable_gpu_data(), stream_));
 }
 
-void Engine::launch_bin_k()
+void Engine::launch_aux()
 {
-    DTB_HIP(launch_bin(d_insts_.gpu_data(), ninst_, d_fired_.gpu_data(), fired_cap_, d_tiles_.gpu_data(),
-                       d_qcnt_.mutable_gpu_data(), d_queue_.mutable_gpu_data(), d_ctl_.mutable_gpu_data(),
-                       prop_blocks_, stream_));
+    // par_ has already advanced: the step just run had parity par_ ^ 1 and
+    // its spikes are queued in set par_.
+    if (nsamples_ > 0) {
+        DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
+        DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(), par_ ^ 1u,
+                              stream_));
+    }
 }
 
 void Engine::launch_step()
 {
     launch_step_k();
-    launch_bin_k();
-    if (nsamples_ > 0) {
-        DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), d_queue_.gpu_data(),
-                                   d_qcnt_.gpu_data(), stream_));
-        DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(), stream_));
-    }
+    par_ ^= 1u;
+    launch_aux();
 }
 
 void Engine::run(int64_t steps, int graph_chunk)
@@ -192,20 +200,26 @@ void Engine::run(int64_t steps, int graph_chunk)
         return;
     pending_ = true;
     if (graph_chunk > 1 && steps >= graph_chunk) {
-        if (!graph_ || graph_steps_ != graph_chunk) {
-            drop_graph();
-            hipGraph_t g = nullptr;
-            DTB_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < graph_chunk; ++k)
-                launch_step();
-            DTB_HIP(hipStreamEndCapture(stream_, &g));
-            DTB_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
-            DTB_HIP(hipGraphDestroy(g));
-            graph_steps_ = graph_chunk;
-        }
         int64_t done = 0;
         while (steps - done >= graph_chunk) {
-            DTB_HIP(hipGraphLaunch(graph_, stream_));
+            const uint32_t p0 = par_;
+            if (!graph_[p0] || graph_steps_[p0] != graph_chunk) {
+                if (graph_[p0]) {
+                    DTB_HIP(hipGraphExecDestroy(graph_[p0]));
+                    graph_[p0] = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                DTB_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < graph_chunk; ++k)
+                    launch_step();
+                DTB_HIP(hipStreamEndCapture(stream_, &g));
+                DTB_HIP(hipGraphInstantiate(&graph_[p0], g, nullptr, nullptr, 0));
+                DTB_HIP(hipGraphDestroy(g));
+                graph_steps_[p0] = graph_chunk;
+                par_ = p0;   // capture recorded the launches without running them
+            }
+            DTB_HIP(hipGraphLaunch(graph_[p0], stream_));
+            par_ = p0 ^ (uint32_t)(graph_chunk & 1);
             done += graph_chunk;
         }
         for (; done < steps; ++done)
@@ -221,8 +235,7 @@ void Engine::settle()
     if (!pending_)
         return;
     DTB_HIP(hipSetDevice(device_));
-    DTB_HIP(launch_settle(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), d_queue_.gpu_data(),
-                          d_qcnt_.mutable_gpu_data(), stream_));
+    DTB_HIP(launch_settle(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
     pending_ = false;
 }
@@ -232,8 +245,7 @@ void Engine::materialize()
     DTB_HIP(hipSetDevice(device_));
     if (dirty_)
         rebuild();
-    DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), d_queue_.gpu_data(),
-                               d_qcnt_.gpu_data(), stream_));
+    DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -244,12 +256,13 @@ void Engine::clear_queues(int id)
     if (dirty_)
         rebuild();
     const uint32_t ntile = (get(id)->n() + kBlock - 1) / kBlock;
-    DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + tile_base_[(size_t)id], 0, ntile * sizeof(uint32_t),
-                           stream_));
+    for (uint32_t p = 0; p < 2; ++p)
+        DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + (size_t)p * ntiles_ + tile_base_[(size_t)id], 0,
+                               ntile * sizeof(uint32_t), stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 
-void Engine::run_timed(int64_t steps, float *ms_step, float *ms_bin)
+void Engine::run_timed(int64_t steps, float *ms_step)
 {
     DTB_HIP(hipSetDevice(device_));
     if (dirty_)
@@ -257,45 +270,35 @@ void Engine::run_timed(int64_t steps, float *ms_step, float *ms_bin)
     if (ninst_ == 0 || steps <= 0)
         return;
     pending_ = true;
-    // Events bracket every kernel on the engine stream; they are read back
-    // once per chunk so the launch pipeline is not drained every step.
+    // Events bracket every step kernel on the engine stream; they are read
+    // back once per chunk so the launch pipeline is not drained every step.
     const int64_t chunk = 256;
-    std::vector<hipEvent_t> ev((size_t)(3 * chunk));
+    std::vector<hipEvent_t> ev((size_t)(2 * chunk));
     for (auto &e : ev)
         DTB_HIP(hipEventCreate(&e));
-    double ts = 0.0, tb = 0.0;
+    double ts = 0.0;
     for (int64_t base = 0; base < steps; base += chunk) {
         const int64_t m = std::min<int64_t>(chunk, steps - base);
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(3 * k)];
+            hipEvent_t *e = &ev[(size_t)(2 * k)];
             DTB_HIP(hipEventRecord(e[0], stream_));
             launch_step_k();
             DTB_HIP(hipEventRecord(e[1], stream_));
-            launch_bin_k();
-            DTB_HIP(hipEventRecord(e[2], stream_));
-            if (nsamples_ > 0) {
-                DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), d_queue_.gpu_data(),
-                                           d_qcnt_.gpu_data(), stream_));
-                DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(),
-                                      stream_));
-            }
+            par_ ^= 1u;
+            launch_aux();
         }
-        DTB_HIP(hipEventSynchronize(ev[(size_t)(3 * (m - 1) + 2)]));
+        DTB_HIP(hipEventSynchronize(ev[(size_t)(2 * (m - 1) + 1)]));
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(3 * k)];
-            float a = 0.0f, b = 0.0f;
+            hipEvent_t *e = &ev[(size_t)(2 * k)];
+            float a = 0.0f;
             DTB_HIP(hipEventElapsedTime(&a, e[0], e[1]));
-            DTB_HIP(hipEventElapsedTime(&b, e[1], e[2]));
             ts += a;
-            tb += b;
         }
     }
     for (auto &e : ev)
         DTB_HIP(hipEventDestroy(e));
     if (ms_step)
         *ms_step = (float)ts;
-    if (ms_bin)
-        *ms_bin = (float)tb;
 }
 
 void Engine::sync()

@@ -10,20 +10,20 @@
 //   J = J * decay + delta (presynaptic_voltage.cpp:46-120), I_syn
 //             (synaptic_current.cpp:7-105)
 //
-// Two launches per step here.  Every workgroup of 256 threads owns a tile of
-// 256 consecutive neurons of one instance.
-//   k_bin   one wave per fired row of the previous launch: splits the row
-//           (its synapses are ordered by target) into runs whose targets lie
-//           in one tile and queues each run on that tile (8-byte items);
-//   k_step  per tile: expands the tile's queued runs across the lanes and adds
-//           the weights into exact Q23.40 accumulators in LDS (64-bit integer
-//           LDS atomics, so the sum does not depend on the order), folds the
-//           delta into J (J = J * decay + delta), computes I_syn, runs OU + LIF
-//           for every substep and appends the fired rows to a compact list
-//           (wave ballot + lane prefix, one reservation per wave).
-// The launch order is k_step, k_bin, k_step, k_bin, ...: k_step of step k
-// consumes the runs that k_bin queued from the spikes of step k-1.  No global
-// atomics touch the delta.
+// One launch per step here (k_step).  Every workgroup of 256 threads owns a
+// tile of 256 consecutive neurons of one instance, and per tile:
+//   1. expands the tile's queued synapse runs across the lanes and adds the
+//      weights into exact Q23.40 accumulators in LDS (64-bit integer LDS
+//      atomics, so the sum does not depend on the order);
+//   2. folds the delta into J (J = J * decay + delta), computes I_syn, runs
+//      OU + LIF for every substep;
+//   3. every wave splits the rows of its fired neurons (their synapses are
+//      ordered by target) into runs whose targets lie in one tile and
+//      appends each run to that tile's queue for the next launch (8-byte
+//      items).
+// The queues are double-buffered: the launch of parity p drains set p and
+// fills set p ^ 1, so step k consumes exactly the runs queued from the spikes
+// of step k - 1.  No global atomics touch the delta.
 //   k_materialize writes the pending delta of every neuron (without consuming
 //   the queues) for sampling and state read-back; k_sample gathers samples.
 #include "dtb_kernels.hpp"
@@ -166,12 +166,101 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
+// Queues the synapses of one fired row for the next launch.  The row's
+// synapses are ordered by target, so they split into runs whose targets lie
+// in one tile; every run is appended (first synapse, length) to its tile's
+// queue.  Called by a whole wave with wave-uniform arguments.  Lane l holds
+// synapses c0 + 8l .. c0 + 8l + 7 of the row (512 per pass).  A run ends where
+// the next synapse's tile differs; its start is the last run start at or
+// before it (within the lane, else from a wave max-scan of the lanes' last
+// starts, else carried from the previous pass).
+__device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, int64_t row_begin, uint32_t len,
+                                        uint32_t tile_base, const TileDesc *__restrict__ tiles,
+                                        uint32_t *__restrict__ qcnt, QItem *__restrict__ queue,
+                                        StepCtl *__restrict__ ctl, uint32_t lane)
+{
+    constexpr uint32_t K = 8;
+    constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
+    const uint32_t *__restrict__ col = col_all + row_begin;
+    uint32_t carry_tile = 0xfffffffcu;   // tile of the synapse before the pass
+    uint32_t carry_begin = 0;            // start of the run open at the pass boundary
+    for (uint32_t c0 = 0; c0 < len; c0 += kWave * K) {
+        const uint32_t j0 = c0 + lane * K;
+        uint32_t tl[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            tl[k] = (j0 + k < len) ? ((col[j0 + k] & kColMask) >> kTileBits) : (0xfffffff0u + k);
+        // neighbours across lanes
+        uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
+        uint32_t nextK = __shfl_down(tl[0], 1, kWave);
+        if (lane == 0)
+            prev0 = carry_tile;
+        if (lane == kWave - 1) {
+            const uint32_t jn = c0 + kWave * K;
+            nextK = (jn < len) ? ((col[jn] & kColMask) >> kTileBits) : 0xffffffe0u;
+        }
+        // last start within this lane (absolute position), or none
+        uint32_t last_start = kNone;
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+            if (j0 + k < len && tl[k] != pv)
+                last_start = j0 + k;
+        }
+        // exclusive max-scan over lanes of last_start (kNone counts as -1)
+        uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t u = __shfl_up(sc, off, kWave);
+            if (lane >= (uint32_t)off)
+                sc = max(sc, u);
+        }
+        uint32_t before = __shfl_up(sc, 1, kWave);
+        if (lane == 0)
+            before = 0u;
+        uint32_t open = (before != 0u) ? before - 1u : carry_begin;   // run open at the lane's first synapse
+        // Reserve all of this lane's queue slots first (independent atomics
+        // and loads in flight together), then store the items.
+        uint32_t pos[K], cap[K];
+        uint64_t qo[K];
+        QItem item[K];
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            const uint32_t j = j0 + k;
+            const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+            const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
+            if (j < len && tl[k] != pv)
+                open = j;
+            cap[k] = 0u;
+            pos[k] = 1u;
+            if (j < len && tl[k] != nx) {
+                const uint32_t tg = tile_base + tl[k];
+                pos[k] = atomicAdd(&qcnt[tg], 1u);
+                qo[k] = tiles[tg].qoff;
+                cap[k] = tiles[tg].qcap;
+                item[k] = ((QItem)(j + 1u - open) << kQBeginBits) | (QItem)(row_begin + (int64_t)open);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            if (pos[k] < cap[k])
+                queue[qo[k] + pos[k]] = item[k];
+            else if (cap[k] != 0u)
+                ctl->overflow = 1u;
+        }
+        const uint32_t wmax = __shfl(sc, kWave - 1, kWave);
+        if (wmax != 0u)
+            carry_begin = wmax - 1u;
+        carry_tile = __shfl(tl[K - 1], kWave - 1, kWave);
+    }
+}
+
+// The step of launch parity `par`: drains queue set par, fills set par ^ 1,
+// reads the step index from step[par] and advances it into step[par ^ 1].
 __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ tiles,
                                                  const InstDev *__restrict__ insts,
-                                                 const QItem *__restrict__ queue,
-                                                 uint32_t *__restrict__ qcnt,
-                                                 FiredDev *__restrict__ fired,
-                                                 uint32_t fired_cap,
+                                                 QueueSet qs,
+                                                 uint32_t par,
                                                  StepCtl *__restrict__ ctl)
 {
     __shared__ unsigned long long acc[kBlock * 4];
@@ -182,15 +271,16 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
-    const uint32_t parity = ctl->tick & 1u;
-    if (blockIdx.x == 0 && tid == 0) {
-        ctl->fired_count[parity ^ 1u] = 0u;   // consumed by the previous binning
-        ctl->cur = parity;                    // read by this step's binning
-    }
+    const QItem *__restrict__ q_in = qs.items + (size_t)par * qs.qtotal;
+    uint32_t *__restrict__ cnt_in = qs.count + (size_t)par * qs.ntiles;
+    QItem *__restrict__ q_out = qs.items + (size_t)(par ^ 1u) * qs.qtotal;
+    uint32_t *__restrict__ cnt_out = qs.count + (size_t)(par ^ 1u) * qs.ntiles;
 
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
-    const int64_t step = *I.step;
+    const int64_t step = I.step[par];
+    if (d.begin == 0 && tid == 0)
+        I.step[par ^ 1u] = step + 1;   // no tile of this launch reads this slot
 
     // Per-neuron loads first: they are independent of the accumulation.
     float4 pa, pb, g, vu, dc, J;
@@ -214,12 +304,12 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    const uint32_t cnt = min(cnt_in[blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
+    accumulate_tile(acc, rs, q_in + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
     __syncthreads();
     if (tid == 0)
-        qcnt[blockIdx.x] = 0u;   // this launch consumed the queue
+        cnt_in[blockIdx.x] = 0u;   // this launch consumed the queue
 
     bool emit = false;
     uint32_t p = d.pop0;
@@ -283,161 +373,46 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         I.J[li] = J;
     }
 
-    // Fired-list compaction: one ballot per wave, lane prefix by popcount.
+    // Spike propagation (presynaptic_voltage.cpp:122-225): every wave queues
+    // the rows of its own fired neurons for the next launch.
     const unsigned long long m = __ballot(emit);
     if (m != 0ull) {
         const uint32_t cnt_w = (uint32_t)__popcll(m);
-        const uint32_t pre = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         const int leader = __ffsll((long long)m) - 1;
-        int64_t rb = 0;
+        long long rb = 0;
         uint32_t len = 0;
         if (emit) {
             rb = I.rowptr[li];
             len = (uint32_t)(I.rowptr[li + 1] - rb);
-        }
-        const unsigned long long ev = wave_sum_u64((unsigned long long)len);
-        uint32_t base = 0;
-        if ((int)lane == leader) {
-            base = atomicAdd(&ctl->fired_count[parity], cnt_w);
-            atomicAdd(I.events, ev);
-            atomicAdd(I.spikes, (unsigned long long)cnt_w);
-        }
-        base = __shfl(base, leader, kWave);
-        if (emit) {
-            FiredDev e;
-            e.inst = d.inst;
-            e.len = len;
-            e.row_begin = rb;
-            fired[(size_t)parity * fired_cap + base + pre] = e;
             if (I.rec_counts != nullptr && !I.blocked[li]) {
                 const int64_t row = step - I.rec_step0;
                 if (row >= 0 && row < (int64_t)I.rec_len)
                     atomicAdd(&I.rec_counts[(size_t)row * I.npop + p], 1u);
             }
         }
-    }
-}
-
-// One wave per fired row (grid-stride).  Lane l holds synapses
-// c0 + 8l .. c0 + 8l + 7 of the row (512 per pass).  A run ends where the next
-// synapse's tile differs; the end's begin is the last run start at or before
-// it (within the lane, else from a wave max-scan of the lanes' last starts,
-// else carried from the previous pass).  Each end queues (first synapse,
-// length) on its tile.
-__global__ __launch_bounds__(kBlock) void k_bin(const InstDev *__restrict__ insts,
-                                                uint32_t ninst,
-                                                const FiredDev *__restrict__ fired,
-                                                uint32_t fired_cap,
-                                                const TileDesc *__restrict__ tiles,
-                                                uint32_t *__restrict__ qcnt,
-                                                QItem *__restrict__ queue,
-                                                StepCtl *__restrict__ ctl)
-{
-    constexpr uint32_t K = 8;
-    constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
-    const uint32_t parity = ctl->cur;
-    const uint32_t F = ctl->fired_count[parity];
-    const FiredDev *list = fired + (size_t)parity * fired_cap;
-    const uint32_t lane = __lane_id();
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave);
-    const uint32_t nwaves = gridDim.x * (kBlock / kWave);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // The step is complete once this launch runs: only this thread writes
-        // the tick and the step counters (no kernel of this launch reads them).
-        ctl->tick = ctl->tick + 1u;
-        for (uint32_t i = 0; i < ninst; ++i)
-            *insts[i].step = *insts[i].step + 1;
-    }
-
-    for (uint32_t r = wave; r < F; r += nwaves) {
-        const FiredDev e = list[r];
-        if (e.inst >= ninst)
-            continue;
-        const InstDev &I = insts[e.inst];
-        const uint32_t *__restrict__ col = I.col + e.row_begin;
-        const uint32_t len = e.len;
-        const uint32_t tile_base = I.tile_base;
-        uint32_t carry_tile = 0xfffffffcu;   // tile of the synapse before the pass
-        uint32_t carry_begin = 0;            // start of the run open at the pass boundary
-        for (uint32_t c0 = 0; c0 < len; c0 += kWave * K) {
-            const uint32_t j0 = c0 + lane * K;
-            uint32_t tl[K];
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k)
-                tl[k] = (j0 + k < len) ? ((col[j0 + k] & kColMask) >> kTileBits) : (0xfffffff0u + k);
-            // neighbours across lanes
-            uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
-            uint32_t nextK = __shfl_down(tl[0], 1, kWave);
-            if (lane == 0)
-                prev0 = carry_tile;
-            if (lane == 63) {
-                const uint32_t jn = c0 + kWave * K;
-                nextK = (jn < len) ? ((col[jn] & kColMask) >> kTileBits) : 0xffffffe0u;
-            }
-            // last start within this lane (absolute position), or none
-            uint32_t last_start = kNone;
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
-                if (j0 + k < len && tl[k] != pv)
-                    last_start = j0 + k;
-            }
-            // exclusive max-scan over lanes of last_start (kNone counts as -1)
-            uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const uint32_t u = __shfl_up(sc, off, kWave);
-                if (lane >= (uint32_t)off)
-                    sc = max(sc, u);
-            }
-            uint32_t before = __shfl_up(sc, 1, kWave);
-            if (lane == 0)
-                before = 0u;
-            uint32_t open = (before != 0u) ? before - 1u : carry_begin;   // run open at the lane's first synapse
-            // Reserve all of this lane's queue slots first (independent
-            // atomics and loads in flight together), then store the items.
-            uint32_t pos[K], cap[K];
-            uint64_t qo[K];
-            QItem item[K];
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                const uint32_t j = j0 + k;
-                const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
-                const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
-                if (j < len && tl[k] != pv)
-                    open = j;
-                cap[k] = 0u;
-                pos[k] = 1u;
-                if (j < len && tl[k] != nx) {
-                    const uint32_t tg = tile_base + tl[k];
-                    pos[k] = atomicAdd(&qcnt[tg], 1u);
-                    qo[k] = tiles[tg].qoff;
-                    cap[k] = tiles[tg].qcap;
-                    item[k] = ((QItem)(j + 1u - open) << kQBeginBits) | (QItem)(e.row_begin + (int64_t)open);
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < K; ++k) {
-                if (pos[k] < cap[k])
-                    queue[qo[k] + pos[k]] = item[k];
-                else if (cap[k] != 0u)
-                    ctl->overflow = 1u;
-            }
-            const uint32_t wmax = __shfl(sc, 63, kWave);
-            if (wmax != 0u)
-                carry_begin = wmax - 1u;
-            carry_tile = __shfl(tl[K - 1], 63, kWave);
+        const unsigned long long ev = wave_sum_u64((unsigned long long)len);
+        if ((int)lane == leader) {
+            atomicAdd(I.events, ev);
+            atomicAdd(I.spikes, (unsigned long long)cnt_w);
+        }
+        unsigned long long rest = m;
+        while (rest != 0ull) {
+            const int src = __ffsll((long long)rest) - 1;
+            rest &= rest - 1ull;
+            const long long rb_s = __shfl(rb, src, kWave);
+            const uint32_t len_s = __shfl(len, src, kWave);
+            bin_row(I.col, (int64_t)rb_s, len_s, I.tile_base, tiles, cnt_out, q_out, ctl, lane);
         }
     }
 }
 
-// Writes the pending delta (the runs queued for the next k_step) of every
-// neuron of every tile without consuming the queues: used before sampling
-// I_syn and before reading the state back.
+// Writes the pending delta (the runs queued in queue set `set` for the next
+// k_step) of every neuron of every tile without consuming the queues: used
+// before sampling I_syn and before reading the state back.
 __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restrict__ tiles,
                                                         const InstDev *__restrict__ insts,
-                                                        const QItem *__restrict__ queue,
-                                                        const uint32_t *__restrict__ qcnt)
+                                                        QueueSet qs,
+                                                        uint32_t set)
 {
     __shared__ unsigned long long acc[kBlock * 4];
     __shared__ RunScratch rs;
@@ -450,22 +425,23 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    const uint32_t cnt = min(qs.count[(size_t)set * qs.ntiles + blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
+    accumulate_tile(acc, rs, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid,
+                    lane, w);
     __syncthreads();
     if (tid < d.count)
         I.dlt[d.begin + tid] = acc_to_delta(acc, tid);
 }
 
-// Folds the pending delta of every tile into J now (J = J * decay + delta,
-// with the decay of the step that produced it), marks J as folded and empties
-// the queues.  Runs before any change of parameters or of the instance set,
-// so no pending spike is lost or folded with a later decay.
+// Folds the pending delta of queue set `set` into J now (J = J * decay +
+// delta, with the decay of the step that produced it), marks J as folded and
+// empties the queues.  Runs before any change of parameters or of the
+// instance set, so no pending spike is lost or folded with a later decay.
 __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ tiles,
                                                    const InstDev *__restrict__ insts,
-                                                   const QItem *__restrict__ queue,
-                                                   uint32_t *__restrict__ qcnt)
+                                                   QueueSet qs,
+                                                   uint32_t set)
 {
     __shared__ unsigned long long acc[kBlock * 4];
     __shared__ RunScratch rs;
@@ -474,13 +450,15 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
+    uint32_t *cnt_p = qs.count + (size_t)set * qs.ntiles + blockIdx.x;
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(qcnt[blockIdx.x], d.qcap);
+    const uint32_t cnt = min(*cnt_p, d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, queue + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
+    accumulate_tile(acc, rs, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid,
+                    lane, w);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
@@ -491,16 +469,18 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
         }
     }
     if (tid == 0)
-        qcnt[blockIdx.x] = 0u;
+        *cnt_p = 0u;
 }
 
-// stat_result.cpp:393-433 (sampled spike flag, V, I_syn, i_ou).  Runs after
-// the binning (which advanced the step counter) and k_materialize (dlt holds
-// this step's delta, folded in exactly as the next k_step will fold it).
+// stat_result.cpp:393-433 (sampled spike flag, V, I_syn, i_ou) of the step
+// just run by the k_step launch of parity `par`.  Runs after k_materialize
+// (dlt holds that step's delta, folded in exactly as the next k_step will
+// fold it).
 __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ insts,
                                                    const SampleDev *__restrict__ samples,
                                                    uint32_t nsamples,
-                                                   const SampleRecDev *__restrict__ recs)
+                                                   const SampleRecDev *__restrict__ recs,
+                                                   uint32_t par)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= nsamples)
@@ -508,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ i
     const SampleDev s = samples[i];
     const InstDev &I = insts[s.inst];
     const SampleRecDev R = recs[s.inst];
-    const int64_t row = *I.step - 1 - I.rec_step0;
+    const int64_t row = I.step[par] - I.rec_step0;
     if (row < 0 || row >= (int64_t)I.rec_len)
         return;
     const uint32_t li = s.neuron;
@@ -538,50 +518,40 @@ __global__ __launch_bounds__(kBlock) void k_normals(uint64_t seed, uint32_t neur
 
 // ---- launchers ----------------------------------------------------------
 
-hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
-                              uint32_t *qcnt, FiredDev *fired, uint32_t fired_cap, StepCtl *ctl, hipStream_t stream)
+hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
+                              uint32_t par, StepCtl *ctl, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt, fired, fired_cap,
-                       ctl);
+    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
     return hipGetLastError();
 }
 
-hipError_t launch_bin(const InstDev *insts, uint32_t ninst, const FiredDev *fired, uint32_t fired_cap,
-                      const TileDesc *tiles, uint32_t *qcnt, QItem *queue, StepCtl *ctl, uint32_t nblocks,
-                      hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_bin, dim3(nblocks), dim3(kBlock), 0, stream, insts, ninst, fired, fired_cap, tiles, qcnt,
-                       queue, ctl);
-    return hipGetLastError();
-}
-
-hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
-                              const uint32_t *qcnt, hipStream_t stream)
+hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
+                              uint32_t set, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_materialize, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt);
+    hipLaunchKernelGGL(k_materialize, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, set);
     return hipGetLastError();
 }
 
-hipError_t launch_settle(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QItem *queue,
-                         uint32_t *qcnt, hipStream_t stream)
+hipError_t launch_settle(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
+                         uint32_t set, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_settle, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, queue, qcnt);
+    hipLaunchKernelGGL(k_settle, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, set);
     return hipGetLastError();
 }
 
 hipError_t launch_sample(const InstDev *insts, const SampleDev *samples, uint32_t nsamples,
-                         const SampleRecDev *recs, hipStream_t stream)
+                         const SampleRecDev *recs, uint32_t par, hipStream_t stream)
 {
     if (nsamples == 0)
         return hipSuccess;
     const uint32_t nb = (nsamples + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_sample, dim3(nb), dim3(kBlock), 0, stream, insts, samples, nsamples, recs);
+    hipLaunchKernelGGL(k_sample, dim3(nb), dim3(kBlock), 0, stream, insts, samples, nsamples, recs, par);
     return hipGetLastError();
 }
 

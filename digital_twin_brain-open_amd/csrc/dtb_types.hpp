@@ -26,8 +26,8 @@ static_assert((1 << kTileBits) == kBlock, "a tile is one workgroup of neurons");
 constexpr uint8_t kFlagSpike = 1u;
 constexpr uint8_t kFlagEmit = 2u;
 // bit 2: J already holds the reference-visible J of the last step (set when a
-// checkpoint is restored); the next neuron step uses it without the
-// decay-and-delta fold.
+// checkpoint is restored or pending input is settled); the next step uses it
+// without the decay-and-delta fold.
 constexpr uint8_t kFlagFolded = 4u;
 
 // Per-population record (reference sub-block, sub_bcounts_ ranges and the
@@ -40,13 +40,6 @@ struct alignas(16) PopDev {
     float f2s;          // sqrt(1 - exp(-2 dt/tau)) * sigma
     uint32_t ou_on;     // OU configured for this population
     uint32_t pad0, pad1;
-};
-
-// Fired-list entry written by the neuron step and consumed by the binning.
-struct alignas(16) FiredDev {
-    uint32_t inst;      // instance index
-    uint32_t len;       // out-degree of the fired row
-    int64_t row_begin;  // first synapse of the row
 };
 
 // A run of consecutive synapses of one fired row whose targets lie in one
@@ -72,7 +65,7 @@ struct alignas(16) InstDev {
     uint8_t *flag;              // kFlagSpike | kFlagEmit | kFlagFolded
     float *iou;                 // OU current (reference i_ou_background_stimuli_)
     float4 *J;                  // J_u after the previous step's decay (delta below pending)
-    float4 *dlt;                // delta of the last step per channel, fp32 of an exact sum
+    float4 *dlt;                // materialised pending delta (sampling / read-back only)
     // connectivity: presynaptic-major CSR, 64-bit row offsets
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
@@ -83,7 +76,8 @@ struct alignas(16) InstDev {
     // counters
     unsigned long long *events; // synaptic events delivered (sum of fired out-degrees)
     unsigned long long *spikes; // spikes emitted
-    int64_t *step;              // steps completed (device side)
+    int64_t *step;              // [2] step index by launch parity: the launch of
+                                // parity p reads step[p] and writes step[p^1] = step[p] + 1
     // scalars
     uint64_t seed;
     int64_t rec_step0;
@@ -104,8 +98,18 @@ struct alignas(16) TileDesc {
     uint32_t begin;     // first instance-local neuron
     uint32_t count;     // neurons in the tile (<= 256)
     uint32_t pop0;      // population of the first neuron
-    uint64_t qoff;      // first queue slot
+    uint64_t qoff;      // first queue slot (within one parity's queue set)
     uint32_t qcap;      // queue capacity: runs targeting this tile over all rows
+    uint32_t pad;
+};
+
+// Queues of both parities: the launch of parity p drains set p and fills set
+// p ^ 1.
+struct QueueSet {
+    QItem *items;       // [2][qtotal]
+    uint32_t *count;    // [2][ntiles]
+    uint64_t qtotal;
+    uint32_t ntiles;
     uint32_t pad;
 };
 
@@ -127,11 +131,8 @@ struct alignas(16) SampleRecDev {
     uint32_t pad[3];
 };
 
-// Shared per-engine control block in device memory.
+// Shared per-engine diagnostics in device memory.
 struct alignas(16) StepCtl {
-    uint32_t fired_count[2];    // entries in fired list [parity]
-    uint32_t cur;               // parity of the launch in flight (written by the neuron step)
-    uint32_t tick;              // steps completed by the engine (parity source)
     uint32_t overflow;          // set if a queue bound was ever exceeded (never expected)
     uint32_t pad[3];
 };

@@ -180,18 +180,15 @@ class Engine:
     def sync(self):
         _lib.check(self._lib.dtb_engine_sync(self._h))
 
-    def set_prop_blocks(self, nblocks):
-        _lib.check(self._lib.dtb_engine_set_prop_blocks(self._h, nblocks))
-
     def check(self):
         _lib.check(self._lib.dtb_engine_check(self._h))
 
     def time_kernels(self, steps):
-        """Runs `steps` steps; returns total ms of (step kernel, binning)
+        """Runs `steps` steps; returns the total ms of the step kernel
         measured with HIP events on the engine stream."""
-        a, b = c_float(0), c_float(0)
-        _lib.check(self._lib.dtb_engine_time_kernels(self._h, steps, byref(a), byref(b)))
-        return a.value, b.value
+        a = c_float(0)
+        _lib.check(self._lib.dtb_engine_time_kernels(self._h, steps, byref(a)))
+        return a.value
 
     def set_step(self, inst, step):
         _lib.check(self._lib.dtb_set_step(self._h, inst, step))

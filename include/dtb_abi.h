@@ -144,11 +144,9 @@ int dtb_engine_run(dtb_engine_t h, int64_t steps, int32_t graph_chunk);
 int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou);
 /* Waits for all work of the engine. */
 int dtb_engine_sync(dtb_engine_t h);
-/* Launch width (workgroups) of the spike binning, 0 = default (one per CU). */
-int dtb_engine_set_prop_blocks(dtb_engine_t h, uint32_t nblocks);
-/* Times `steps` steps with HIP events around each kernel (ms totals): the step
- * kernel (delta accumulation + neuron update) and the spike binning. */
-int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step, float *ms_bin);
+/* Times `steps` steps with HIP events around each step kernel (delta
+ * accumulation + neuron update + spike binning); total in ms. */
+int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step);
 /* Waits for the engine and reports DTB_INVALID_NETWORK if a propagation queue
  * bound was ever exceeded (a defect; never expected). */
 int dtb_engine_check(dtb_engine_t h);

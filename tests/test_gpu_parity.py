@@ -131,20 +131,39 @@ def test_noise_spike_mode(dtb, orc, gpu):
     assert ref["raster"].sum() > 100
 
 
-def test_graph_equals_eager_and_launch_width(dtb, gpu):
+def test_graph_equals_eager(dtb, gpu):
     net = dtb.gen_uniform(6000, 150, seed=9)
     res = []
-    for graph, blocks in [(False, 0), (True, 0), (False, 7), (True, 3000)]:
-        eng = dtb.Engine(0)
-        if blocks:
-            eng.set_prop_blocks(blocks)
-        out, st, _ = run_gpu(dtb, net, 257, 17, mu=1.5, graph=graph, engine=eng)
-        eng.close()
+    for graph in (False, True):
+        out, st, _ = run_gpu(dtb, net, 257, 17, mu=1.5, graph=graph)
         res.append((out["freq"], st))
-    for freq, st in res[1:]:
-        assert np.array_equal(freq, res[0][0])
-        assert np.array_equal(bits(st["V"]), bits(res[0][1]["V"]))
-        assert np.array_equal(bits(st["J"]), bits(res[0][1]["J"]))
+    freq, st = res[1]
+    assert np.array_equal(freq, res[0][0])
+    assert np.array_equal(bits(st["V"]), bits(res[0][1]["V"]))
+    assert np.array_equal(bits(st["J"]), bits(res[0][1]["J"]))
+
+
+def test_graph_chunks_both_parities(dtb, gpu):
+    """Replayed graphs of even and odd chunk lengths, entered at either launch
+    parity and mixed with eager steps, step exactly like eager stepping."""
+    net = dtb.gen_uniform(6000, 150, seed=9)
+    res = []
+    for plan in ([(257, 0)], [(1, 0), (256, 64)], [(3, 0), (254, 7)], [(100, 5), (157, 2)]):
+        eng = dtb.Engine(0)
+        blk = dtb.Block(net, delta_t=DT, seed=17, engine=eng)
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 1.5, 0.1)
+        eng.configure(blk.inst, 1, True)
+        for steps, chunk in plan:
+            eng.run(steps, graph_chunk=chunk)
+        eng.check()
+        res.append((blk.state(), blk.counters()))
+        eng.close()
+    assert res[0][1][0] == 257 and res[0][1][2] > 100
+    for st, cnt in res[1:]:
+        assert cnt == res[0][1]
+        for k in ("V", "J", "i_ou", "I_syn"):
+            assert np.array_equal(bits(st[k]), bits(res[0][0][k])), k
+        assert np.array_equal(st["t"], res[0][0]["t"])
 
 
 def test_checkpoint_resume(dtb, gpu):
