@@ -5,7 +5,8 @@ Workload (one GPU's share of the whole-brain model): 2,000,000 LIF neurons
 in 200 voxels (lognormal sizes, 80/20 E/I each), in-degree 500 (70% within
 the voxel, 30% from a sparse lognormal voxel-to-voxel matrix at 5% density),
 4 receptor channels, fp32 state and weights, dt = 0.1 ms, OU background
-(mean 5.0, sigma 0.1, tau 4 ms, about 10 Hz).  Synthetic, seeded.
+(mean 5.0, sigma 0.1, tau 4 ms, about 10 Hz), one parameter record per
+population (400).  Synthetic, seeded.
 
 A step is one launch of the step kernel (delta accumulation, neuron update
 and spike binning) over the whole block with the network resident in HBM.
@@ -32,11 +33,12 @@ NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes of one step (k_step):
-# per neuron: parameters 5 x float4 (80 B) read; J float4 (16 B) read + write;
-# V, refractory counter, i_ou 3 x 4 B read + write; flag byte read + write
-# (= 138 B); per fired row: its two row offsets (16 B); per delivered synapse:
-# column index + weight pair (12 B).
-STEP_NEURON_BYTES = 80 + 2 * 16 + 2 * 12 + 2 * 1
+# per neuron: parameter class (2 B) read; J float4 (16 B) read + write; V,
+# refractory counter, i_ou 3 x 4 B read + write; flag byte read + write
+# (= 60 B; the 400 parameter records, 80 B each, stay in cache); per fired
+# row: its two row offsets (16 B); per delivered synapse: column index +
+# weight pair (12 B).
+STEP_NEURON_BYTES = 2 + 2 * 16 + 2 * 12 + 2 * 1
 STEP_ROW_BYTES = 16
 STEP_SYN_BYTES = 12
 
@@ -120,6 +122,11 @@ def main():
     eng = dtb.Engine(0)
     blk = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED, engine=eng)
     blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, OU_MEAN, OU_SIGMA)
+    # One parameter record per population, as after per-population
+    # assimilation updates: g_AMPA differs by 1e-4 relative per population.
+    pops = np.asarray(blk.subblk_id, dtype=np.uint32)
+    g = net.props[0, 10] * (1.0 + 1e-4 * np.arange(len(pops)))
+    blk.assign_property_by_subblk(pops, np.full(len(pops), 7, np.uint32), g.astype(np.float32))
     eng.configure(blk.inst, 1, True)
     log("uploaded in %.1f s" % (time.perf_counter() - t0))
 

@@ -175,6 +175,7 @@ int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const
         e.get(inst);
         e.settle();
         e.get(inst)->update_props(neuron_ids, prop_ids, vals, (size_t)n, e.stream());
+        e.mark_dirty();   // the parameter tables may have been reallocated
     });
 }
 
@@ -187,6 +188,7 @@ int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, con
         e.get(inst);
         e.settle();
         e.get(inst)->update_prop_cols(prop_ids, brain_ids, vals, (size_t)n, assign != 0, e.stream());
+        e.mark_dirty();   // the parameter tables may have been reallocated
     });
 }
 

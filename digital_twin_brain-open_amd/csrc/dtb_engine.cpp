@@ -144,11 +144,6 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     ou_sigma_.assign(np, 0.0f);
     ou_tau_.assign(np, 1.0f);
 
-    pA_.resize(n_);
-    pB_.resize(n_);
-    gU_.resize(n_);
-    vU_.resize(n_);
-    dec_.resize(n_);
     J_.resize(n_);
     blocked_.resize(n_);
     flag_.resize(n_);
@@ -229,27 +224,111 @@ uint32_t Instance::tile_runs(uint32_t tile) const
     return fine_runs_.at(tile);
 }
 
+namespace {
+
+// The kernel-side parameter record of one neuron (five float4 columns).
+struct ParamRec {
+    float4 a, b, g, v, d;
+};
+
+bool same_bits(const ParamRec &x, const ParamRec &y)
+{
+    return std::memcmp(&x, &y, sizeof(ParamRec)) == 0;
+}
+
+struct ParamRecHash {
+    size_t operator()(const ParamRec &r) const
+    {
+        uint32_t w[sizeof(ParamRec) / 4];
+        std::memcpy(w, &r, sizeof(ParamRec));
+        uint64_t h = 0x9e3779b97f4a7c15ull;
+        for (uint32_t x : w)
+            h = (h ^ x) * 0x100000001b3ull;
+        return (size_t)(h ^ (h >> 29));
+    }
+};
+
+struct ParamRecEq {
+    bool operator()(const ParamRec &x, const ParamRec &y) const { return same_bits(x, y); }
+};
+
+}  // namespace
+
 void Instance::derive_params(hipStream_t s)
 {
     const float dt_sub = dt_ / (float)t_steps_;
+    const float *prev = nullptr;
+    ParamRec cur{};
+    auto derive = [&](uint32_t i) -> const ParamRec & {
+        const float *p = &props_[(size_t)i * kNumCols];
+        if (prev != nullptr && std::memcmp(p, prev, sizeof(float) * kNumCols) == 0)
+            return cur;   // same property row as the neuron before
+        prev = p;
+        cur.a = make_float4(p[kColVReset], p[kColVTh], 1.0f / p[kColC], p[kColVLeak]);
+        cur.b = make_float4(p[kColGLeak], p[kColIExt], p[kColNoiseRate],
+                            bits_as_float(refractory_substeps(p[kColTRef], dt_, t_steps_)));
+        cur.g = make_float4(p[kColGU + 0], p[kColGU + 1], p[kColGU + 2], p[kColGU + 3]);
+        cur.v = make_float4(p[kColVU + 0], p[kColVU + 1], p[kColVU + 2], p[kColVU + 3]);
+        // property.cpp:19-29: exp(-dt / tau)
+        cur.d = make_float4(host_expf((-dt_sub) / p[kColTauU + 0]), host_expf((-dt_sub) / p[kColTauU + 1]),
+                            host_expf((-dt_sub) / p[kColTauU + 2]), host_expf((-dt_sub) / p[kColTauU + 3]));
+        return cur;
+    };
+
+    // Deduplicate the records: class per neuron + one record per class.
+    std::vector<ParamRec> recs;
+    std::vector<uint16_t> cls(n_);
+    std::unordered_map<ParamRec, uint32_t, ParamRecHash, ParamRecEq> index;
+    bool dense = false;
+    for (uint32_t i = 0; i < n_ && !dense; ++i) {
+        const ParamRec &r = derive(i);
+        if (!recs.empty() && same_bits(r, recs[cls[i - 1]])) {
+            cls[i] = cls[i - 1];
+            continue;
+        }
+        auto it = index.find(r);
+        if (it == index.end()) {
+            if (recs.size() == kMaxParamClasses) {
+                dense = true;
+                break;
+            }
+            it = index.emplace(r, (uint32_t)recs.size()).first;
+            recs.push_back(r);
+        }
+        cls[i] = (uint16_t)it->second;
+    }
+    dense_params_ = dense;
+    const size_t nrec = dense ? n_ : recs.size();
+    pA_.resize(nrec);
+    pB_.resize(nrec);
+    gU_.resize(nrec);
+    vU_.resize(nrec);
+    dec_.resize(nrec);
+    cls_.resize(dense ? 0 : n_);
     float4 *pa = pA_.mutable_cpu_data();
     float4 *pb = pB_.mutable_cpu_data();
     float4 *gu = gU_.mutable_cpu_data();
     float4 *vu = vU_.mutable_cpu_data();
     float4 *dc = dec_.mutable_cpu_data();
-    uint8_t *bl = blocked_.mutable_cpu_data();
-    for (uint32_t i = 0; i < n_; ++i) {
-        const float *p = &props_[(size_t)i * kNumCols];
-        pa[i] = make_float4(p[kColVReset], p[kColVTh], 1.0f / p[kColC], p[kColVLeak]);
-        pb[i] = make_float4(p[kColGLeak], p[kColIExt], p[kColNoiseRate],
-                            bits_as_float(refractory_substeps(p[kColTRef], dt_, t_steps_)));
-        gu[i] = make_float4(p[kColGU + 0], p[kColGU + 1], p[kColGU + 2], p[kColGU + 3]);
-        vu[i] = make_float4(p[kColVU + 0], p[kColVU + 1], p[kColVU + 2], p[kColVU + 3]);
-        // property.cpp:19-29: exp(-dt / tau)
-        dc[i] = make_float4(host_expf((-dt_sub) / p[kColTauU + 0]), host_expf((-dt_sub) / p[kColTauU + 1]),
-                            host_expf((-dt_sub) / p[kColTauU + 2]), host_expf((-dt_sub) / p[kColTauU + 3]));
-        bl[i] = p[kColBlocked] != 0.0f;
+    auto put = [&](size_t k, const ParamRec &r) {
+        pa[k] = r.a;
+        pb[k] = r.b;
+        gu[k] = r.g;
+        vu[k] = r.v;
+        dc[k] = r.d;
+    };
+    if (dense) {
+        prev = nullptr;
+        for (uint32_t i = 0; i < n_; ++i)
+            put(i, derive(i));
+    } else {
+        for (size_t k = 0; k < recs.size(); ++k)
+            put(k, recs[k]);
+        std::memcpy(cls_.mutable_cpu_data(), cls.data(), sizeof(uint16_t) * n_);
     }
+    uint8_t *bl = blocked_.mutable_cpu_data();
+    for (uint32_t i = 0; i < n_; ++i)
+        bl[i] = props_[(size_t)i * kNumCols + kColBlocked] != 0.0f;
     PopDev *pd = pops_.mutable_cpu_data();
     for (size_t q = 0; q < pop_ids_.size(); ++q) {
         PopDev r{};
@@ -267,6 +346,7 @@ void Instance::derive_params(hipStream_t s)
     gU_.to_gpu(s);
     vU_.to_gpu(s);
     dec_.to_gpu(s);
+    cls_.to_gpu(s);
     blocked_.to_gpu(s);
     pops_.to_gpu(s);
     DTB_HIP(hipStreamSynchronize(s));
@@ -501,16 +581,19 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
     }
     if (J || isyn) {
         // Fold the pending delta exactly as the next neuron step will.
-        const float4 *dc = dec_.cpu_data();
-        const float4 *g = gU_.cpu_data();
-        const float4 *vu = vU_.cpu_data();
+        const float4 *dcs = dec_.cpu_data();
+        const float4 *gs = gU_.cpu_data();
+        const float4 *vus = vU_.cpu_data();
+        const uint16_t *cl = dense_params_ ? nullptr : cls_.cpu_data();
         for (uint32_t i = 0; i < n_; ++i) {
+            const size_t c = cl ? cl[i] : i;
+            const float4 *dc = dcs + c, *g = gs + c, *vu = vus + c;
             float4 r = j[i];
             if (!(fl[i] & kFlagFolded)) {
-                r.x = std::fma(j[i].x, dc[i].x, dl[i].x);
-                r.y = std::fma(j[i].y, dc[i].y, dl[i].y);
-                r.z = std::fma(j[i].z, dc[i].z, dl[i].z);
-                r.w = std::fma(j[i].w, dc[i].w, dl[i].w);
+                r.x = std::fma(j[i].x, dc->x, dl[i].x);
+                r.y = std::fma(j[i].y, dc->y, dl[i].y);
+                r.z = std::fma(j[i].z, dc->z, dl[i].z);
+                r.w = std::fma(j[i].w, dc->w, dl[i].w);
             }
             if (J) {
                 J[(size_t)i * 4 + 0] = r.x;
@@ -519,11 +602,11 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
                 J[(size_t)i * 4 + 3] = r.w;
             }
             if (isyn) {
-                const float a = g[i].x * (vu[i].x - v[i]) * r.x;
-                const float b = g[i].y * (vu[i].y - v[i]) * r.y;
-                const float c = g[i].z * (vu[i].z - v[i]) * r.z;
-                const float d = g[i].w * (vu[i].w - v[i]) * r.w;
-                isyn[i] = a + b + c + d;
+                const float a = g->x * (vu->x - v[i]) * r.x;
+                const float b = g->y * (vu->y - v[i]) * r.y;
+                const float c2 = g->z * (vu->z - v[i]) * r.z;
+                const float d = g->w * (vu->w - v[i]) * r.w;
+                isyn[i] = a + b + c2 + d;
             }
         }
     }
@@ -571,6 +654,7 @@ InstDev Instance::dev(uint32_t tile_base)
     d.gU = gU_.gpu_data();
     d.vU = vU_.gpu_data();
     d.dec = dec_.gpu_data();
+    d.cls = dense_params_ ? nullptr : cls_.gpu_data();
     d.blocked = blocked_.gpu_data();
     d.V = V_.mutable_gpu_data();
     d.tcnt = tcnt_.mutable_gpu_data();

@@ -248,6 +248,8 @@ public:
     void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
 
     InstDev dev(uint32_t tile_base);
+    // Distinct parameter records (classes); n() when stored per neuron.
+    uint32_t param_classes() const { return (uint32_t)pA_.count(); }
     SampleRecDev sample_rec();
     // Runs of consecutive synapses (of one row) whose targets share a tile of
     // 2^kTileBits neurons, per tile: the exact queue bound of the binning.
@@ -273,7 +275,10 @@ private:
     std::vector<float> ou_mean_, ou_sigma_, ou_tau_;
     std::vector<uint32_t> samples_;
 
-    DataAllocator<float4> pA_, pB_, gU_, vU_, dec_, J_;
+    DataAllocator<float4> pA_, pB_, gU_, vU_, dec_;    // parameter records per class
+    DataAllocator<uint16_t> cls_;                      // parameter class per neuron (empty if dense)
+    bool dense_params_ = false;                        // one class per neuron
+    DataAllocator<float4> J_;
     DataAllocator<uint8_t> blocked_, flag_;
     DataAllocator<float> V_, iou_;
     DataAllocator<int32_t> tcnt_;

@@ -51,6 +51,12 @@ __device__ __forceinline__ uint32_t qitem_len(QItem q)
     return (uint32_t)(q >> kQBeginBits);
 }
 
+// Row of the parameter records that holds neuron li's parameters.
+__device__ __forceinline__ uint32_t param_index(const InstDev &I, uint32_t li)
+{
+    return I.cls != nullptr ? (uint32_t)I.cls[li] : li;
+}
+
 // synaptic_current.cpp:39-43 in source order, no contraction.
 __device__ __forceinline__ float syn_current(const float4 g, const float4 vu, const float4 J, const float V)
 {
@@ -170,7 +176,7 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
 // synapses are ordered by target, so they split into runs whose targets lie
 // in one tile; every run is appended (first synapse, length) to its tile's
 // queue.  Called by a whole wave with wave-uniform arguments.  Lane l holds
-// synapses c0 + 8l .. c0 + 8l + 7 of the row (512 per pass).  A run ends where
+// synapses c0 + 4l .. c0 + 4l + 3 of the row (256 per pass).  A run ends where
 // the next synapse's tile differs; its start is the last run start at or
 // before it (within the lane, else from a wave max-scan of the lanes' last
 // starts, else carried from the previous pass).
@@ -179,7 +185,7 @@ __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, in
                                         uint32_t *__restrict__ qcnt, QItem *__restrict__ queue,
                                         StepCtl *__restrict__ ctl, uint32_t lane)
 {
-    constexpr uint32_t K = 8;
+    constexpr uint32_t K = 4;
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const uint32_t *__restrict__ col = col_all + row_begin;
     uint32_t carry_tile = 0xfffffffcu;   // tile of the synapse before the pass
@@ -288,11 +294,12 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     int32_t t = 0;
     uint32_t fl = 0;
     if (active) {
-        pa = I.pA[li];
-        pb = I.pB[li];
-        g = I.gU[li];
-        vu = I.vU[li];
-        dc = I.dec[li];
+        const uint32_t c = param_index(I, li);
+        pa = I.pA[c];
+        pb = I.pB[c];
+        g = I.gU[c];
+        vu = I.vU[c];
+        dc = I.dec[c];
         J = I.J[li];
         V = I.V[li];
         t = I.tcnt[li];
@@ -464,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
         const uint32_t li = d.begin + tid;
         const uint32_t fl = I.flag[li];
         if (!(fl & kFlagFolded)) {
-            I.J[li] = fold_delta(I.J[li], I.dec[li], acc_to_delta(acc, tid));
+            I.J[li] = fold_delta(I.J[li], I.dec[param_index(I, li)], acc_to_delta(acc, tid));
             I.flag[li] = (uint8_t)(fl | kFlagFolded);
         }
     }
@@ -499,8 +506,9 @@ __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ i
     if (R.v)
         R.v[o] = V;
     if (R.isyn) {
-        const float4 J = fold_delta(I.J[li], I.dec[li], I.dlt[li]);   // flag bit 2 is clear here
-        R.isyn[o] = syn_current(I.gU[li], I.vU[li], J, V);
+        const uint32_t c = param_index(I, li);
+        const float4 J = fold_delta(I.J[li], I.dec[c], I.dlt[li]);   // flag bit 2 is clear here
+        R.isyn[o] = syn_current(I.gU[c], I.vU[c], J, V);
     }
     if (R.iou)
         R.iou[o] = I.iou[li];

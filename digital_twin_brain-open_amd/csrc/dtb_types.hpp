@@ -51,13 +51,22 @@ constexpr uint64_t kQBeginMask = (1ull << kQBeginBits) - 1ull;
 constexpr int64_t kMaxSynapses = (int64_t)1 << kQBeginBits;
 constexpr int64_t kMaxRowLength = (int64_t)1 << (64 - kQBeginBits);
 
+// Largest number of distinct parameter records kept as classes (2-byte class
+// index per neuron); beyond it the records are stored per neuron.
+constexpr size_t kMaxParamClasses = 65536;
+
 // Everything a kernel needs about one instance.
 struct alignas(16) InstDev {
-    // parameters, float4 per neuron:
+    // Parameter records, one float4 each, indexed by parameter class:
     //   pA = (v_reset, v_th, 1/C, v_leak)
     //   pB = (g_leak, i_ext, noise_rate, bits(refractory substeps))
     //   gU, vU, dec = (AMPA, NMDA, GABA_A, GABA_B)
+    // Neurons with bit-identical records share a class (in practice one per
+    // population), so the step reads 2 bytes of class per neuron instead of
+    // 80 bytes of parameters.  cls == nullptr: one class per neuron (class =
+    // neuron index), used when the records have more than 65536 values.
     const float4 *pA, *pB, *gU, *vU, *dec;
+    const uint16_t *cls;        // [n] parameter class, or nullptr
     const uint8_t *blocked;     // excluded from population statistics
     // state
     float *V;

@@ -252,6 +252,39 @@ def test_property_updates_match_oracle(dtb, orc, gpu):
     assert_state_equal(st, sim)
 
 
+def test_parameter_classes_and_per_neuron_records(dtb, orc, gpu):
+    """Parameters shared by populations (a few records) and parameters that
+    differ on every neuron (more records than the class index holds, kept per
+    neuron) give the oracle's results; switching from one layout to the other
+    between runs, with spikes still pending, changes nothing either."""
+    n, seed = 70000, 5
+    net = dtb.gen_uniform(n, 100, seed=14)
+    rng = np.random.default_rng(3)
+    vth = (-50.0 + rng.uniform(-0.5, 0.5, n)).astype(np.float32)
+    gamp = np.where(net.props[:, 3] == 0, 0.011, 0.009).astype(np.float32)
+
+    blk = dtb.Block(net, delta_t=DT, seed=seed)
+    blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 1.5, 0.1)
+    blk.assign_property_by_subblk([0, 1], [7, 7], [0.011, 0.009])   # two records
+    a = blk.run(100)
+    blk.update_property(np.arange(n, dtype=np.uint32), np.full(n, 5, np.uint32), vth)   # 70000 records
+    b = blk.run(100)
+    st = blk.state()
+    blk.engine.close()
+
+    on = orc.OracleNet(net.props.copy(), net.rowptr, net.col, net.w, net.kind_per_synapse(), dt=DT, seed=seed)
+    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
+    sim = orc.Sim(on, 32)
+    on.props[:, 10] = gamp
+    ra = sim.run(100, pop_counts=True)
+    on.props[:, 8] = vth
+    rb = sim.run(100, pop_counts=True)
+    assert np.array_equal(a["freq"], ra["freq"])
+    assert np.array_equal(b["freq"], rb["freq"])
+    assert_state_equal(st, sim)
+    assert rb["freq"].sum() > 100
+
+
 def test_fp64_oracle_v_within_1e5(dtb, orc, gpu):
     """V of the GPU fp32 path within 1e-5 relative of the fp64 oracle driven
     by the same spike raster; every spike decision the fp64 model disagrees
