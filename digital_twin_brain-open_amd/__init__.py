@@ -7,7 +7,9 @@ putting the repository root on ``sys.path``.
 """
 from ._lib import DtbError, load, EXPORTED, LIB_PATH  # noqa: F401
 from .engine import (Block, Engine, NetworkArrays, gen_uniform, gen_voxel, device_count, ou_normals,  # noqa: F401
-                     COLUMN_TO_PROP, DEFAULT_G_UI, NCOL)
+                     load_npz, save_npz, COLUMN_TO_PROP, DEFAULT_G_UI, NCOL)
+from .blockwrapper import BlockWrapper  # noqa: F401
 
 __all__ = ["Block", "Engine", "NetworkArrays", "gen_uniform", "gen_voxel", "device_count", "ou_normals",
+           "load_npz", "save_npz", "BlockWrapper",
            "DtbError", "load", "EXPORTED", "LIB_PATH", "COLUMN_TO_PROP", "DEFAULT_G_UI", "NCOL"]

@@ -31,7 +31,16 @@ RUN_SAMPLE_SPIKE = 1 << 2
 RUN_SAMPLE_VMEMB = 1 << 3
 RUN_SAMPLE_ISYN = 1 << 4
 RUN_SAMPLE_IOU = 1 << 5
+RUN_VMEAN = 1 << 6
+RUN_IMEAN = 1 << 7
 RUN_GRAPH = 1 << 8
+RUN_RECEPTOR_IMEAN = 1 << 9
+
+OK = 0
+INVALID_PARAMETER = 1
+UNSET_SAMPLE_FAILURE = 2
+UNSUPPORTED = 3
+INVALID_NETWORK = 5
 
 
 class DtbError(RuntimeError):
@@ -61,6 +70,12 @@ class RunOutputs(ctypes.Structure):
         ("sample_vmemb", c_void_p),
         ("sample_isynaptic", c_void_p),
         ("sample_iou", c_void_p),
+        ("vmean", c_void_p),
+        ("imean", c_void_p),
+        ("ampa_imean", c_void_p),
+        ("nmda_imean", c_void_p),
+        ("gabaa_imean", c_void_p),
+        ("gabab_imean", c_void_p),
     ]
 
 
@@ -94,6 +109,8 @@ _SIGS = {
                                 c_void_p, c_void_p]),
     "dtb_gen_voxel": (c_int, [c_int64, c_uint32, c_uint32, c_float, c_float, c_uint64, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dtb_npz_size": (c_int, [c_char_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)]),
+    "dtb_npz_read": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(sorted(_SIGS))

@@ -9,6 +9,7 @@
 #include "dtb_engine.hpp"
 #include "dtb_gen.hpp"
 #include "dtb_kernels.hpp"
+#include "dtb_npz.hpp"
 
 struct dtb_engine_s {
     std::unique_ptr<dtb::Engine> eng;
@@ -211,15 +212,39 @@ int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t f
         const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
         const bool samples = want_samples && out &&
                              (out->sample_spike || out->sample_vmemb || out->sample_isynaptic || out->sample_iou);
-        I->begin_record(iters, freq, samples, e.stream());
+        // Per-population means (RunRequest output_vmean / output_imean /
+        // output_receptor_imeas, snn.proto:63-77).
+        uint32_t stat_mask = 0;
+        float *means[dtb::kStatFields] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (out) {
+            if ((flags & DTB_RUN_VMEAN) && out->vmean) {
+                stat_mask |= dtb::kStatV;
+                means[0] = out->vmean;
+            }
+            if ((flags & DTB_RUN_IMEAN) && out->imean) {
+                stat_mask |= dtb::kStatI;
+                means[1] = out->imean;
+            }
+            if ((flags & DTB_RUN_RECEPTOR_IMEAN) && out->ampa_imean && out->nmda_imean && out->gabaa_imean &&
+                out->gabab_imean) {
+                stat_mask |= dtb::kStatReceptor;
+                means[2] = out->ampa_imean;
+                means[3] = out->nmda_imean;
+                means[4] = out->gabaa_imean;
+                means[5] = out->gabab_imean;
+            }
+        }
+        I->begin_record(iters, freq, samples, stat_mask, e.stream());
         e.mark_dirty();
         e.run(iters, (flags & DTB_RUN_GRAPH) ? 64 : 0);
+        if (stat_mask & (dtb::kStatI | dtb::kStatReceptor))
+            e.stat_tail();
         e.sync();
         if (out)
             I->fetch_record(freq ? out->freq : nullptr, (flags & DTB_RUN_SAMPLE_SPIKE) ? out->sample_spike : nullptr,
                             (flags & DTB_RUN_SAMPLE_VMEMB) ? out->sample_vmemb : nullptr,
                             (flags & DTB_RUN_SAMPLE_ISYN) ? out->sample_isynaptic : nullptr,
-                            (flags & DTB_RUN_SAMPLE_IOU) ? out->sample_iou : nullptr, e.stream());
+                            (flags & DTB_RUN_SAMPLE_IOU) ? out->sample_iou : nullptr, means, e.stream());
         I->end_record();
         e.mark_dirty();
     });
@@ -348,6 +373,23 @@ int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float 
         need(g_ui && props && rowptr && colinds && weights && connkinds, "output arrays are required");
         dtb::gen_voxel(n, nvox, k, intra_frac, density, seed, g_ui, props, rowptr, colinds, weights, connkinds,
                        vox_sizes);
+    });
+}
+
+int dtb_npz_size(const char *path, int64_t *n, int64_t *nsyn, int32_t *nblocks)
+{
+    return guarded([&] {
+        need(path != nullptr, "path is required");
+        dtb::npz_block_size(path, n, nsyn, nblocks);
+    });
+}
+
+int dtb_npz_read(const char *path, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
+                 uint32_t *connkinds, int64_t *block_begin)
+{
+    return guarded([&] {
+        need(path && props && rowptr && colinds && weights && connkinds, "output arrays are required");
+        dtb::npz_block_read(path, props, rowptr, colinds, weights, connkinds, block_begin);
     });
 }
 

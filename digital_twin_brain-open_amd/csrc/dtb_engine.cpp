@@ -156,7 +156,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
     pops_.resize(np);
     step_.resize(2);
-    counters_.resize(2);
+    counters_.resize(2 * (size_t)((n_ + kWave - 1) / kWave));
 
     derive_params(s);
     init_state(s);
@@ -474,7 +474,7 @@ bool Instance::configure_run(int32_t t_steps, bool use_ou, hipStream_t s)
     return changed;
 }
 
-void Instance::begin_record(int64_t steps, bool freq, bool samples, hipStream_t s)
+void Instance::begin_record(int64_t steps, bool freq, bool samples, uint32_t stat_mask, hipStream_t s)
 {
     if (samples && samples_.empty())
         throw Error(kUnsetSampleFailure, "samples requested but not set");
@@ -482,6 +482,11 @@ void Instance::begin_record(int64_t steps, bool freq, bool samples, hipStream_t 
     rec_len_ = steps;
     rec_freq_ = freq;
     rec_samples_ = samples;
+    rec_stat_mask_ = stat_mask;
+    if (stat_mask != 0) {
+        rec_stat_.resize((size_t)steps * npop() * kStatFields);
+        DTB_HIP(hipMemsetAsync(rec_stat_.mutable_gpu_data(), 0, rec_stat_.size(), s));
+    }
     if (freq) {
         rec_counts_.resize((size_t)steps * npop());
         DTB_HIP(hipMemsetAsync(rec_counts_.mutable_gpu_data(), 0, rec_counts_.size(), s));
@@ -502,11 +507,33 @@ void Instance::begin_record(int64_t steps, bool freq, bool samples, hipStream_t 
 void Instance::end_record()
 {
     rec_freq_ = rec_samples_ = false;
+    rec_stat_mask_ = 0;
     rec_len_ = 0;
 }
 
-void Instance::fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, hipStream_t s)
+void Instance::fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, float *const *means,
+                            hipStream_t s)
 {
+    if (means != nullptr && rec_stat_mask_ != 0) {
+        // stat_result.cpp:266-391 divides by the population size minus its
+        // blocked neurons: mean = (double) sum * 2^-24 / count, as float.
+        std::vector<long long> sums(rec_stat_.count());
+        rec_stat_.download(sums.data(), sums.size(), s);
+        DTB_HIP(hipStreamSynchronize(s));
+        const uint32_t np = npop();
+        std::vector<double> cnt(np, 0.0);
+        for (uint32_t q = 0; q < np; ++q)
+            for (uint32_t i = pop_begin_[q]; i < pop_begin_[q + 1]; ++i)
+                cnt[q] += props_[(size_t)i * kNumCols + kColBlocked] != 0.0f ? 0.0 : 1.0;
+        for (int k = 0; k < kStatFields; ++k) {
+            if (means[k] == nullptr)
+                continue;
+            for (int64_t r = 0; r < rec_len_; ++r)
+                for (uint32_t q = 0; q < np; ++q)
+                    means[k][(size_t)r * np + q] =
+                        (float)((double)sums[((size_t)r * np + q) * kStatFields + k] * 0x1p-24 / cnt[q]);
+        }
+    }
     if (freq && rec_freq_)
         rec_counts_.download(freq, rec_counts_.count(), s);
     if (rec_samples_) {
@@ -539,13 +566,18 @@ void Instance::set_step(int64_t step, hipStream_t s)
 
 void Instance::counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s)
 {
-    unsigned long long c[2] = {0, 0};
+    std::vector<unsigned long long> w(counters_.count());
     int64_t st[2] = {0, 0};
-    counters_.download(c, 2, s);
+    counters_.download(w.data(), w.size(), s);
     step_.download(st, 2, s);
     DTB_HIP(hipStreamSynchronize(s));
     if (step)
         *step = std::max(st[0], st[1]);
+    unsigned long long c[2] = {0, 0};
+    for (size_t k = 0; k < w.size(); k += 2) {
+        c[0] += w[k];
+        c[1] += w[k + 1];
+    }
     if (events)
         *events = c[0];
     if (spikes)
@@ -667,8 +699,9 @@ InstDev Instance::dev(uint32_t tile_base)
     d.w = w_.gpu_data();
     d.pops = pops_.gpu_data();
     d.rec_counts = (rec_freq_ && rec_len_ > 0) ? rec_counts_.mutable_gpu_data() : nullptr;
-    d.events = counters_.mutable_gpu_data();
-    d.spikes = counters_.mutable_gpu_data() + 1;
+    d.rec_stat = (rec_stat_mask_ != 0 && rec_len_ > 0) ? rec_stat_.mutable_gpu_data() : nullptr;
+    d.stat_mask = rec_stat_mask_;
+    d.wave_ctr = counters_.mutable_gpu_data();
     d.step = step_.mutable_gpu_data();
     d.seed = seed_;
     d.rec_step0 = rec_step0_;

@@ -34,6 +34,7 @@ enum StatusCode : int {
     kHipError = 100,
     kOutOfMemory = 101,
     kNoDevice = 102,
+    kInternal = 103,
 };
 
 struct Error : std::runtime_error {
@@ -229,10 +230,15 @@ public:
 
     // Configure t_steps / OU mode for the next run (recomputes substep decays).
     bool configure_run(int32_t t_steps, bool use_ou, hipStream_t s);
-    // Recording window of `steps` rows starting at the current step.
-    void begin_record(int64_t steps, bool freq, bool samples, hipStream_t s);
+    // Recording window of `steps` rows starting at the current step;
+    // stat_mask selects per-population means (kStatV | kStatI | kStatReceptor).
+    void begin_record(int64_t steps, bool freq, bool samples, uint32_t stat_mask, hipStream_t s);
     void end_record();
-    void fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, hipStream_t s);
+    uint32_t stat_mask() const { return rec_stat_mask_; }
+    // means: [6] arrays [steps][npop] (vmean, imean, AMPA, NMDA, GABA_A,
+    // GABA_B), any may be null.
+    void fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, float *const *means,
+                      hipStream_t s);
 
     // Reference-visible state after the last completed step: V, refractory
     // counter, spike flag, emitted flag, i_ou, J[4] (pending delta folded
@@ -288,8 +294,10 @@ private:
     DataAllocator<float2> w_;
     DataAllocator<PopDev> pops_;
     DataAllocator<int64_t> step_;                   // [2]
-    DataAllocator<unsigned long long> counters_;   // events, spikes
+    DataAllocator<unsigned long long> counters_;   // per wave: events, spikes
     DataAllocator<uint32_t> rec_counts_;
+    DataAllocator<long long> rec_stat_;            // [steps][npop][kStatFields]
+    uint32_t rec_stat_mask_ = 0;
     DataAllocator<uint8_t> rec_spike_;
     DataAllocator<float> rec_v_, rec_isyn_, rec_iou_;
     int64_t rec_step0_ = 0;
@@ -326,6 +334,9 @@ public:
     void materialize();
     // Drops the pending delta of one instance (checkpoint restore).
     void clear_queues(int id);
+    // Records the I_syn-based statistics of the last step run (imean,
+    // receptor imeans), which the next step's launch would otherwise take.
+    void stat_tail();
     // Folds every pending delta into J with the current tables and
     // parameters.  Must run before parameters, samples, recording or the
     // instance set change (a table rebuild empties the queues).

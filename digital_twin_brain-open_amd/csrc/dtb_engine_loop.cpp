@@ -262,6 +262,18 @@ void Engine::clear_queues(int id)
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 
+void Engine::stat_tail()
+{
+    DTB_HIP(hipSetDevice(device_));
+    if (dirty_)
+        rebuild();
+    if (ninst_ == 0)
+        return;
+    DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
+    DTB_HIP(launch_stat_tail(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), par_, stream_));
+    DTB_HIP(hipStreamSynchronize(stream_));
+}
+
 void Engine::run_timed(int64_t steps, float *ms_step)
 {
     DTB_HIP(hipSetDevice(device_));

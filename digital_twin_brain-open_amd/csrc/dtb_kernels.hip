@@ -162,6 +162,53 @@ __device__ __forceinline__ void accumulate_tile(unsigned long long *acc, RunScra
     }
 }
 
+// A statistic's value as the exact integer rint(x * 2^24).
+__device__ __forceinline__ long long stat_q(float x)
+{
+    return (long long)rintf(x * 0x1p24f);
+}
+
+// Receptor currents g_u (V_u - V) J_u (synaptic_current.cpp:39-42 terms).
+__device__ __forceinline__ float4 receptor_currents(const float4 g, const float4 vu, const float4 J, const float V)
+{
+    float4 r;
+    r.x = g.x * (vu.x - V) * J.x;
+    r.y = g.y * (vu.y - V) * J.y;
+    r.z = g.z * (vu.z - V) * J.z;
+    r.w = g.w * (vu.w - V) * J.w;
+    return r;
+}
+
+// Adds this wave's per-population sums of the statistics selected by kmask
+// (bit k: field k of kStatFields) of the lanes with `valid` into
+// row[pop * kStatFields + k]: one 64-bit atomic per population present in
+// the wave and field.  Called by the whole wave.
+__device__ __forceinline__ void wave_pop_stats(long long *row, bool valid, uint32_t p, const long long *v,
+                                               uint32_t kmask, uint32_t lane)
+{
+    unsigned long long rest = __ballot(valid);
+    while (rest != 0ull) {
+        const int src = __ffsll((long long)rest) - 1;
+        const uint32_t ps = __shfl(p, src, kWave);
+        const bool mine = valid && p == ps;
+        rest &= ~__ballot(mine);
+#pragma unroll
+        for (int k = 0; k < kStatFields; ++k) {
+            if (!(kmask & (1u << k)))
+                continue;
+            const unsigned long long s = wave_sum_u64(mine ? (unsigned long long)v[k] : 0ull);
+            if ((int)lane == src)
+                atomicAdd(reinterpret_cast<unsigned long long *>(row + (size_t)ps * kStatFields + k), s);
+        }
+    }
+}
+
+// Field mask of wave_pop_stats for the I_syn-based statistics of stat_mask.
+__device__ __forceinline__ uint32_t current_fields(uint32_t stat_mask)
+{
+    return ((stat_mask & kStatI) ? 2u : 0u) | ((stat_mask & kStatReceptor) ? 0x3cu : 0u);
+}
+
 __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, uint32_t i)
 {
     float4 r;
@@ -320,11 +367,22 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
 
     bool emit = false;
     uint32_t p = d.pop0;
+    long long stv[kStatFields];   // statistics of this neuron (when recorded)
     if (active) {
         if (!(fl & kFlagFolded))
             J = fold_delta(J, dc, acc_to_delta(acc, tid));
         uint32_t f = fl & kFlagSpike;
         float isyn = syn_current(g, vu, J, V);
+        if (I.rec_stat != nullptr && current_fields(I.stat_mask) != 0u) {
+            // I_syn and receptor currents at the end of the previous step
+            // (J folded, V before this step's update).
+            const float4 rc = receptor_currents(g, vu, J, V);
+            stv[1] = stat_q(isyn);
+            stv[2] = stat_q(rc.x);
+            stv[3] = stat_q(rc.y);
+            stv[4] = stat_q(rc.z);
+            stv[5] = stat_q(rc.w);
+        }
 
         while (p + 1 < I.npop && li >= I.pops[p].end)
             ++p;
@@ -378,6 +436,20 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         I.flag[li] = (uint8_t)(f | (emit ? kFlagEmit : 0u));
         I.iou[li] = iou;
         I.J[li] = J;
+        stv[0] = stat_q(V);
+    }
+
+    if (I.rec_stat != nullptr) {
+        // stat_result.cpp:266-391: V of this step, I_syn-based statistics of
+        // the previous step (its J is complete only now), blocked neurons
+        // excluded.
+        const bool valid = active && !I.blocked[li];
+        const int64_t row_v = step - I.rec_step0;
+        if ((I.stat_mask & kStatV) && row_v >= 0 && row_v < (int64_t)I.rec_len)
+            wave_pop_stats(I.rec_stat + (size_t)row_v * I.npop * kStatFields, valid, p, stv, 1u, lane);
+        const uint32_t cf = current_fields(I.stat_mask);
+        if (cf != 0u && row_v - 1 >= 0 && row_v - 1 < (int64_t)I.rec_len)
+            wave_pop_stats(I.rec_stat + (size_t)(row_v - 1) * I.npop * kStatFields, valid, p, stv, cf, lane);
     }
 
     // Spike propagation (presynaptic_voltage.cpp:122-225): every wave queues
@@ -391,16 +463,29 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         if (emit) {
             rb = I.rowptr[li];
             len = (uint32_t)(I.rowptr[li + 1] - rb);
-            if (I.rec_counts != nullptr && !I.blocked[li]) {
-                const int64_t row = step - I.rec_step0;
-                if (row >= 0 && row < (int64_t)I.rec_len)
-                    atomicAdd(&I.rec_counts[(size_t)row * I.npop + p], 1u);
-            }
         }
         const unsigned long long ev = wave_sum_u64((unsigned long long)len);
         if ((int)lane == leader) {
-            atomicAdd(I.events, ev);
-            atomicAdd(I.spikes, (unsigned long long)cnt_w);
+            unsigned long long *c = I.wave_ctr + 2u * ((d.begin >> 6) + w);
+            c[0] += ev;
+            c[1] += (unsigned long long)cnt_w;
+        }
+        if (I.rec_counts != nullptr) {
+            // stat_result.cpp:161-206: one atomic per population present in
+            // the wave (blocked neurons are not counted)
+            const int64_t row = step - I.rec_step0;
+            if (row >= 0 && row < (int64_t)I.rec_len) {
+                const bool counted = emit && !I.blocked[li];
+                unsigned long long rest = __ballot(counted);
+                while (rest != 0ull) {
+                    const int src = __ffsll((long long)rest) - 1;
+                    const uint32_t ps = __shfl(p, src, kWave);
+                    const unsigned long long same = __ballot(counted && p == ps);
+                    if ((int)lane == src)
+                        atomicAdd(&I.rec_counts[(size_t)row * I.npop + ps], (uint32_t)__popcll(same));
+                    rest &= ~same;
+                }
+            }
         }
         unsigned long long rest = m;
         while (rest != 0ull) {
@@ -479,6 +564,49 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
         *cnt_p = 0u;
 }
 
+// I_syn-based statistics (imean, receptor imeans) of the last step run: the
+// k_step launch of the next step would record them, so after the last step
+// they are taken here from the materialised pending delta.  `par` is the
+// parity of the next launch (its step slot holds the next step index).
+__global__ __launch_bounds__(kBlock) void k_stat_tail(const TileDesc *__restrict__ tiles,
+                                                      const InstDev *__restrict__ insts,
+                                                      uint32_t par)
+{
+    const TileDesc d = tiles[blockIdx.x];
+    const InstDev &I = insts[d.inst];
+    const uint32_t cf = current_fields(I.stat_mask);
+    if (I.rec_stat == nullptr || cf == 0u)
+        return;
+    const int64_t row = I.step[par] - 1 - I.rec_step0;
+    if (row < 0 || row >= (int64_t)I.rec_len)
+        return;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t li = d.begin + tid;
+    const bool active = tid < d.count;
+    long long stv[kStatFields];
+    uint32_t p = d.pop0;
+    bool valid = false;
+    if (active) {
+        while (p + 1 < I.npop && li >= I.pops[p].end)
+            ++p;
+        const uint32_t c = param_index(I, li);
+        const float V = I.V[li];
+        float4 J = I.J[li];
+        if (!(I.flag[li] & kFlagFolded))
+            J = fold_delta(J, I.dec[c], I.dlt[li]);
+        const float4 g = I.gU[c], vu = I.vU[c];
+        const float4 rc = receptor_currents(g, vu, J, V);
+        stv[1] = stat_q(syn_current(g, vu, J, V));
+        stv[2] = stat_q(rc.x);
+        stv[3] = stat_q(rc.y);
+        stv[4] = stat_q(rc.z);
+        stv[5] = stat_q(rc.w);
+        valid = !I.blocked[li];
+    }
+    wave_pop_stats(I.rec_stat + (size_t)row * I.npop * kStatFields, valid, p, stv, cf, lane);
+}
+
 // stat_result.cpp:393-433 (sampled spike flag, V, I_syn, i_ou) of the step
 // just run by the k_step launch of parity `par`.  Runs after k_materialize
 // (dlt holds that step's delta, folded in exactly as the next k_step will
@@ -550,6 +678,15 @@ hipError_t launch_settle(const TileDesc *tiles, uint32_t ntiles, const InstDev *
     if (ntiles == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_settle, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, set);
+    return hipGetLastError();
+}
+
+hipError_t launch_stat_tail(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, uint32_t par,
+                            hipStream_t stream)
+{
+    if (ntiles == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_stat_tail, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, par);
     return hipGetLastError();
 }
 

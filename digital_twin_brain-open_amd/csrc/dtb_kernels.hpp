@@ -19,6 +19,11 @@ hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const Inst
 hipError_t launch_settle(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
                          uint32_t set, hipStream_t stream);
 
+// I_syn-based statistics of the last step run into the recording (after
+// launch_materialize of the pending set); `par` = parity of the next launch.
+hipError_t launch_stat_tail(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, uint32_t par,
+                            hipStream_t stream);
+
 // Samples of the step run by the k_step launch of parity `par`.
 hipError_t launch_sample(const InstDev *insts, const SampleDev *samples, uint32_t nsamples,
                          const SampleRecDev *recs, uint32_t par, hipStream_t stream);

@@ -1,0 +1,572 @@
+// dtb_npz.cpp -- reader of the reference's block files.
+//
+// A block file is a NumPy .npz archive written by python/make_block.py:608-614
+// (np.savez) with the arrays
+//   property             float32 [N][22]   (src/configuration.cpp:1778-2156)
+//   output_neuron_idx    uint32  [C]       postsynaptic neuron, block-local
+//   input_block_idx      integer [C]       presynaptic block, relative to this block
+//   input_neuron_idx     uint32  [C]       presynaptic neuron, local to its block
+//   input_channel_offset uint8   [C]       0: AMPA/NMDA pair, 2: GABA_A/GABA_B pair
+//   weight               [C][2]            float64 / float32 / float16 / uint8
+// (read by src/configuration.cpp:1079-1667).  Weights are converted to fp32
+// as the reference's propagation does (presynaptic_voltage.cpp:153-176:
+// double and half cast, uint8 divided by 255).
+//
+// The archive is memory-mapped; stored members are read in place, deflated
+// members are inflated with zlib.  Zip64 records are supported (members over
+// 4 GiB).
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <zlib.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dtb_engine.hpp"
+#include "dtb_npz.hpp"
+
+namespace dtb {
+
+namespace {
+
+uint16_t rd16(const uint8_t *p) { return (uint16_t)(p[0] | (p[1] << 8)); }
+uint32_t rd32(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24); }
+uint64_t rd64(const uint8_t *p) { return (uint64_t)rd32(p) | ((uint64_t)rd32(p + 4) << 32); }
+
+[[noreturn]] void bad(const std::string &file, const std::string &what)
+{
+    throw Error(kInvalidNetwork, file + ": " + what);
+}
+
+// Read-only memory map of a whole file.
+class MappedFile {
+public:
+    explicit MappedFile(const std::string &path) : path_(path)
+    {
+        fd_ = ::open(path.c_str(), O_RDONLY);
+        if (fd_ < 0)
+            throw Error(kInvalidParameter, "cannot open " + path);
+        struct stat st {};
+        if (fstat(fd_, &st) != 0 || st.st_size <= 0) {
+            ::close(fd_);
+            throw Error(kInvalidParameter, "cannot stat or empty file " + path);
+        }
+        size_ = (size_t)st.st_size;
+        void *m = mmap(nullptr, size_, PROT_READ, MAP_PRIVATE, fd_, 0);
+        if (m == MAP_FAILED) {
+            ::close(fd_);
+            throw Error(kInvalidParameter, "cannot map " + path);
+        }
+        data_ = static_cast<const uint8_t *>(m);
+    }
+    ~MappedFile()
+    {
+        munmap(const_cast<uint8_t *>(data_), size_);
+        ::close(fd_);
+    }
+    MappedFile(const MappedFile &) = delete;
+    MappedFile &operator=(const MappedFile &) = delete;
+    const uint8_t *data() const { return data_; }
+    size_t size() const { return size_; }
+    const std::string &path() const { return path_; }
+
+private:
+    std::string path_;
+    int fd_ = -1;
+    const uint8_t *data_ = nullptr;
+    size_t size_ = 0;
+};
+
+// One array of an archive: dtype descriptor, shape and its bytes (in the map
+// for stored members, owned for deflated ones).
+struct NpyArray {
+    std::string descr;
+    std::vector<uint64_t> shape;
+    const uint8_t *data = nullptr;
+    uint64_t nbytes = 0;
+    std::vector<uint8_t> owned;
+
+    uint64_t count() const
+    {
+        uint64_t c = 1;
+        for (uint64_t d : shape)
+            c *= d;
+        return c;
+    }
+    char kind() const { return descr.size() >= 2 ? descr[1] : '?'; }
+    int itemsize() const { return descr.size() >= 3 ? std::atoi(descr.c_str() + 2) : 0; }
+};
+
+class NpzArchive {
+public:
+    explicit NpzArchive(const std::string &path) : file_(path) { read_directory(); }
+
+    bool has(const std::string &name) const { return members_.count(name + ".npy") != 0; }
+
+    // header_only: shape and dtype without touching (or inflating) the data.
+    NpyArray get(const std::string &name, bool header_only) const
+    {
+        auto it = members_.find(name + ".npy");
+        if (it == members_.end())
+            bad(file_.path(), "missing array '" + name + "'");
+        const Member &m = it->second;
+        const uint8_t *base = file_.data();
+        if (m.local + 30 > file_.size() || rd32(base + m.local) != 0x04034b50u)
+            bad(file_.path(), "bad local header of " + name);
+        const uint64_t off = m.local + 30 + rd16(base + m.local + 26) + rd16(base + m.local + 28);
+        if (off + m.csize > file_.size())
+            bad(file_.path(), "truncated member " + name);
+        NpyArray a;
+        const uint8_t *blob = base + off;
+        uint64_t blob_size = m.usize;
+        if (m.method == 8) {
+            // Deflated: inflate enough for the header, or everything.
+            const uint64_t want = header_only ? std::min<uint64_t>(m.usize, 1u << 16) : m.usize;
+            a.owned.resize(want);
+            inflate_into(blob, m.csize, a.owned.data(), want, name);
+            blob = a.owned.data();
+            blob_size = want;
+        } else if (m.method != 0) {
+            bad(file_.path(), "unsupported compression of " + name);
+        }
+        const uint64_t hdr = parse_header(blob, blob_size, a, name);
+        const uint64_t expect = a.count() * (uint64_t)a.itemsize();
+        if (hdr + expect > m.usize)
+            bad(file_.path(), "array " + name + " shorter than its shape");
+        a.data = blob + hdr;
+        a.nbytes = expect;
+        if (header_only)
+            a.data = nullptr;
+        return a;
+    }
+
+    const std::string &path() const { return file_.path(); }
+
+private:
+    struct Member {
+        uint16_t method = 0;
+        uint64_t csize = 0, usize = 0, local = 0;
+    };
+
+    void read_directory()
+    {
+        const uint8_t *b = file_.data();
+        const size_t n = file_.size();
+        if (n < 22)
+            bad(file_.path(), "not a zip archive");
+        // End of central directory record (comment up to 64 KiB).
+        size_t eocd = SIZE_MAX;
+        for (size_t k = n - 22 + 1; k-- > 0 && n - k <= 22 + 65535;)
+            if (rd32(b + k) == 0x06054b50u) {
+                eocd = k;
+                break;
+            }
+        if (eocd == SIZE_MAX)
+            bad(file_.path(), "no end-of-central-directory record");
+        uint64_t entries = rd16(b + eocd + 10);
+        uint64_t cd_size = rd32(b + eocd + 12);
+        uint64_t cd_off = rd32(b + eocd + 16);
+        if (eocd >= 20 && rd32(b + eocd - 20) == 0x07064b50u) {
+            const uint64_t z64 = rd64(b + eocd - 20 + 8);
+            if (z64 + 56 > n || rd32(b + z64) != 0x06064b50u)
+                bad(file_.path(), "bad zip64 end record");
+            entries = rd64(b + z64 + 32);
+            cd_size = rd64(b + z64 + 40);
+            cd_off = rd64(b + z64 + 48);
+        }
+        if (cd_off + cd_size > n)
+            bad(file_.path(), "central directory out of range");
+        uint64_t p = cd_off;
+        for (uint64_t e = 0; e < entries; ++e) {
+            if (p + 46 > n || rd32(b + p) != 0x02014b50u)
+                bad(file_.path(), "bad central directory entry");
+            Member m;
+            m.method = rd16(b + p + 10);
+            m.csize = rd32(b + p + 20);
+            m.usize = rd32(b + p + 24);
+            const uint16_t nlen = rd16(b + p + 28), xlen = rd16(b + p + 30), clen = rd16(b + p + 32);
+            m.local = rd32(b + p + 42);
+            if (p + 46 + nlen + xlen + clen > n)
+                bad(file_.path(), "central directory entry out of range");
+            std::string name(reinterpret_cast<const char *>(b + p + 46), nlen);
+            // Zip64 extended information: the fields that are 0xffffffff, in order.
+            const uint8_t *x = b + p + 46 + nlen;
+            for (uint32_t q = 0; q + 4 <= xlen;) {
+                const uint16_t id = rd16(x + q), sz = rd16(x + q + 2);
+                if (id == 0x0001) {
+                    uint32_t r = q + 4;
+                    if (m.usize == 0xffffffffu && r + 8 <= q + 4u + sz) {
+                        m.usize = rd64(x + r);
+                        r += 8;
+                    }
+                    if (m.csize == 0xffffffffu && r + 8 <= q + 4u + sz) {
+                        m.csize = rd64(x + r);
+                        r += 8;
+                    }
+                    if (m.local == 0xffffffffu && r + 8 <= q + 4u + sz)
+                        m.local = rd64(x + r);
+                }
+                q += 4u + sz;
+            }
+            members_[name] = m;
+            p += 46 + nlen + xlen + clen;
+        }
+    }
+
+    void inflate_into(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t want, const std::string &name) const
+    {
+        z_stream z{};
+        if (inflateInit2(&z, -MAX_WBITS) != Z_OK)
+            throw Error(kInternal, "zlib init failed");
+        uint64_t in_done = 0, out_done = 0;
+        int rc = Z_OK;
+        while (out_done < want && rc != Z_STREAM_END) {
+            const uint64_t in_chunk = std::min<uint64_t>(csize - in_done, 1u << 30);
+            const uint64_t out_chunk = std::min<uint64_t>(want - out_done, 1u << 30);
+            z.next_in = const_cast<Bytef *>(src + in_done);
+            z.avail_in = (uInt)in_chunk;
+            z.next_out = dst + out_done;
+            z.avail_out = (uInt)out_chunk;
+            rc = inflate(&z, Z_NO_FLUSH);
+            in_done += in_chunk - z.avail_in;
+            out_done += out_chunk - z.avail_out;
+            if (rc != Z_OK && rc != Z_STREAM_END) {
+                inflateEnd(&z);
+                bad(file_.path(), "corrupt deflate data in " + name);
+            }
+            if (rc == Z_OK && z.avail_in == in_chunk && z.avail_out == out_chunk) {
+                inflateEnd(&z);
+                bad(file_.path(), "deflate stream of " + name + " stalled");
+            }
+        }
+        inflateEnd(&z);
+        if (out_done < want)
+            bad(file_.path(), "member " + name + " inflates short");
+    }
+
+    // .npy format 1.0 / 2.0 / 3.0 header; returns the offset of the data.
+    uint64_t parse_header(const uint8_t *p, uint64_t size, NpyArray &a, const std::string &name) const
+    {
+        if (size < 10 || std::memcmp(p, "\x93NUMPY", 6) != 0)
+            bad(file_.path(), name + " is not a .npy array");
+        const int major = p[6];
+        uint64_t hlen = 0, pre = 0;
+        if (major == 1) {
+            hlen = rd16(p + 8);
+            pre = 10;
+        } else {
+            if (size < 12)
+                bad(file_.path(), name + ": short header");
+            hlen = rd32(p + 8);
+            pre = 12;
+        }
+        if (pre + hlen > size)
+            bad(file_.path(), name + ": header out of range");
+        const std::string h(reinterpret_cast<const char *>(p + pre), hlen);
+        auto field = [&](const char *key) -> size_t {
+            const size_t k = h.find(key);
+            if (k == std::string::npos)
+                bad(file_.path(), name + ": header lacks " + key);
+            return h.find(':', k) + 1;
+        };
+        size_t k = h.find('\'', field("'descr'"));
+        const size_t e = h.find('\'', k + 1);
+        a.descr = h.substr(k + 1, e - k - 1);
+        k = field("'fortran_order'");
+        while (k < h.size() && h[k] == ' ')
+            ++k;
+        if (h.compare(k, 4, "True") == 0)
+            bad(file_.path(), name + ": Fortran-ordered arrays are not supported");
+        k = h.find('(', field("'shape'"));
+        const size_t close = h.find(')', k);
+        std::string dims = h.substr(k + 1, close - k - 1);
+        size_t q = 0;
+        while (q < dims.size()) {
+            while (q < dims.size() && (dims[q] == ' ' || dims[q] == ','))
+                ++q;
+            if (q >= dims.size())
+                break;
+            a.shape.push_back(std::strtoull(dims.c_str() + q, nullptr, 10));
+            while (q < dims.size() && dims[q] != ',')
+                ++q;
+        }
+        if (a.descr.size() < 3 || (a.descr[0] != '<' && a.descr[0] != '|' && a.descr[0] != '='))
+            bad(file_.path(), name + ": unsupported dtype " + a.descr);
+        return pre + hlen;
+    }
+
+    MappedFile file_;
+    std::map<std::string, Member> members_;
+};
+
+// Integer array element as int64 (any little-endian integer dtype).
+int64_t int_at(const NpyArray &a, uint64_t i)
+{
+    const uint8_t *p = a.data + i * (uint64_t)a.itemsize();
+    const bool sgn = a.kind() == 'i';
+    switch (a.itemsize()) {
+    case 1: return sgn ? (int64_t)(int8_t)p[0] : (int64_t)p[0];
+    case 2: return sgn ? (int64_t)(int16_t)rd16(p) : (int64_t)rd16(p);
+    case 4: return sgn ? (int64_t)(int32_t)rd32(p) : (int64_t)rd32(p);
+    default: return (int64_t)rd64(p);
+    }
+}
+
+void check_int(const NpzArchive &z, const NpyArray &a, const char *name)
+{
+    if ((a.kind() != 'i' && a.kind() != 'u') || a.shape.size() != 1)
+        bad(z.path(), std::string(name) + " must be a 1-d integer array");
+}
+
+// IEEE half -> float (exact).
+float half_to_float(uint16_t h)
+{
+    const uint32_t s = (uint32_t)(h >> 15) << 31;
+    const uint32_t e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = s;
+        } else {
+            float f = std::ldexp((float)m, -24);
+            std::memcpy(&bits, &f, 4);
+            bits |= s;
+        }
+    } else if (e == 31) {
+        bits = s | 0x7f800000u | (m << 13);
+    } else {
+        bits = s | ((e + 112) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+float weight_at(const NpyArray &a, uint64_t i)
+{
+    const uint8_t *p = a.data + i * (uint64_t)a.itemsize();
+    if (a.kind() == 'f' && a.itemsize() == 8) {
+        double d;
+        std::memcpy(&d, p, 8);
+        return (float)d;
+    }
+    if (a.kind() == 'f' && a.itemsize() == 4) {
+        float f;
+        std::memcpy(&f, p, 4);
+        return f;
+    }
+    if (a.kind() == 'f' && a.itemsize() == 2)
+        return half_to_float(rd16(p));
+    // uint8 / int8 block weights are the uchar2 code divided by 255
+    // (presynaptic_voltage.cpp:171-176).
+    return (float)p[0] / 255.0f;
+}
+
+bool is_dir(const std::string &p)
+{
+    struct stat st {};
+    return stat(p.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+bool is_file(const std::string &p)
+{
+    struct stat st {};
+    return stat(p.c_str(), &st) == 0 && S_ISREG(st.st_mode);
+}
+
+struct BlockShape {
+    uint64_t n = 0, conns = 0;
+};
+
+BlockShape block_shape(const NpzArchive &z)
+{
+    const NpyArray prop = z.get("property", true);
+    if (prop.descr != "<f4" || prop.shape.size() != 2 || prop.shape[1] != (uint64_t)kNumCols)
+        bad(z.path(), "property must be float32 [N][22]");
+    const NpyArray nid = z.get("input_neuron_idx", true);
+    BlockShape s;
+    s.n = prop.shape[0];
+    s.conns = nid.shape.empty() ? 0 : nid.shape[0];
+    return s;
+}
+
+}  // namespace
+
+std::vector<std::string> npz_block_files(const std::string &path)
+{
+    std::vector<std::string> files;
+    if (is_dir(path)) {
+        for (int b = 0;; ++b) {
+            const std::string f = path + "/block_" + std::to_string(b) + ".npz";
+            if (!is_file(f))
+                break;
+            files.push_back(f);
+        }
+        if (files.empty())
+            throw Error(kInvalidParameter, "no block_0.npz in " + path);
+    } else if (is_file(path)) {
+        files.push_back(path);
+    } else {
+        throw Error(kInvalidParameter, "no such file or directory: " + path);
+    }
+    return files;
+}
+
+void npz_block_size(const std::string &path, int64_t *n, int64_t *nsyn, int32_t *nblocks)
+{
+    const auto files = npz_block_files(path);
+    uint64_t tn = 0, ts = 0;
+    for (const auto &f : files) {
+        NpzArchive z(f);
+        const BlockShape s = block_shape(z);
+        tn += s.n;
+        ts += s.conns;
+    }
+    if (tn == 0 || tn > 0x7fffffffull)
+        throw Error(kInvalidNetwork, "block files hold no neurons or more than 2^31");
+    if (n)
+        *n = (int64_t)tn;
+    if (nsyn)
+        *nsyn = (int64_t)ts;
+    if (nblocks)
+        *nblocks = (int32_t)files.size();
+}
+
+void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
+                    uint32_t *connkinds, int64_t *block_begin)
+{
+    const auto files = npz_block_files(path);
+    const size_t B = files.size();
+    std::vector<uint64_t> base(B + 1, 0);
+    std::vector<uint64_t> conns(B, 0);
+    for (size_t b = 0; b < B; ++b) {
+        NpzArchive z(files[b]);
+        const BlockShape s = block_shape(z);
+        base[b + 1] = base[b] + s.n;
+        conns[b] = s.conns;
+    }
+    const uint64_t n = base[B];
+    uint64_t S = 0;
+    for (uint64_t c : conns)
+        S += c;
+    if (block_begin)
+        for (size_t b = 0; b <= B; ++b)
+            block_begin[b] = (int64_t)base[b];
+
+    // Pass 1: properties, validation and out-degree of every presynaptic neuron.
+    std::vector<int64_t> deg(n + 1, 0);
+    for (size_t b = 0; b < B; ++b) {
+        NpzArchive z(files[b]);
+        const NpyArray prop = z.get("property", false);
+        std::memcpy(props + base[b] * kNumCols, prop.data, prop.nbytes);
+        if (conns[b] == 0)
+            continue;
+        const NpyArray bid = z.get("input_block_idx", false), nid = z.get("input_neuron_idx", false);
+        const NpyArray oid = z.get("output_neuron_idx", false), ch = z.get("input_channel_offset", false);
+        const NpyArray w = z.get("weight", false);
+        check_int(z, bid, "input_block_idx");
+        check_int(z, nid, "input_neuron_idx");
+        check_int(z, oid, "output_neuron_idx");
+        check_int(z, ch, "input_channel_offset");
+        if (bid.count() != conns[b] || oid.count() != conns[b] || ch.count() != conns[b])
+            bad(files[b], "connection arrays differ in length");
+        if (w.shape.size() != 2 || w.shape[0] != conns[b] || w.shape[1] != 2)
+            bad(files[b], "weight must be [C][2]");
+        if (!((w.kind() == 'f' && (w.itemsize() == 8 || w.itemsize() == 4 || w.itemsize() == 2)) ||
+              ((w.kind() == 'u' || w.kind() == 'i') && w.itemsize() == 1)))
+            bad(files[b], "weight dtype must be float64, float32, float16 or uint8");
+        const uint64_t nb = base[b + 1] - base[b];
+        for (uint64_t j = 0; j < conns[b]; ++j) {
+            const int64_t src_b = (int64_t)b + int_at(bid, j);
+            if (src_b < 0 || src_b >= (int64_t)B)
+                bad(files[b], "input_block_idx refers to a block that is not loaded");
+            const int64_t src_n = int_at(nid, j);
+            if (src_n < 0 || (uint64_t)src_n >= base[src_b + 1] - base[src_b])
+                bad(files[b], "input_neuron_idx out of range");
+            const int64_t dst = int_at(oid, j);
+            if (dst < 0 || (uint64_t)dst >= nb)
+                bad(files[b], "output_neuron_idx out of range");
+            const int64_t c = int_at(ch, j);
+            if (c != 0 && c != 2)
+                bad(files[b], "input_channel_offset must be 0 or 2");
+            ++deg[base[src_b] + (uint64_t)src_n + 1];
+        }
+    }
+    rowptr[0] = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        rowptr[i + 1] = rowptr[i] + deg[i + 1];
+
+    // Pass 2: scatter into presynaptic rows (file order within a row).
+    std::vector<int64_t> fill(rowptr, rowptr + n);
+    std::vector<uint8_t> kind((size_t)S);
+    for (size_t b = 0; b < B; ++b) {
+        if (conns[b] == 0)
+            continue;
+        NpzArchive z(files[b]);
+        const NpyArray bid = z.get("input_block_idx", false), nid = z.get("input_neuron_idx", false);
+        const NpyArray oid = z.get("output_neuron_idx", false), ch = z.get("input_channel_offset", false);
+        const NpyArray w = z.get("weight", false);
+        for (uint64_t j = 0; j < conns[b]; ++j) {
+            const uint64_t src = base[(uint64_t)((int64_t)b + int_at(bid, j))] + (uint64_t)int_at(nid, j);
+            const int64_t k = fill[src]++;
+            colinds[k] = (uint32_t)(base[b] + (uint64_t)int_at(oid, j));
+            weights[2 * k] = weight_at(w, 2 * j);
+            weights[2 * k + 1] = weight_at(w, 2 * j + 1);
+            kind[(size_t)k] = int_at(ch, j) == 2 ? 1 : 0;
+        }
+    }
+
+    // Rows sorted by postsynaptic id (stable): the step's binning forms one
+    // queue item per run of targets in a tile, so sorted rows give the fewest.
+    unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (int)std::max<uint64_t>(1, std::min<uint64_t>(std::min(hw ? hw : 4u, 16u), S / (1 << 20)));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        th.emplace_back([&, t]() {
+            std::vector<uint32_t> idx;
+            std::vector<uint32_t> c2;
+            std::vector<float> w2;
+            std::vector<uint8_t> k2;
+            const uint64_t a = n * t / nt, e = n * (t + 1) / nt;
+            for (uint64_t r = a; r < e; ++r) {
+                const int64_t lo = rowptr[r], len = rowptr[r + 1] - lo;
+                if (len < 2 || std::is_sorted(colinds + lo, colinds + lo + len))
+                    continue;
+                idx.resize((size_t)len);
+                for (int64_t q = 0; q < len; ++q)
+                    idx[(size_t)q] = (uint32_t)q;
+                std::stable_sort(idx.begin(), idx.end(),
+                                 [&](uint32_t x, uint32_t y) { return colinds[lo + x] < colinds[lo + y]; });
+                c2.resize((size_t)len);
+                w2.resize((size_t)len * 2);
+                k2.resize((size_t)len);
+                for (int64_t q = 0; q < len; ++q) {
+                    const int64_t s = lo + idx[(size_t)q];
+                    c2[(size_t)q] = colinds[s];
+                    w2[(size_t)(2 * q)] = weights[2 * s];
+                    w2[(size_t)(2 * q + 1)] = weights[2 * s + 1];
+                    k2[(size_t)q] = kind[(size_t)s];
+                }
+                std::memcpy(colinds + lo, c2.data(), sizeof(uint32_t) * (size_t)len);
+                std::memcpy(weights + 2 * lo, w2.data(), sizeof(float) * 2 * (size_t)len);
+                std::memcpy(kind.data() + lo, k2.data(), (size_t)len);
+            }
+        });
+    }
+    for (auto &x : th)
+        x.join();
+    const uint64_t words = (S + 31) / 32;
+    std::memset(connkinds, 0, sizeof(uint32_t) * (size_t)words);
+    for (uint64_t j = 0; j < S; ++j)
+        if (kind[(size_t)j])
+            connkinds[j >> 5] |= 1u << (j & 31);
+}
+
+}  // namespace dtb

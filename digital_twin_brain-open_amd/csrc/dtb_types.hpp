@@ -51,6 +51,14 @@ constexpr uint64_t kQBeginMask = (1ull << kQBeginBits) - 1ull;
 constexpr int64_t kMaxSynapses = (int64_t)1 << kQBeginBits;
 constexpr int64_t kMaxRowLength = (int64_t)1 << (64 - kQBeginBits);
 
+// Per-population statistics (stat_result.cpp:266-391): which sums a run
+// records.  Each neuron's value enters as the exact integer rint(x * 2^24),
+// so the sums do not depend on the order of the additions.
+constexpr uint32_t kStatV = 1u;          // membrane potential (vmean)
+constexpr uint32_t kStatI = 2u;          // synaptic current (imean)
+constexpr uint32_t kStatReceptor = 4u;   // per-receptor currents (receptor imeans)
+constexpr int kStatFields = 6;           // V, I_syn, AMPA, NMDA, GABA_A, GABA_B
+
 // Largest number of distinct parameter records kept as classes (2-byte class
 // index per neuron); beyond it the records are stored per neuron.
 constexpr size_t kMaxParamClasses = 65536;
@@ -82,9 +90,12 @@ struct alignas(16) InstDev {
     const PopDev *pops;         // [npop]
     // recording (device buffers, row = step - rec_step0)
     uint32_t *rec_counts;       // [rec_len][npop] spikes per population
+    long long *rec_stat;        // [rec_len][npop][6] sums of V, I_syn, 4 receptor currents (x 2^24)
     // counters
-    unsigned long long *events; // synaptic events delivered (sum of fired out-degrees)
-    unsigned long long *spikes; // spikes emitted
+    // per wave of 64 neurons: (synaptic events delivered = sum of fired
+    // out-degrees, spikes emitted); each slot is updated only by the wave
+    // that owns those neurons, so no atomics are needed
+    unsigned long long *wave_ctr;   // [ceil(n/64)][2]
     int64_t *step;              // [2] step index by launch parity: the launch of
                                 // parity p reads step[p] and writes step[p^1] = step[p] + 1
     // scalars
@@ -97,7 +108,7 @@ struct alignas(16) InstDev {
     int32_t use_ou;
     float dt_sub;
     uint32_t tile_base;         // engine-wide index of this instance's first tile
-    uint32_t pad;
+    uint32_t stat_mask;         // kStatV | kStatI | kStatReceptor: statistics recorded in rec_stat
 };
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons

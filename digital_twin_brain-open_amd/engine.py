@@ -8,6 +8,7 @@ the native engine in-process instead of the gRPC/MPI service.
 ``Engine`` exposes the multi-instance engine (batched stepping).
 """
 import ctypes
+import os
 from ctypes import byref, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
 
 import numpy as np
@@ -98,6 +99,63 @@ def gen_uniform(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
     _lib.check(lib.dtb_gen_uniform(n, k, seed, _ptr(g), sub_base, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w),
                                    _ptr(kinds)))
     return NetworkArrays(props, rowptr, col, w, kinds)
+
+
+def load_npz(path):
+    """Reads the reference's block files through the native reader: one
+    ``block_<i>.npz`` or a directory of ``block_0.npz .. block_<B-1>.npz``
+    (python/make_block.py:608-614; src/configuration.cpp:1079-1667), blocks
+    concatenated in block order.  Returns (NetworkArrays, block_begin)."""
+    lib = _lib.load()
+    p = os.fsencode(os.fspath(path))
+    n, s, b = c_int64(0), c_int64(0), c_int32(0)
+    _lib.check(lib.dtb_npz_size(p, byref(n), byref(s), byref(b)))
+    props = np.empty((n.value, NCOL), dtype=np.float32)
+    rowptr = np.empty(n.value + 1, dtype=np.int64)
+    col = np.empty(s.value, dtype=np.uint32)
+    w = np.empty((s.value, 2), dtype=np.float32)
+    kinds = np.empty(max((s.value + 31) // 32, 1), dtype=np.uint32)
+    begin = np.empty(b.value + 1, dtype=np.int64)
+    _lib.check(lib.dtb_npz_read(p, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w), _ptr(kinds), _ptr(begin)))
+    return NetworkArrays(props, rowptr, col, w, kinds), begin
+
+
+def save_npz(net, directory, dtype="single", block_begin=None):
+    """Writes ``net`` as the reference's block files
+    ``<directory>/<dtype>/block_<i>.npz`` (python/make_block.py:580-614): per
+    block its properties and in-edges sorted by (postsynaptic neuron,
+    relative presynaptic block), input_block_idx relative to the block,
+    input_channel_offset 0 (AMPA/NMDA) or 2 (GABA_A/GABA_B).  dtype "single"
+    stores the weights as float32, "double" float64, "half" float16, "uint8"
+    as round(255 w).  Returns the directory holding the files."""
+    wtype = {"single": np.float32, "double": np.float64, "half": np.float16, "uint8": np.uint8}[dtype]
+    begin = np.asarray([0, net.n] if block_begin is None else block_begin, dtype=np.int64)
+    out = os.path.join(os.fspath(directory), dtype)
+    os.makedirs(out, exist_ok=True)
+    pre = np.repeat(np.arange(net.n, dtype=np.int64), np.diff(net.rowptr))
+    post = net.col.astype(np.int64)
+    pre_blk = np.searchsorted(begin, pre, side="right") - 1
+    post_blk = np.searchsorted(begin, post, side="right") - 1
+    kind = net.kind_per_synapse()
+    for i in range(len(begin) - 1):
+        sel = np.flatnonzero(post_blk == i)
+        rel = (pre_blk[sel] - i).astype(np.int16)
+        onid = (post[sel] - begin[i]).astype(np.uint32)
+        order = np.lexsort((rel, onid))
+        sel, rel, onid = sel[order], rel[order], onid[order]
+        w = net.w[sel]
+        if wtype is np.uint8:
+            w = np.clip(np.rint(w.astype(np.float64) * 255.0), 0, 255).astype(np.uint8)
+        else:
+            w = w.astype(wtype)
+        np.savez(os.path.join(out, "block_%d" % i),
+                 property=net.props[begin[i]:begin[i + 1]],
+                 output_neuron_idx=onid,
+                 input_block_idx=rel,
+                 input_neuron_idx=(pre[sel] - begin[pre_blk[sel]]).astype(np.uint32),
+                 input_channel_offset=(kind[sel] * 2).astype(np.uint8),
+                 weight=w)
+    return out
 
 
 def gen_voxel(n, nvox, k, seed, intra_frac=0.7, density=0.05, g_ui=DEFAULT_G_UI):
@@ -231,7 +289,8 @@ class Engine:
                                                   int(bool(assign))))
 
     def run_block(self, inst, iters, t_steps=1, use_ou=True, freqs=True, sample_spike=False, sample_vmemb=False,
-                  sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0):
+                  sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0, vmean=False,
+                  imean=False, receptor_imean=False):
         flags = (_lib.RUN_USE_OU if use_ou else 0) | (_lib.RUN_GRAPH if graph else 0)
         out = {}
         o = _lib.RunOutputs()
@@ -239,6 +298,19 @@ class Engine:
             flags |= _lib.RUN_FREQ
             out["freq"] = np.zeros((iters, npop), dtype=np.uint32)
             o.freq = _ptr(out["freq"])
+        if vmean:
+            flags |= _lib.RUN_VMEAN
+            out["vmean"] = np.zeros((iters, npop), dtype=np.float32)
+            o.vmean = _ptr(out["vmean"])
+        if imean:
+            flags |= _lib.RUN_IMEAN
+            out["imean"] = np.zeros((iters, npop), dtype=np.float32)
+            o.imean = _ptr(out["imean"])
+        if receptor_imean:
+            flags |= _lib.RUN_RECEPTOR_IMEAN
+            out["receptor_imean"] = np.zeros((4, iters, npop), dtype=np.float32)
+            o.ampa_imean, o.nmda_imean, o.gabaa_imean, o.gabab_imean = (_ptr(out["receptor_imean"][k])
+                                                                        for k in range(4))
         for key, flag, dt, field in (("spike", _lib.RUN_SAMPLE_SPIKE, np.uint8, "sample_spike"),
                                      ("vmemb", _lib.RUN_SAMPLE_VMEMB, np.float32, "sample_vmemb"),
                                      ("isynaptic", _lib.RUN_SAMPLE_ISYN, np.float32, "sample_isynaptic"),
@@ -322,12 +394,16 @@ class Block:
         self.engine.update_prop_cols(self.inst, property_idx, population_id, hyper_parameter, assign=True)
 
     def run(self, iterations, freqs=True, output_sample_spike=False, output_sample_vmemb=False,
-            output_sample_iou=False, output_sample_isynaptic=False, iou=True, t_steps=1, graph=False):
-        """dist_blockwrapper.py:179-350; returns a dict of per-step arrays."""
+            output_sample_iou=False, output_sample_isynaptic=False, iou=True, t_steps=1, graph=False,
+            vmean=False, imean=False, receptor_imean=False):
+        """dist_blockwrapper.py:179-350; returns a dict of per-step arrays
+        ("freq", "spike", "vmemb", "isynaptic", "iou", "vmean", "imean",
+        "receptor_imean" [4][iters][npop])."""
         return self.engine.run_block(self.inst, iterations, t_steps=t_steps, use_ou=iou, freqs=freqs,
                                      sample_spike=output_sample_spike, sample_vmemb=output_sample_vmemb,
                                      sample_isynaptic=output_sample_isynaptic, sample_iou=output_sample_iou,
-                                     graph=graph, nsample=self._nsample, npop=len(self.pop_ids))
+                                     graph=graph, nsample=self._nsample, npop=len(self.pop_ids), vmean=vmean,
+                                     imean=imean, receptor_imean=receptor_imean)
 
     def state(self):
         st = self.engine.read_state(self.inst, self._n)

@@ -45,7 +45,10 @@ enum dtb_run_flags {
     DTB_RUN_SAMPLE_VMEMB = 1u << 3,    /* output_sample_vmemb */
     DTB_RUN_SAMPLE_ISYN = 1u << 4,     /* output_sample_isynaptic */
     DTB_RUN_SAMPLE_IOU = 1u << 5,      /* output_sample_iou */
-    DTB_RUN_GRAPH = 1u << 8            /* replay the step loop from a HIP graph */
+    DTB_RUN_VMEAN = 1u << 6,           /* output_vmean */
+    DTB_RUN_IMEAN = 1u << 7,           /* output_imean */
+    DTB_RUN_GRAPH = 1u << 8,           /* replay the step loop from a HIP graph */
+    DTB_RUN_RECEPTOR_IMEAN = 1u << 9   /* output_receptor_imeas */
 };
 
 /* PropType, include/stage.hpp:320-341 == protos/snn.proto:94-115. */
@@ -84,6 +87,16 @@ typedef struct dtb_run_outputs {
     float *sample_vmemb;        /* [iters][nsample] */
     float *sample_isynaptic;    /* [iters][nsample] */
     float *sample_iou;          /* [iters][nsample] */
+    /* Per-population means over the non-blocked neurons at the end of each
+     * step (stat_result.cpp:266-391): V, I_syn, and the receptor currents
+     * g_u (V_u - V) J_u; [iters][npop] each.  Each neuron's value enters the
+     * sum as a multiple of 2^-24 (exact, order-free sums). */
+    float *vmean;
+    float *imean;
+    float *ampa_imean;
+    float *nmda_imean;
+    float *gabaa_imean;
+    float *gabab_imean;
 } dtb_run_outputs;
 
 /* ---- library ------------------------------------------------------------ */
@@ -182,6 +195,24 @@ int dtb_gen_uniform(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uin
 int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
                   const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                   uint32_t *connkinds, uint32_t *vox_sizes);
+
+/* ---- reference block files (host) --------------------------------------- */
+
+/* The reference's block files (python/make_block.py:608-614, np.savez with
+ * property / output_neuron_idx / input_block_idx / input_neuron_idx /
+ * input_channel_offset / weight; read by src/configuration.cpp:1079-1667,
+ * 1778-2156 through BrainBlock::init_connection_table_gpu and
+ * init_config_params_gpu, src/block.cpp:36-127).  `path` is one block_<i>.npz
+ * or a directory of block_0.npz .. block_<B-1>.npz, whose blocks are
+ * concatenated in block order into one network (connections between the
+ * blocks become ordinary synapses).  Stored and deflated archives, zip64.
+ * First call: totals.  Second call: fills caller-allocated arrays in the
+ * dtb_network layout (rows sorted by postsynaptic id, weights converted to
+ * fp32 as the reference's propagation does: float64/float16 cast, uint8 / 255);
+ * block_begin [nblocks+1] may be NULL. */
+int dtb_npz_size(const char *path, int64_t *n, int64_t *nsyn, int32_t *nblocks);
+int dtb_npz_read(const char *path, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
+                 uint32_t *connkinds, int64_t *block_begin);
 
 #ifdef __cplusplus
 }

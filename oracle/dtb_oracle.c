@@ -516,8 +516,13 @@ static inline int lif32(const derived32 *d, dtbo_state32 *st, int64_t i, float I
     return fired;
 }
 
+int64_t dtbo_stat_q(float x)
+{
+    return (int64_t)rintf(x * 0x1p24f);
+}
+
 int dtbo_run32(const dtbo_net *net, dtbo_state32 *st, int32_t nsteps, int32_t use_ou,
-               uint8_t *raster, float *vtrace, uint32_t *pop_counts, int32_t nthreads)
+               uint8_t *raster, float *vtrace, uint32_t *pop_counts, int64_t *stat_sums, int32_t nthreads)
 {
     if (net->t_steps < 1 || (net->t_steps > 1 && !use_ou))
         return 1;   /* dist_simulator.cpp:3225-3234: t_steps > 1 needs OU */
@@ -599,6 +604,25 @@ int dtbo_run32(const dtbo_net *net, dtbo_state32 *st, int32_t nsteps, int32_t us
                 for (int64_t i = net->pop_begin[q]; i < net->pop_begin[q + 1]; ++i)
                     cnt += (uint32_t)(emit[i] && !d.blocked[i]);
                 pop_counts[(size_t)k * (size_t)net->npop + (size_t)q] = cnt;
+            }
+        }
+        if (stat_sums) {
+            /* stat_result.cpp:266-391: per-population V, I_syn and receptor
+             * currents (synaptic_current.cpp:39-43 terms) at the end of the
+             * step, blocked neurons excluded. */
+            for (int32_t q = 0; q < net->npop; ++q) {
+                int64_t acc[6] = {0, 0, 0, 0, 0, 0};
+                for (int64_t i = net->pop_begin[q]; i < net->pop_begin[q + 1]; ++i) {
+                    if (d.blocked[i])
+                        continue;
+                    const float V = st->V[i];
+                    const float *J = st->J + i * 4, *g = d.g + i * 4, *vu = d.vu + i * 4;
+                    acc[0] += dtbo_stat_q(V);
+                    acc[1] += dtbo_stat_q(st->I_syn[i]);
+                    for (int c = 0; c < 4; ++c)
+                        acc[2 + c] += dtbo_stat_q(g[c] * (vu[c] - V) * J[c]);
+                }
+                memcpy(stat_sums + ((size_t)k * (size_t)net->npop + (size_t)q) * 6, acc, sizeof(acc));
             }
         }
         st->step = step + 1;

@@ -115,9 +115,17 @@ void dtbo_init64(const dtbo_net *net, dtbo_state64 *st);
 /* Runs nsteps steps.  raster: [nsteps][n] emitted spike flags (or NULL);
  * vtrace: [nsteps][n] V after each step (or NULL); pop_counts:
  * [nsteps][npop] emitted spikes per population excluding blocked neurons (or
- * NULL).  use_ou = 0 selects the noise-spike mode.  Returns 0 on success. */
+ * NULL); stat_sums: [nsteps][npop][6] per-population sums over non-blocked
+ * neurons of V, I_syn and the four receptor currents g_u (V_u - V) J_u at the
+ * end of each step, each value quantized to a multiple of 2^-24 (see
+ * dtbo_stat_q) so the sums are exact (or NULL; stat_result.cpp:266-391
+ * averages these).  use_ou = 0 selects the noise-spike mode.  Returns 0 on
+ * success. */
 int dtbo_run32(const dtbo_net *net, dtbo_state32 *st, int32_t nsteps, int32_t use_ou,
-               uint8_t *raster, float *vtrace, uint32_t *pop_counts, int32_t nthreads);
+               uint8_t *raster, float *vtrace, uint32_t *pop_counts, int64_t *stat_sums, int32_t nthreads);
+
+/* A statistic's value as the exact integer rint(x * 2^24). */
+int64_t dtbo_stat_q(float x);
 
 /* fp64 model.  force_raster: [nsteps][n] spike decisions to follow instead of
  * the fp64 threshold test (or NULL for a free run).  mismatch_margin (may be

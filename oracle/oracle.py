@@ -65,7 +65,9 @@ def load():
     lib.dtbo_init32.argtypes = [POINTER(_Net), POINTER(_State32)]
     lib.dtbo_init64.argtypes = [POINTER(_Net), POINTER(_State64)]
     lib.dtbo_run32.argtypes = [POINTER(_Net), POINTER(_State32), c_int32, c_int32, c_void_p, c_void_p, c_void_p,
-                               c_int32]
+                               c_void_p, c_int32]
+    lib.dtbo_stat_q.argtypes = [c_float]
+    lib.dtbo_stat_q.restype = c_int64
     lib.dtbo_run32.restype = c_int32
     lib.dtbo_run64.argtypes = [POINTER(_Net), POINTER(_State64), c_int32, c_int32, c_void_p, c_void_p, c_void_p,
                                c_void_p, c_int32]
@@ -196,7 +198,10 @@ class Sim:
                 self.st.i_ou[self.net.pop_begin[q]:self.net.pop_begin[q + 1]] = self.net.ou_mean[q]
 
     def run(self, nsteps, use_ou=True, raster=False, vtrace=False, pop_counts=False, force_raster=None,
-            margins=False):
+            margins=False, stats=False):
+        """stats: also returns out["stat_sums"] [nsteps][npop][6] (exact sums of
+        V, I_syn, AMPA, NMDA, GABA_A, GABA_B currents as multiples of 2^-24)
+        and out["stat_means"] (the same divided by the non-blocked counts)."""
         n = self.net.n
         out = {}
         r = np.zeros((nsteps, n), dtype=np.uint8) if raster else None
@@ -205,10 +210,14 @@ class Sim:
         s = self.st.struct()
         if self.prec == 32:
             pc = np.zeros((nsteps, self.net.npop), dtype=np.uint32) if pop_counts else None
+            ss = np.zeros((nsteps, self.net.npop, 6), dtype=np.int64) if stats else None
             rc = load().dtbo_run32(ctypes.byref(self._ns), ctypes.byref(s), nsteps, int(use_ou), _p(r), _p(v),
-                                   _p(pc), self.nthreads)
+                                   _p(pc), _p(ss), self.nthreads)
             if pc is not None:
                 out["freq"] = pc
+            if ss is not None:
+                out["stat_sums"] = ss
+                out["stat_means"] = stat_means(ss, self.net.props, self.net.pop_begin)
         else:
             fr = None if force_raster is None else np.ascontiguousarray(force_raster, dtype=np.uint8)
             m = np.zeros(nsteps, dtype=np.float64) if margins else None
@@ -224,6 +233,16 @@ class Sim:
         if v is not None:
             out["V"] = v
         return out
+
+
+def stat_means(sums, props, pop_begin):
+    """Per-population means from exact 2^-24 sums: float32 of
+    (double) sum * 2^-24 / (neurons of the population that are not blocked)."""
+    blocked = props[:, 1] != 0
+    cnt = np.array([np.count_nonzero(~blocked[pop_begin[q]:pop_begin[q + 1]]) for q in range(len(pop_begin) - 1)],
+                   dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return (sums.astype(np.float64) * 2.0 ** -24 / cnt[None, :, None]).astype(np.float32)
 
 
 def dense_delta(net, fired):

@@ -321,3 +321,68 @@ def test_batched_equals_alone(dtb, gpu):
         assert np.array_equal(bits(st2["J"]), bits(st["J"]))
         assert b.counters()[0] == 200
     eng.close()
+
+
+def test_population_means_match_oracle(dtb, orc, gpu):
+    """vmean / imean / receptor imeans (stat_result.cpp:266-391) bit-exact
+    against the oracle, with blocked neurons, 12 populations, two runs (the
+    second replayed from graphs) so every run's last-step tail is covered."""
+    n, seed = 6000, 12
+    net = dtb.gen_voxel(n, 6, 80, seed=7)
+    net.props[::9, 1] = 1.0
+    blk = dtb.Block(net, delta_t=DT, seed=seed)
+    blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 3.0, 0.2)
+    kw = dict(freqs=True, vmean=True, imean=True, receptor_imean=True)
+    a = blk.run(150, **kw)
+    b = blk.run(150, graph=True, **kw)
+    st = blk.state()
+    blk.engine.close()
+
+    on = orc.OracleNet.from_network(net, dt=DT, seed=seed)
+    on.set_ou(on.pop_ids, 3.0, 0.2, 4.0)
+    sim = orc.Sim(on, 32)
+    ra = sim.run(150, pop_counts=True, stats=True)
+    rb = sim.run(150, pop_counts=True, stats=True)
+    for got, ref in ((a, ra), (b, rb)):
+        assert np.array_equal(got["freq"], ref["freq"])
+        assert np.array_equal(bits(got["vmean"]), bits(ref["stat_means"][:, :, 0])), "vmean"
+        assert np.array_equal(bits(got["imean"]), bits(ref["stat_means"][:, :, 1])), "imean"
+        for k in range(4):
+            assert np.array_equal(bits(got["receptor_imean"][k]), bits(ref["stat_means"][:, :, 2 + k])), k
+    assert_state_equal(st, sim)
+    assert rb["freq"].sum() > 1000
+
+
+def test_blockwrapper_mirror(dtb, orc, gpu, tmp_path):
+    """BlockWrapper (dist_blockwrapper.py signatures) over reference block
+    files: unsorted OU population ids, relative mul_property_by_subblk,
+    samples with repeats in caller order, per-step tuples in the reference
+    order."""
+    net = dtb.gen_uniform(3000, 100, seed=3)
+    path = dtb.save_npz(net, tmp_path, "single")
+    bw = dtb.BlockWrapper(None, path, DT, seed=5)
+    assert bw.total_neurons == 3000 and list(bw.subblk_id) == [0, 1]
+    bw.update_ou_background_stimuli(np.array([1, 0]), np.array([4.0, 4.0], np.float32),
+                                     np.array([1.6, 1.5], np.float32), np.array([0.1, 0.1], np.float32))
+    bw.mul_property_by_subblk(np.array([[0, 10]]), np.array([1.2], np.float32))
+    bw.mul_property_by_subblk(np.array([[0, 10]]), np.array([1.5], np.float32))
+    samples = np.array([2999, 5, 5, 100])
+    bw.set_samples(samples)
+    outs = list(bw.run(200, freqs=True, vmean=True, output_sample_spike=True, output_sample_vmemb=True, iou=True))
+    bw.close()
+    assert len(outs) == 200 and len(outs[0]) == 4
+    freq, vmean, spike, vm = (np.stack([o[k] for o in outs]) for k in range(4))
+
+    on = orc.OracleNet.from_network(net, dt=DT, seed=5)
+    on.set_ou(on.pop_ids, np.array([1.5, 1.6], np.float32), 0.1, 4.0)
+    sim = orc.Sim(on, 32)
+    p0 = on.props[:, 3] == 0
+    g = on.props[p0, 10]
+    step1 = (g * np.float32(1.2)).astype(np.float32)
+    on.props[p0, 10] = (step1 * (np.float32(1.5) / np.float32(1.2))).astype(np.float32)
+    ref = sim.run(200, raster=True, vtrace=True, pop_counts=True, stats=True)
+    assert np.array_equal(freq, ref["freq"].astype(np.int32))
+    assert np.array_equal(bits(vmean), bits(ref["stat_means"][:, :, 0]))
+    assert np.array_equal(spike, ref["raster"][:, samples])
+    assert np.array_equal(bits(vm), bits(ref["V"][:, samples]))
+    assert ref["freq"].sum() > 100

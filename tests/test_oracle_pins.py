@@ -324,3 +324,37 @@ def test_oracle_matches_reference_cpu_model_fixture(orc):
         assert rel.max() < vtol
         assert np.abs(sim.st.J - z["J_final"]).max() <= 1e-5 * np.abs(z["J_final"]).max()
         assert np.abs(sim.st.i_ou - z["i_bg_final"]).max() < 1e-5
+
+
+def test_population_means_of_v_isyn_and_receptor_currents(dtb, orc):
+    """stat_result.cpp:266-391: per-population means over non-blocked neurons
+    of V, I_syn and the receptor currents g_u (V_u - V) J_u
+    (synaptic_current.cpp:39-43) at the end of every step, checked against
+    float64 means of the state after each single step."""
+    net = dtb.gen_uniform(1500, 60, seed=4)
+    net.props[::5, 1] = 1.0
+    on = orc.OracleNet.from_network(net, seed=9)
+    on.set_ou(on.pop_ids, 2.0, 0.2, 4.0)
+    sim = orc.Sim(on, 32)
+    blocked = net.props[:, 1] != 0
+    g = net.props[:, 10:14].astype(np.float64)
+    vu = net.props[:, 14:18].astype(np.float64)
+    spikes = 0
+    for step in range(120):
+        out = sim.run(1, raster=True, stats=True)
+        spikes += int(out["raster"].sum())
+        m = out["stat_means"][0]
+        V = sim.st.V.astype(np.float64)
+        J = sim.st.J.astype(np.float64)
+        rc = g * (vu - V[:, None]) * J
+        for q in range(on.npop):
+            sel = np.zeros(net.n, bool)
+            sel[on.pop_begin[q]:on.pop_begin[q + 1]] = True
+            sel &= ~blocked
+            expect = [V[sel].mean(), sim.st.I_syn[sel].astype(np.float64).mean()] + list(rc[sel].mean(0))
+            for k in range(6):
+                assert abs(m[q, k] - expect[k]) <= 1e-5 * abs(expect[k]) + 1e-6, (step, q, k, m[q, k], expect[k])
+        # the four receptor sums add up to the I_syn sum within one unit per neuron
+        s = out["stat_sums"][0]
+        assert np.all(np.abs(s[:, 2:].sum(1) - s[:, 1]) <= 4 * np.diff(on.pop_begin))
+    assert spikes > 50
