@@ -31,6 +31,9 @@ IN_DEGREE = 500
 OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
 NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of one k_step launch on this workload,
+# scaled by a device-copy calibration (the file gives commands and factors).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md")
 
 # Algorithmic HBM bytes of one step (k_step):
 # per neuron: parameter class (2 B) read; J float4 (16 B) read + write; V,
@@ -41,6 +44,18 @@ HBM_PEAK_GBS = 8000.0
 STEP_NEURON_BYTES = 2 + 2 * 16 + 2 * 12 + 2 * 1
 STEP_ROW_BYTES = 16
 STEP_SYN_BYTES = 12
+
+
+def profiled_traffic(neurons):
+    """HBM bytes per k_step launch measured with rocprofv3 counters on the
+    default workload, or None for another size."""
+    if neurons != N_NEURONS or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def log(msg):
@@ -153,7 +168,6 @@ def main():
     us = ms_step * 1e3 / args.steps
     algo_bytes = STEP_NEURON_BYTES * net.n + STEP_ROW_BYTES * sp_step + STEP_SYN_BYTES * ev_step
     achieved = algo_bytes / (us * 1e-6) / 1e9
-    traffic_env = os.environ.get("DTB_BENCH_TRAFFIC_BYTES")
 
     res = {
         "metric": "synaptic events/s (2M-neuron block)",
@@ -178,7 +192,7 @@ def main():
         "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
         "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": float(traffic_env) if traffic_env else None},
+                     "traffic": profiled_traffic(args.neurons)},
     }
     if args.cpu_steps > 0:
         t0 = time.perf_counter()

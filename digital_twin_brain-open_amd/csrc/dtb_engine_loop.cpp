@@ -282,8 +282,9 @@ void Engine::run_timed(int64_t steps, float *ms_step)
     if (ninst_ == 0 || steps <= 0)
         return;
     pending_ = true;
-    // Events bracket every step kernel on the engine stream; they are read
-    // back once per chunk so the launch pipeline is not drained every step.
+    // Every step kernel records its own start and stop events (at the
+    // dispatch boundaries, on the engine stream); they are read back once
+    // per chunk so the launch pipeline is not drained every step.
     const int64_t chunk = 256;
     std::vector<hipEvent_t> ev((size_t)(2 * chunk));
     for (auto &e : ev)
@@ -293,9 +294,8 @@ void Engine::run_timed(int64_t steps, float *ms_step)
         const int64_t m = std::min<int64_t>(chunk, steps - base);
         for (int64_t k = 0; k < m; ++k) {
             hipEvent_t *e = &ev[(size_t)(2 * k)];
-            DTB_HIP(hipEventRecord(e[0], stream_));
-            launch_step_k();
-            DTB_HIP(hipEventRecord(e[1], stream_));
+            DTB_HIP(launch_step_kernel_timed(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
+                                             d_ctl_.mutable_gpu_data(), e[0], e[1], stream_));
             par_ ^= 1u;
             launch_aux();
         }

@@ -26,6 +26,8 @@
 // of step k - 1.  No global atomics touch the delta.
 //   k_materialize writes the pending delta of every neuron (without consuming
 //   the queues) for sampling and state read-back; k_sample gathers samples.
+#include <hip/hip_ext.h>
+
 #include "dtb_kernels.hpp"
 #include "dtb_rng.hpp"
 
@@ -660,6 +662,15 @@ hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const Inst
     if (ntiles == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
+                                    uint32_t par, StepCtl *ctl, hipEvent_t start, hipEvent_t stop, hipStream_t stream)
+{
+    if (ntiles == 0)
+        return hipSuccess;
+    hipExtLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par, ctl);
     return hipGetLastError();
 }
 

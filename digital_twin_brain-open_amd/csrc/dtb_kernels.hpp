@@ -10,6 +10,11 @@ namespace dtb {
 hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
                               uint32_t par, StepCtl *ctl, hipStream_t stream);
 
+// launch_step_kernel with `start` / `stop` recorded at the kernel's own
+// dispatch boundaries (kernel time without launch gaps).
+hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
+                                    uint32_t par, StepCtl *ctl, hipEvent_t start, hipEvent_t stop, hipStream_t stream);
+
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
                               uint32_t set, hipStream_t stream);
