@@ -11,6 +11,13 @@ population (400).  Synthetic, seeded.
 A step is one launch of the step kernel (delta accumulation, neuron update
 and spike binning) over the whole block with the network resident in HBM.
 Prints one JSON line.
+
+The line also carries "ensemble": the data-assimilation ensemble (64
+instances of 20k-200k neurons, log-uniform, in-degree 200, per-instance g_ui
+uniform in 0.5-1.5 x the 10k-neuron block's, OU mean 3.0) stepped by one
+launch per step for all instances ("batched") against the same kernels run
+one instance at a time ("plain"), with every instance's counters and state
+checked identical between the two.
 """
 import argparse
 import importlib
@@ -56,6 +63,68 @@ def profiled_traffic(neurons):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+ENS_INSTANCES, ENS_MIN, ENS_MAX, ENS_DEGREE, ENS_MU, ENS_SEED = 64, 20_000, 200_000, 200, 3.0, 4242
+
+
+def ensemble_bench(dtb, steps, warmup, chunk=64):
+    """Batched (all instances in one launch per step, graph replay) vs plain
+    (each instance alone on its own engine, same kernels, graph replay)."""
+    rng = np.random.default_rng(ENS_SEED)
+    sizes = np.exp(rng.uniform(np.log(ENS_MIN), np.log(ENS_MAX), ENS_INSTANCES)).astype(np.int64)
+    g0 = np.asarray(dtb.DEFAULT_G_UI, dtype=np.float32)
+    gs = (g0[None, :] * rng.uniform(0.5, 1.5, (ENS_INSTANCES, 4))).astype(np.float32)
+    t0 = time.perf_counter()
+    nets = [dtb.gen_uniform(int(n), ENS_DEGREE, seed=ENS_SEED + 1 + i, g_ui=gs[i]) for i, n in enumerate(sizes)]
+    log("ensemble: generated %d instances, %d neurons, %d synapses in %.1f s"
+        % (len(nets), int(sizes.sum()), sum(x.nsyn for x in nets), time.perf_counter() - t0))
+
+    def setup(eng, i, net):
+        b = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED + i, engine=eng)
+        b.update_ou_background_stimuli(b.subblk_id, OU_TAU, ENS_MU, OU_SIGMA)
+        eng.configure(b.inst, 1, True)
+        return b
+
+    eng = dtb.Engine(0)
+    blks = [setup(eng, i, net) for i, net in enumerate(nets)]
+    eng.run(warmup, graph_chunk=chunk)
+    eng.sync()
+    c0 = [eng.counters(b.inst) for b in blks]
+    t0 = time.perf_counter()
+    eng.run(steps, graph_chunk=chunk)
+    eng.sync()
+    tb = time.perf_counter() - t0
+    eng.check()
+    c1 = [eng.counters(b.inst) for b in blks]
+    vb = [b.state()["V"].view(np.uint32).copy() for b in blks]
+    eng.close()
+    ev_b = sum(c1[i][1] - c0[i][1] for i in range(len(blks)))
+
+    tp, ev_p, same = 0.0, 0, True
+    for i, net in enumerate(nets):
+        e = dtb.Engine(0)
+        b = setup(e, i, net)
+        e.run(warmup, graph_chunk=chunk)
+        e.sync()
+        s0 = e.counters(b.inst)
+        t0 = time.perf_counter()
+        e.run(steps, graph_chunk=chunk)
+        e.sync()
+        tp += time.perf_counter() - t0
+        s1 = e.counters(b.inst)
+        ev_p += s1[1] - s0[1]
+        same = same and s1 == c1[i] and np.array_equal(b.state()["V"].view(np.uint32), vb[i])
+        e.close()
+    spikes = sum(c1[i][2] - c0[i][2] for i in range(len(blks)))
+    return {"workload": "%d instances, %d-%d neurons log-uniform (%d total), in-degree %d, g_ui x U(0.5,1.5), "
+                        "OU mu %.1f, dt %.1f ms" % (ENS_INSTANCES, ENS_MIN, ENS_MAX, int(sizes.sum()), ENS_DEGREE,
+                                                   ENS_MU, DT_MS),
+            "steps": steps, "warmup": warmup, "graph_chunk": chunk,
+            "batched_events_per_s": ev_b / tb, "plain_events_per_s": ev_p / tp, "speedup": tp / tb,
+            "batched_ms_per_step": tb * 1e3 / steps, "plain_ms_per_step": tp * 1e3 / steps,
+            "mean_rate_hz": spikes / (int(sizes.sum()) * steps * DT_MS * 1e-3),
+            "identical_counters_and_state": bool(same)}
 
 
 def log(msg):
@@ -114,6 +183,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=500)
     ap.add_argument("--cpu-steps", type=int, default=100, help="CPU oracle sample length (0 = skip)")
     ap.add_argument("--neurons", type=int, default=N_NEURONS)
+    ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
@@ -201,6 +271,11 @@ def main():
     else:
         res["cpu_baseline"] = None
     eng.close()
+    del blk, net
+    if args.ensemble_steps > 0:
+        t0 = time.perf_counter()
+        res["ensemble"] = ensemble_bench(dtb, args.ensemble_steps, min(args.warmup, 500))
+        log("ensemble %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(res), flush=True)
 
 

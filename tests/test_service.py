@@ -1,0 +1,96 @@
+"""The snn.Snn gRPC service (protos/snn.proto:5-219) over libdtb.so."""
+import importlib
+import socket
+
+import numpy as np
+import pytest
+
+grpc = pytest.importorskip("grpc")
+
+
+def _free_port():
+    with socket.socket() as s:
+        s.bind(("127.0.0.1", 0))
+        return s.getsockname()[1]
+
+
+@pytest.fixture
+def service(dtb):
+    svc = importlib.import_module("digital_twin_brain-open_amd.snn_service")
+    addr = "127.0.0.1:%d" % _free_port()
+    server, servicer = svc.serve(addr)
+    channel = grpc.insecure_channel(addr)
+    client = svc.SnnClient(channel, servicer.m)
+    yield svc, client, servicer
+    channel.close()
+    server.stop(None)
+    servicer.close()
+
+
+def test_messages_round_trip(service):
+    svc, client, s = service
+    m = s.m
+    r = m.RunRequest(iter=7, iter_offset=3, t_steps=2, output_freq=True, freq_char=True, output_vmean=True,
+                     use_ou_background=True, output_sample_isynaptic=True)
+    assert m.RunRequest.FromString(r.SerializeToString()) == r
+    resp = m.RunResponse(status=0, freq=[np.arange(4, dtype=np.int32).tobytes()])
+    back = m.RunResponse.FromString(resp.SerializeToString())
+    assert np.frombuffer(back.freq[0], dtype=np.int32).tolist() == [0, 1, 2, 3]
+    u = m.UpdateHyperParaRequest(block_id=0, prop_id=[7, 10], brain_id=[3, 4], hpara_val=[1.5, 2.0], assigned=True)
+    assert m.UpdateHyperParaRequest.FromString(u.SerializeToString()) == u
+
+
+def test_version_errors_and_unsupported_calls(service):
+    svc, client, s = service
+    m = s.m
+    assert client.GetVersion(m.Empty()).version == importlib.import_module(
+        "digital_twin_brain-open_amd")._lib.load().dtb_version()
+    # Init of a missing path answers SNN_INVALID_PARAMETER, not a transport error
+    resp = list(client.Init(m.InitRequest(file_path="/nonexistent/blocks", delta_t=0.1)))
+    assert len(resp) == 1 and resp[0].status == 1
+    # Run before Init
+    assert [r.status for r in client.Run(m.RunRequest(iter=2))] == [1]
+    # updates before Init report failure
+    assert not client.Updateprop(iter([m.UpdatePropRequest(neuron_id=[0], prop_id=[5], prop_val=[1.0])])).success
+    # currents this build does not model report failure
+    assert not client.Updatedopaminecurrentparam(iter([m.UpdateDopamineCurrentParamRequest(brain_id=[0])])).success
+    assert not client.Updategamma(iter([m.UpdateGammaRequest(brain_id=[0])])).success
+    assert client.Shutdown(m.ShutdownRequest()).shutdown
+    assert s.stop_event.is_set()
+
+
+@pytest.mark.gpu
+def test_service_run_equals_block(dtb, gpu, service, tmp_path):
+    """Init from reference block files, OU and hyperparameter updates and a
+    Run streamed over gRPC give the same per-step counts and samples as the
+    same calls on a Block."""
+    svc, client, s = service
+    m = s.m
+    net = dtb.gen_uniform(4000, 100, seed=8)
+    path = dtb.save_npz(net, tmp_path, "single")
+    init = list(client.Init(m.InitRequest(file_path=path, delta_t=0.1)))
+    assert init[0].status == 0 and init[0].neurons_per_block == 4000
+    assert [i.subblk_id for i in init[0].subblk_info] == [0, 1]
+    assert client.Updateoubackgroundparam(iter([m.UpdateOUBackgroundParamRequest(
+        brain_id=[0, 1], mean=[1.5, 1.5], deviation=[0.1, 0.1], correlation_time=[4.0, 4.0])])).success
+    assert client.Updatehyperpara(iter([m.UpdateHyperParaRequest(prop_id=[7], brain_id=[1], hpara_val=[1.25],
+                                                                 assigned=False)])).success
+    ids = [3, 700, 3999]
+    assert client.Updatesample(iter([m.UpdateSampleRequest(sample_idx=ids)])).success
+    rs = list(client.Run(m.RunRequest(iter=150, output_freq=True, use_ou_background=True, output_sample_spike=True,
+                                      output_sample_vmemb=True, output_vmean=True)))
+    assert len(rs) == 150 and all(r.status == 0 for r in rs)
+    freq = np.stack([np.frombuffer(r.freq[0], dtype=np.int32) for r in rs])
+    vm = np.stack([np.frombuffer(r.sample_vmemb[0], dtype=np.float32) for r in rs])
+    vmean = np.stack([np.frombuffer(r.vmean[0], dtype=np.float32) for r in rs])
+
+    blk = dtb.Block(net, delta_t=0.1, seed=0)
+    blk.update_ou_background_stimuli(np.array([0, 1]), 4.0, 1.5, 0.1)
+    blk.mul_property_by_subblk([1], [7], [1.25])
+    blk.set_samples(np.array(ids, np.uint32))
+    ref = blk.run(150, output_sample_spike=True, output_sample_vmemb=True, vmean=True)
+    blk.engine.close()
+    assert np.array_equal(freq, ref["freq"].astype(np.int32))
+    assert np.array_equal(vm.view(np.uint32), ref["vmemb"].view(np.uint32))
+    assert np.array_equal(vmean.view(np.uint32), ref["vmean"].view(np.uint32))
+    assert freq.sum() > 50
