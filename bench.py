@@ -131,7 +131,33 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dtb, net, blk, steps):
+# Propagation stress: 500k neurons, in-degree 5000 (2.5e9 synapses), g_ui at
+# 0.02 x the 10k-neuron block's and OU mean 12 (sigma 0.5) for bursty
+# synchronous firing near 50 Hz.
+STRESS_NEURONS, STRESS_DEGREE, STRESS_G_SCALE, STRESS_MU, STRESS_SIGMA = 500_000, 5000, 0.02, 12.0, 0.5
+
+
+def make_workload(dtb, name, neurons):
+    """(network, OU mean, OU sigma, description) of a named workload."""
+    if name == "block":
+        net = dtb.gen_voxel(neurons, N_VOXELS, IN_DEGREE, seed=NET_SEED)
+        # One parameter record per population, as after per-population
+        # assimilation updates: g_AMPA differs by 1e-4 relative per population.
+        sub = net.props[:, 3].astype(np.int64)
+        net.props[:, 10] = (net.props[:, 10] * (1.0 + 1e-4 * sub)).astype(np.float32)
+        desc = ("one GPU's block: %d neurons, %d voxels, in-degree %d, dt %.1f ms, OU mu %.1f"
+                % (net.n, N_VOXELS, IN_DEGREE, DT_MS, OU_MEAN))
+        return net, OU_MEAN, OU_SIGMA, desc
+    if name == "stress":
+        g = (np.asarray(dtb.DEFAULT_G_UI, dtype=np.float32) * np.float32(STRESS_G_SCALE)).astype(np.float32)
+        net = dtb.gen_uniform(neurons, STRESS_DEGREE, seed=NET_SEED, g_ui=g)
+        desc = ("propagation stress: %d neurons, in-degree %d, g_ui x %.2f, dt %.1f ms, OU mu %.1f sigma %.1f"
+                % (net.n, STRESS_DEGREE, STRESS_G_SCALE, DT_MS, STRESS_MU, STRESS_SIGMA))
+        return net, STRESS_MU, STRESS_SIGMA, desc
+    raise ValueError(name)
+
+
+def cpu_baseline(dtb, net, blk, steps, mu, sigma):
     """The CPU oracle (a C port of the reference step, OpenMP) on this host,
     started from the GPU's state after warm-up so that it runs the same
     dynamics; returns events/s over `steps` steps."""
@@ -139,7 +165,7 @@ def cpu_baseline(dtb, net, blk, steps):
     orc = importlib.import_module("oracle")
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     on = orc.OracleNet.from_network(net, dt=DT_MS, seed=SIM_SEED)
-    on.set_ou(on.pop_ids, OU_MEAN, OU_SIGMA, OU_TAU)
+    on.set_ou(on.pop_ids, mu, sigma, OU_TAU)
     sim = orc.Sim(on, 32, nthreads=threads)
     st = blk.state()
     sim.load_state(st["V"], st["t"], st["f"], st["i_ou"], st["J"], st["I_syn"], st["step"])
@@ -149,8 +175,8 @@ def cpu_baseline(dtb, net, blk, steps):
     el = time.perf_counter() - t0
     events = int((out["raster"].astype(np.int64) * deg[None, :]).sum())
     return {"value": events / el, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-            "sample": "%d steps of the same 2M-neuron block from the GPU state after warm-up "
-                      "(%d events, %.1f s, s/step %.4f, RTF %.1f)" % (steps, events, el, el / steps,
+            "sample": "%d steps of the same %d-neuron network from the GPU state after warm-up "
+                      "(%d events, %.1f s, s/step %.4f, RTF %.1f)" % (steps, net.n, events, el, el / steps,
                                                                      el / (steps * DT_MS * 1e-3))}
 
 
@@ -179,10 +205,12 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run device copies of this size (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10000; stress 2000)")
+    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 500; stress 200)")
     ap.add_argument("--cpu-steps", type=int, default=100, help="CPU oracle sample length (0 = skip)")
-    ap.add_argument("--neurons", type=int, default=N_NEURONS)
+    ap.add_argument("--workload", choices=("block", "stress"), default="block",
+                    help="block: the 2M-neuron block (headline); stress: 500k neurons at in-degree 5000")
+    ap.add_argument("--neurons", type=int, default=0, help="override the workload's neuron count")
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
@@ -200,18 +228,19 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.neurons == 0:
+        args.neurons = N_NEURONS if args.workload == "block" else STRESS_NEURONS
+    if args.steps <= 0:
+        args.steps = 10000 if args.workload == "block" else 2000
+    if args.warmup < 0:
+        args.warmup = 500 if args.workload == "block" else 200
     t0 = time.perf_counter()
-    net = dtb.gen_voxel(args.neurons, N_VOXELS, IN_DEGREE, seed=NET_SEED)
+    net, mu, sigma, desc = make_workload(dtb, args.workload, args.neurons)
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
     blk = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED, engine=eng)
-    blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, OU_MEAN, OU_SIGMA)
-    # One parameter record per population, as after per-population
-    # assimilation updates: g_AMPA differs by 1e-4 relative per population.
-    pops = np.asarray(blk.subblk_id, dtype=np.uint32)
-    g = net.props[0, 10] * (1.0 + 1e-4 * np.arange(len(pops)))
-    blk.assign_property_by_subblk(pops, np.full(len(pops), 7, np.uint32), g.astype(np.float32))
+    blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, mu, sigma)
     eng.configure(blk.inst, 1, True)
     log("uploaded in %.1f s" % (time.perf_counter() - t0))
 
@@ -240,7 +269,8 @@ def main():
     achieved = algo_bytes / (us * 1e-6) / 1e9
 
     res = {
-        "metric": "synaptic events/s (2M-neuron block)",
+        "metric": "synaptic events/s (%s)" % ("2M-neuron block" if args.workload == "block" else
+                                              "propagation stress"),
         "value": value,
         "unit": "synaptic events/s",
         "n_gpus": 1,
@@ -252,9 +282,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded voxel network, random weights k*2^-24)",
-        "config": {"workload": "one GPU's block: %d neurons, %d voxels, in-degree %d, dt %.1f ms, OU mu %.1f"
-                               % (net.n, N_VOXELS, IN_DEGREE, DT_MS, OU_MEAN),
-                   "neurons": net.n, "synapses": net.nsyn, "parallelism": "single GPU"},
+        "config": {"workload": desc, "neurons": net.n, "synapses": net.nsyn, "parallelism": "single GPU"},
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
@@ -262,17 +290,17 @@ def main():
         "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
         "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.neurons)},
+                     "traffic": profiled_traffic(args.neurons) if args.workload == "block" else None},
     }
     if args.cpu_steps > 0:
         t0 = time.perf_counter()
-        res["cpu_baseline"] = cpu_baseline(dtb, net, blk, args.cpu_steps)
+        res["cpu_baseline"] = cpu_baseline(dtb, net, blk, args.cpu_steps, mu, sigma)
         log("cpu baseline %.1f s" % (time.perf_counter() - t0))
     else:
         res["cpu_baseline"] = None
     eng.close()
     del blk, net
-    if args.ensemble_steps > 0:
+    if args.ensemble_steps > 0 and args.workload == "block":
         t0 = time.perf_counter()
         res["ensemble"] = ensemble_bench(dtb, args.ensemble_steps, min(args.warmup, 500))
         log("ensemble %.1f s" % (time.perf_counter() - t0))

@@ -109,6 +109,10 @@ _SIGS = {
                                 c_void_p, c_void_p]),
     "dtb_gen_voxel": (c_int, [c_int64, c_uint32, c_uint32, c_float, c_float, c_uint64, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dtb_gen_fanout": (c_int, [c_int64, c_uint32, c_uint64, c_void_p, c_uint32, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p]),
+    "dtb_instance_add_fanout": (c_int, [c_void_p, c_int64, c_uint32, c_uint64, c_void_p, c_uint32, c_float, c_uint64,
+                                        POINTER(c_int)]),
     "dtb_npz_size": (c_int, [c_char_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)]),
     "dtb_npz_read": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }

@@ -120,6 +120,26 @@ int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t 
     });
 }
 
+int dtb_instance_add_fanout(dtb_engine_t h, int64_t n, uint32_t k, uint64_t net_seed, const float *g_ui,
+                            uint32_t sub_base, float dt, uint64_t seed, int *inst_id)
+{
+    return guarded([&] {
+        need(g_ui != nullptr && inst_id != nullptr, "g_ui and inst_id are required");
+        dtb::Engine &e = engine(h);
+        DTB_HIP(hipSetDevice(e.device()));
+        dtb::FanoutSpec f;
+        f.n = n;
+        f.K = k;
+        f.net_seed = net_seed;
+        for (int c = 0; c < 4; ++c)
+            f.g_ui[c] = g_ui[c];
+        f.sub_base = sub_base;
+        e.sync();
+        auto inst = std::make_unique<dtb::Instance>(f, dt, seed, e.stream());
+        *inst_id = e.add(std::move(inst));
+    });
+}
+
 int dtb_instance_remove(dtb_engine_t h, int inst)
 {
     return guarded([&] { engine(h).remove(inst); });
@@ -373,6 +393,15 @@ int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float 
         need(g_ui && props && rowptr && colinds && weights && connkinds, "output arrays are required");
         dtb::gen_voxel(n, nvox, k, intra_frac, density, seed, g_ui, props, rowptr, colinds, weights, connkinds,
                        vox_sizes);
+    });
+}
+
+int dtb_gen_fanout(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint32_t sub_base, float *props,
+                   int64_t *rowptr, uint32_t *colinds, float *weights, uint32_t *connkinds)
+{
+    return guarded([&] {
+        need(g_ui && props && rowptr && colinds && weights && connkinds, "output arrays are required");
+        dtb::gen_fanout(n, k, seed, g_ui, sub_base, props, rowptr, colinds, weights, connkinds);
     });
 }
 

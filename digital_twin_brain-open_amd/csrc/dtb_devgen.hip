@@ -1,0 +1,110 @@
+// dtb_devgen.hip -- seeded fan-out networks generated directly in HBM.
+//
+// Serving admits networks faster than a host can build and upload them, so
+// this generator writes the presynaptic-major CSR on the device.  Each
+// presynaptic neuron r has exactly K synapses; draw k in [0, K) picks the
+// target t = mulhi(x0, n - 1) (+1 at or above r: no self-connection) and the
+// weight pair (x1 >> 8, x2 >> 8) 2^-24 from Philox4x32-10 with counter
+// (r, k, 1, generator tag) and key = seed.  Row r's synapses are ordered by
+// (t, k); the channel pair is fixed by the row (inhibitory rows r >= 0.8 n).
+// gen_fanout in dtb_gen.cpp is the same generator on the host.
+#include "dtb_devgen.hpp"
+#include "dtb_rng.hpp"
+
+namespace dtb {
+
+namespace {
+
+__device__ __forceinline__ uint32_t dmulhi32(uint32_t a, uint32_t b)
+{
+    return __umulhi(a, b);
+}
+
+__device__ __forceinline__ unsigned long long fanout_key(uint64_t seed, uint32_t n, uint32_t r, uint32_t k)
+{
+    const Philox4 x = philox4x32_10(r, k, 1u, kTagGen, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t t = dmulhi32(x.x, n - 1u);
+    t += (t >= r) ? 1u : 0u;
+    return ((unsigned long long)t << 16) | (unsigned long long)k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fanout_rowptr(uint32_t n, uint32_t K, int64_t *__restrict__ rowptr)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i <= n)
+        rowptr[i] = (int64_t)i * (int64_t)K;
+}
+
+// One workgroup per row (grid-stride): the row's P keys (P = power of two
+// >= K, padding sorts last) are sorted in LDS by a bitonic network, then the
+// first K are written out.
+__global__ __launch_bounds__(kBlock) void k_fanout_rows(uint32_t n, uint32_t K, uint32_t P, uint64_t seed,
+                                                        uint32_t n_exc, uint32_t *__restrict__ col,
+                                                        float2 *__restrict__ w, uint32_t *__restrict__ indeg,
+                                                        uint32_t *__restrict__ tile_runs)
+{
+    extern __shared__ unsigned long long keys[];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
+        for (uint32_t i = tid; i < P; i += kBlock)
+            keys[i] = (i < K) ? fanout_key(seed, n, r, i) : ~0ull;
+        __syncthreads();
+        for (uint32_t size = 2; size <= P; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t i = tid; i < P / 2; i += kBlock) {
+                    const uint32_t lo = 2 * i - (i & (stride - 1));
+                    const uint32_t hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const unsigned long long a = keys[lo], b = keys[hi];
+                    if ((a > b) == up) {
+                        keys[lo] = b;
+                        keys[hi] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const uint32_t kind = (r >= n_exc) ? kKindBit : 0u;
+        const int64_t base = (int64_t)r * (int64_t)K;
+        for (uint32_t i = tid; i < K; i += kBlock) {
+            const unsigned long long key = keys[i];
+            const uint32_t t = (uint32_t)(key >> 16);
+            const uint32_t k = (uint32_t)(key & 0xffffu);
+            const Philox4 x = philox4x32_10(r, k, 1u, kTagGen, (uint32_t)seed, (uint32_t)(seed >> 32));
+            col[base + i] = t | kind;
+            w[base + i] = make_float2((float)(x.y >> 8) * 0x1p-24f, (float)(x.z >> 8) * 0x1p-24f);
+            atomicAdd(&indeg[t], 1u);
+            const bool start = (i == 0) || ((uint32_t)(keys[i - 1] >> 16) >> kTileBits) != (t >> kTileBits);
+            if (start)
+                atomicAdd(&tile_runs[t >> kTileBits], 1u);
+        }
+        __syncthreads();   // keys are reused by the next row
+    }
+}
+
+}  // namespace
+
+uint32_t fanout_sort_width(uint32_t K)
+{
+    uint32_t P = 1;
+    while (P < K)
+        P <<= 1;
+    return P < 2 ? 2 : P;
+}
+
+hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *rowptr, uint32_t *col, float2 *w,
+                             uint32_t *indeg, uint32_t *tile_runs, hipStream_t stream)
+{
+    const uint32_t P = fanout_sort_width(K);
+    const uint32_t n_exc = (uint32_t)((uint64_t)n * 8u / 10u);
+    hipLaunchKernelGGL(k_fanout_rowptr, dim3((n + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, n, K, rowptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    const uint32_t grid = n < 65536u ? n : 65536u;
+    hipLaunchKernelGGL(k_fanout_rows, dim3(grid), dim3(kBlock), sizeof(unsigned long long) * P, stream, n, K, P,
+                       seed, n_exc, col, w, indeg, tile_runs);
+    return hipGetLastError();
+}
+
+}  // namespace dtb

@@ -1,0 +1,18 @@
+// dtb_devgen.hpp -- seeded fan-out networks generated on the device.
+#pragma once
+
+#include "dtb_types.hpp"
+
+namespace dtb {
+
+// Sort width of one row (power of two >= K).
+uint32_t fanout_sort_width(uint32_t K);
+
+// Writes rowptr [n+1], col [n K] (channel bit in bit 31) and w [n K] of the
+// fan-out network (n, K, seed), and adds to indeg [n] (synapses per target)
+// and tile_runs [ceil(n / 256)] (runs of one row's synapses that share a
+// target tile); both must be zero-filled by the caller.  K <= 8192.
+hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *rowptr, uint32_t *col, float2 *w,
+                             uint32_t *indeg, uint32_t *tile_runs, hipStream_t stream);
+
+}  // namespace dtb

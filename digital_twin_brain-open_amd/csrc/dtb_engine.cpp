@@ -13,6 +13,8 @@
 #include <unordered_map>
 #include <unordered_set>
 
+#include "dtb_devgen.hpp"
+#include "dtb_gen.hpp"
 #include "dtb_kernels.hpp"
 
 namespace dtb {
@@ -101,12 +103,8 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
         throw Error(kInvalidParameter, "props and rowptr are required");
     if (net.nsyn < 0 || (net.nsyn > 0 && (net.col == nullptr || net.w == nullptr || net.kind_bits == nullptr)))
         throw Error(kInvalidParameter, "synapse arrays are required");
-    if (!(dt > 0.0f))
-        throw Error(kInvalidParameter, "dt must be positive");
     n_ = (uint32_t)net.n;
     nsyn_ = net.nsyn;
-    dt_ = dt;
-    seed_ = seed;
 
     // CSR validation (rows ascending, columns in range).
     if (net.rowptr[0] != 0 || net.rowptr[net.n] != net.nsyn)
@@ -120,7 +118,37 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
         if (net.col[j] >= n_)
             throw Error(kInvalidNetwork, "column index out of range");
 
-    props_.assign(net.props, net.props + (size_t)net.n * kNumCols);
+    init_common(net.props, dt, seed, s);
+
+    rowptr_.upload(net.rowptr, (size_t)n_ + 1, s);
+    // Pack the channel bit into bit 31 of the column index so that one
+    // 4-byte load per synapse carries both.
+    const int64_t chunk = (int64_t)1 << 24;
+    std::vector<uint32_t> stage((size_t)std::min<int64_t>(std::max<int64_t>(nsyn_, 1), chunk));
+    for (int64_t base = 0; base < nsyn_; base += chunk) {
+        const int64_t m = std::min<int64_t>(chunk, nsyn_ - base);
+        for (int64_t j = 0; j < m; ++j) {
+            const int64_t g = base + j;
+            const uint32_t kind = (net.kind_bits[g >> 5] >> (g & 31)) & 1u;
+            stage[(size_t)j] = net.col[g] | (kind ? kKindBit : 0u);
+        }
+        DTB_HIP(hipMemcpyAsync(col_.mutable_gpu_data() + base, stage.data(), (size_t)m * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, s));
+        DTB_HIP(hipStreamSynchronize(s));
+    }
+    if (nsyn_ > 0)
+        w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
+    count_fine_runs(net);
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+void Instance::init_common(const float *props, float dt, uint64_t seed, hipStream_t s)
+{
+    if (!(dt > 0.0f))
+        throw Error(kInvalidParameter, "dt must be positive");
+    dt_ = dt;
+    seed_ = seed;
+    props_.assign(props, props + (size_t)n_ * kNumCols);
 
     // Populations: contiguous runs of the sub-block column
     // (configuration.cpp:1902-1930 requires each sub-block to be contiguous).
@@ -161,25 +189,26 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     derive_params(s);
     init_state(s);
 
-    rowptr_.upload(net.rowptr, (size_t)n_ + 1, s);
-    // Pack the channel bit into bit 31 of the column index so that one
-    // 4-byte load per synapse carries both.
-    const int64_t chunk = (int64_t)1 << 24;
-    std::vector<uint32_t> stage((size_t)std::min<int64_t>(std::max<int64_t>(nsyn_, 1), chunk));
-    for (int64_t base = 0; base < nsyn_; base += chunk) {
-        const int64_t m = std::min<int64_t>(chunk, nsyn_ - base);
-        for (int64_t j = 0; j < m; ++j) {
-            const int64_t g = base + j;
-            const uint32_t kind = (net.kind_bits[g >> 5] >> (g & 31)) & 1u;
-            stage[(size_t)j] = net.col[g] | (kind ? kKindBit : 0u);
-        }
-        DTB_HIP(hipMemcpyAsync(col_.mutable_gpu_data() + base, stage.data(), (size_t)m * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, s));
-        DTB_HIP(hipStreamSynchronize(s));
-    }
-    if (nsyn_ > 0)
-        w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
-    count_fine_runs(net);
+}
+
+Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
+{
+    if (f.n < 2 || f.n > (int64_t)kColMask || f.K == 0 || f.K > 8192)
+        throw Error(kInvalidParameter, "fan-out network: need n >= 2 and 1 <= K <= 8192");
+    if (f.n * (int64_t)f.K >= kMaxSynapses)
+        throw Error(kInvalidNetwork, "too many synapses for one block");
+    n_ = (uint32_t)f.n;
+    nsyn_ = f.n * (int64_t)f.K;
+    std::vector<float> props((size_t)n_ * kNumCols);
+    gen_default_props(f.n, f.g_ui, f.sub_base, props.data());
+    init_common(props.data(), dt, seed, s);
+
+    const uint32_t ntile = (n_ + (1u << kTileBits) - 1) >> kTileBits;
+    DataAllocator<uint32_t> indeg(n_), runs(ntile);
+    DTB_HIP(launch_gen_fanout(n_, f.K, f.net_seed, rowptr_.mutable_gpu_data(), col_.mutable_gpu_data(),
+                              w_.mutable_gpu_data(), indeg.mutable_gpu_data(), runs.mutable_gpu_data(), s));
+    fine_runs_.assign(ntile, 0u);
+    runs.download(fine_runs_.data(), ntile, s);
     DTB_HIP(hipStreamSynchronize(s));
 }
 

@@ -202,9 +202,21 @@ struct RunOptions {
 class Engine;
 
 // One simulated block.
+// A fan-out network generated on the device (gen_fanout / launch_gen_fanout).
+struct FanoutSpec {
+    int64_t n = 0;
+    uint32_t K = 0;
+    uint64_t net_seed = 0;
+    float g_ui[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t sub_base = 0;
+};
+
 class Instance {
 public:
     Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t stream);
+    // The network is generated in device memory (no host copy of the
+    // synapses, nothing crosses PCIe but the property matrix).
+    Instance(const FanoutSpec &spec, float dt, uint64_t seed, hipStream_t stream);
     ~Instance();
 
     uint32_t n() const { return n_; }
@@ -262,6 +274,7 @@ public:
     uint32_t tile_runs(uint32_t tile) const;
 
 private:
+    void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
     void derive_params(hipStream_t s);
     void init_state(hipStream_t s);
     void count_fine_runs(const NetworkHost &net);

@@ -225,6 +225,52 @@ void gen_uniform(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint
     build_csr(n, K, fn, rowptr, col, w, kind_bits);
 }
 
+void gen_default_props(int64_t n, const float g_ui[4], uint32_t sub_base, float *props)
+{
+    const uint32_t n_exc = (uint32_t)(n * 8 / 10);
+    for (uint32_t i = 0; i < (uint32_t)n; ++i)
+        fill_props(props, i, sub_base + (i >= n_exc ? 1u : 0u), g_ui);
+}
+
+void gen_fanout(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint32_t sub_base, float *props,
+                int64_t *rowptr, uint32_t *col, float *w, uint32_t *kind_bits)
+{
+    if (n < 2 || n > (int64_t)kColMask || K == 0 || K > 8192)
+        throw Error(kInvalidParameter, "gen_fanout: need n >= 2 and 1 <= K <= 8192");
+    gen_default_props(n, g_ui, sub_base, props);
+    const uint32_t nn = (uint32_t)n, n_exc = (uint32_t)(n * 8 / 10);
+    for (int64_t r = 0; r <= n; ++r)
+        rowptr[r] = r * (int64_t)K;
+    parallel_ranges(n, thread_count(n * (int64_t)K), [&](int, int64_t a, int64_t b) {
+        std::vector<uint64_t> keys(K);
+        for (int64_t r = a; r < b; ++r) {
+            for (uint32_t k = 0; k < K; ++k) {
+                const Philox4 x = philox4x32_10((uint32_t)r, k, 1u, kTagGen, (uint32_t)seed, (uint32_t)(seed >> 32));
+                uint32_t t = mulhi32(x.x, nn - 1);
+                t += (t >= (uint32_t)r) ? 1u : 0u;
+                keys[k] = ((uint64_t)t << 16) | k;
+            }
+            std::sort(keys.begin(), keys.end());
+            for (uint32_t i = 0; i < K; ++i) {
+                const uint32_t t = (uint32_t)(keys[i] >> 16), k = (uint32_t)(keys[i] & 0xffffu);
+                const Philox4 x = philox4x32_10((uint32_t)r, k, 1u, kTagGen, (uint32_t)seed, (uint32_t)(seed >> 32));
+                const int64_t j = r * (int64_t)K + i;
+                col[j] = t;
+                w[2 * j] = weight24(x.y);
+                w[2 * j + 1] = weight24(x.z);
+            }
+        }
+    });
+    const int64_t nsyn = n * (int64_t)K, words = (nsyn + 31) / 32;
+    for (int64_t wd = 0; wd < words; ++wd) {
+        uint32_t bits = 0;
+        for (int64_t j = wd * 32; j < std::min<int64_t>(nsyn, wd * 32 + 32); ++j)
+            if (j / K >= n_exc)
+                bits |= 1u << (j & 31);
+        kind_bits[wd] = bits;
+    }
+}
+
 void gen_voxel(int64_t n, uint32_t nvox, uint32_t K, float intra_frac, float density, uint64_t seed,
                const float g_ui[4], float *props, int64_t *rowptr, uint32_t *col, float *w, uint32_t *kind_bits,
                uint32_t *vox_sizes)

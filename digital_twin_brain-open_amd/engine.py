@@ -101,6 +101,23 @@ def gen_uniform(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
     return NetworkArrays(props, rowptr, col, w, kinds)
 
 
+def gen_fanout(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
+    """n neurons (80% E, 20% I); every presynaptic neuron has exactly k
+    synapses to uniformly drawn targets (mean in-degree k); the network that
+    Engine.add_fanout generates on the device."""
+    lib = _lib.load()
+    nsyn = int(n) * int(k)
+    props = np.empty((n, NCOL), dtype=np.float32)
+    rowptr = np.empty(n + 1, dtype=np.int64)
+    col = np.empty(nsyn, dtype=np.uint32)
+    w = np.empty((nsyn, 2), dtype=np.float32)
+    kinds = np.empty((nsyn + 31) // 32, dtype=np.uint32)
+    g = np.asarray(g_ui, dtype=np.float32)
+    _lib.check(lib.dtb_gen_fanout(n, k, seed, _ptr(g), sub_base, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w),
+                                  _ptr(kinds)))
+    return NetworkArrays(props, rowptr, col, w, kinds)
+
+
 def load_npz(path):
     """Reads the reference's block files through the native reader: one
     ``block_<i>.npz`` or a directory of ``block_0.npz .. block_<B-1>.npz``
@@ -226,6 +243,15 @@ class Engine:
         _lib.check(self._lib.dtb_instance_add(self._h, byref(s), dt, seed, byref(iid)))
         return iid.value
 
+    def add_fanout(self, n, k, net_seed, g_ui=DEFAULT_G_UI, sub_base=0, dt=0.1, seed=0):
+        """Adds the fan-out network gen_fanout(n, k, net_seed, g_ui, sub_base),
+        generated in device memory."""
+        g = np.ascontiguousarray(g_ui, dtype=np.float32)
+        iid = c_int(-1)
+        _lib.check(self._lib.dtb_instance_add_fanout(self._h, n, k, net_seed, _ptr(g), sub_base, dt, seed,
+                                                     byref(iid)))
+        return iid.value
+
     def remove(self, inst):
         _lib.check(self._lib.dtb_instance_remove(self._h, inst))
 
@@ -348,9 +374,11 @@ class Engine:
 class Block:
     """One block on one GPU with the reference client vocabulary."""
 
-    def __init__(self, net, delta_t=0.1, seed=0, device=0, engine=None):
+    def __init__(self, net, delta_t=0.1, seed=0, device=0, engine=None, inst=None):
+        """net: NetworkArrays uploaded as a new instance; or net=None with
+        `inst` an instance already on `engine` (e.g. Engine.add_fanout)."""
         self.engine = engine if engine is not None else Engine(device)
-        self.inst = self.engine.add(net, delta_t, seed)
+        self.inst = self.engine.add(net, delta_t, seed) if net is not None else inst
         self.delta_t = delta_t
         info = self.engine.info(self.inst)
         self._n = info["n"]

@@ -119,6 +119,12 @@ int dtb_engine_destroy(dtb_engine_t h);
  * + init_all_stages_gpu (src/block.cpp:36-179).  dt in ms; seed keys the
  * Philox streams (the reference seeds from /dev/urandom, src/block.cpp:129-134). */
 int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t seed, int *inst_id);
+/* Adds a block whose network is generated in device memory (the fan-out
+ * network of dtb_gen_fanout with the same arguments, bit for bit): no host
+ * copy of the synapses.  Used where networks are admitted faster than a host
+ * can build and upload them (serving).  k <= 8192. */
+int dtb_instance_add_fanout(dtb_engine_t h, int64_t n, uint32_t k, uint64_t net_seed, const float *g_ui,
+                            uint32_t sub_base, float dt, uint64_t seed, int *inst_id);
 int dtb_instance_remove(dtb_engine_t h, int inst);
 /* Neuron and population counts; pop_ids [npop] (brain ids, column 3) and
  * pop_begin [npop+1] may be NULL (sub_bids_/sub_bcounts_, include/block.hpp). */
@@ -192,6 +198,12 @@ int dtb_ou_normals(int device, uint64_t seed, uint32_t neuron0, uint64_t substep
  * (columns 10-13). */
 int dtb_gen_uniform(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint32_t sub_base, float *props,
                     int64_t *rowptr, uint32_t *colinds, float *weights, uint32_t *connkinds);
+/* Fan-out network: every presynaptic neuron has exactly k synapses to
+ * uniformly drawn targets (no self-connections; mean in-degree k), 80/20 E/I
+ * by presynaptic index, rows ordered by target, weights k 2^-24 from
+ * Philox4x32-10 keyed by seed.  Same output layout as dtb_gen_uniform. */
+int dtb_gen_fanout(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint32_t sub_base, float *props,
+                   int64_t *rowptr, uint32_t *colinds, float *weights, uint32_t *connkinds);
 int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
                   const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                   uint32_t *connkinds, uint32_t *vox_sizes);

@@ -112,3 +112,29 @@ def test_network_from_edges_roundtrip(dtb):
     assert np.array_equal(net.rowptr, src.rowptr)
     assert np.array_equal(net.col, src.col)
     assert np.array_equal(net.kinds[: len(src.kinds)], src.kinds)
+
+
+def test_gen_fanout_structure(dtb):
+    """Fan-out generator: k synapses per row, rows sorted by target, no
+    self-connections, channel by presynaptic index (80/20), weights k 2^-24,
+    deterministic."""
+    n, k = 3000, 37
+    a = dtb.gen_fanout(n, k, seed=5)
+    b = dtb.gen_fanout(n, k, seed=5)
+    c = dtb.gen_fanout(n, k, seed=6)
+    assert np.array_equal(a.col, b.col) and np.array_equal(a.w, b.w)
+    assert not np.array_equal(a.col, c.col)
+    assert np.array_equal(a.rowptr, np.arange(n + 1) * k)
+    rows = a.col.reshape(n, k)
+    assert np.all(np.diff(rows.astype(np.int64), axis=1) >= 0)
+    assert not np.any(rows == np.arange(n)[:, None])
+    kind = a.kind_per_synapse().reshape(n, k)
+    assert np.array_equal(kind[:, 0], (np.arange(n) >= n * 8 // 10).astype(np.uint8))
+    assert np.all(kind == kind[:, :1])
+    q = a.w * 2.0 ** 24
+    assert np.array_equal(q, np.floor(q)) and a.w.min() >= 0 and a.w.max() < 1
+    # targets roughly uniform: mean in-degree k
+    indeg = np.bincount(a.col, minlength=n)
+    assert abs(indeg.mean() - k) < 1e-9 and indeg.std() < 3 * np.sqrt(k)
+    pops, begin = a.populations()
+    assert list(pops) == [0, 1] and list(begin) == [0, n * 8 // 10, n]

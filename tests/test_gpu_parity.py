@@ -386,3 +386,27 @@ def test_blockwrapper_mirror(dtb, orc, gpu, tmp_path):
     assert np.array_equal(spike, ref["raster"][:, samples])
     assert np.array_equal(bits(vm), bits(ref["V"][:, samples]))
     assert ref["freq"].sum() > 100
+
+
+def test_device_generated_network_equals_host(dtb, orc, gpu):
+    """A fan-out network generated in device memory steps exactly like the
+    same network built on the host (gen_fanout) and uploaded, and both match
+    the oracle."""
+    n, k, nseed, seed, steps = 20000, 300, 77, 9, 300
+    host = dtb.gen_fanout(n, k, nseed)
+    eng = dtb.Engine(0)
+    bd = dtb.Block(None, delta_t=DT, engine=eng, inst=eng.add_fanout(n, k, nseed, dt=DT, seed=seed))
+    bh = dtb.Block(host, delta_t=DT, seed=seed, engine=eng)
+    for b in (bd, bh):
+        b.update_ou_background_stimuli(b.subblk_id, 4.0, 2.0, 0.1)
+        eng.configure(b.inst, 1, True)
+    eng.run(steps)
+    eng.check()
+    sd, sh = bd.state(), bh.state()
+    assert bd.counters() == bh.counters()
+    for key in ("V", "J", "i_ou", "I_syn"):
+        assert np.array_equal(bits(sd[key]), bits(sh[key])), key
+    eng.close()
+    ref, sim = run_oracle(orc, host, steps, seed, mu=2.0)
+    assert_state_equal(sd, sim)
+    assert ref["raster"].sum() > 1000
