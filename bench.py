@@ -127,6 +127,36 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
             "identical_counters_and_state": bool(same)}
 
 
+SERVE_REQUESTS, SERVE_RATE, SERVE_SEED, SERVE_PLAIN = 100, 20.0, 31, 12
+
+
+def serving_bench(dtb, nreq, rate, plain):
+    """Serving stream (Poisson arrivals at `rate`/s, networks of 10k-1M
+    neurons at out-degree 100-1000 generated on the device, 100-2000 ms of
+    model time each), continuous batching under the HBM budget; then the
+    first `plain` requests again, one at a time, for the plain comparison."""
+    serving = importlib.import_module("digital_twin_brain-open_amd.serving")
+    reqs = serving.poisson_requests(nreq, rate, SERVE_SEED)
+    srv = serving.Server(device=0)
+    try:
+        stats = srv.serve(reqs, realtime=True)
+    finally:
+        srv.close()
+    t_plain, ev_plain, same = 0.0, 0, True
+    for r in reqs[:plain]:
+        t0 = time.perf_counter()
+        ev, sp, vh = serving.run_alone(r)
+        t_plain += time.perf_counter() - t0
+        ev_plain += ev
+        same = same and (ev, sp, vh) == (r.events, r.spikes, r.v_hash)
+    sizes = np.array([r.n for r in reqs])
+    stats.update({"workload": "%d requests, Poisson %.0f/s, 10k-1M neurons log-uniform (mean %d), out-degree "
+                              "100-1000, 100-2000 ms of model time, OU mu 3.0" % (nreq, rate, int(sizes.mean())),
+                  "plain_requests": plain, "plain_events_per_s": ev_plain / t_plain,
+                  "plain_same_results": bool(same)})
+    return stats
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -208,8 +238,10 @@ def main():
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10000; stress 2000)")
     ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 500; stress 200)")
     ap.add_argument("--cpu-steps", type=int, default=100, help="CPU oracle sample length (0 = skip)")
-    ap.add_argument("--workload", choices=("block", "stress"), default="block",
-                    help="block: the 2M-neuron block (headline); stress: 500k neurons at in-degree 5000")
+    ap.add_argument("--workload", choices=("block", "stress", "serving"), default="block",
+                    help="block: the 2M-neuron block (headline); stress: 500k neurons at in-degree 5000; "
+                         "serving: a request stream with continuous batching")
+    ap.add_argument("--requests", type=int, default=SERVE_REQUESTS, help="serving: requests in the stream")
     ap.add_argument("--neurons", type=int, default=0, help="override the workload's neuron count")
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     args = ap.parse_args()
@@ -228,6 +260,16 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "serving":
+        t0 = time.perf_counter()
+        st = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN)
+        res = {"metric": "synaptic events/s (serving)", "value": st["events_per_s"], "unit": "synaptic events/s",
+               "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+               "data": "synthetic (seeded fan-out networks generated on the device)",
+               "config": {"workload": st["workload"], "parallelism": "single GPU"}, "serving": st}
+        log("serving %.1f s" % (time.perf_counter() - t0))
+        print(json.dumps(res), flush=True)
+        return
     if args.neurons == 0:
         args.neurons = N_NEURONS if args.workload == "block" else STRESS_NEURONS
     if args.steps <= 0:

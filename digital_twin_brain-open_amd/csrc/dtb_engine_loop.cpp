@@ -149,7 +149,10 @@ void Engine::rebuild()
     d_samples_.resize(std::max<size_t>(samples.size(), 1));
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
     d_qcnt_.resize(2 * std::max<size_t>(tiles.size(), 1));
-    d_queue_.resize((size_t)(2 * qtotal_));
+    // The queues only grow: when instances come and go (serving) the buffer
+    // is reused instead of being freed, reallocated and zero-filled.
+    if (d_queue_.count() < (size_t)(2 * qtotal_))
+        d_queue_.resize((size_t)(2 * qtotal_));
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
     d_qcnt_.mutable_gpu_data();

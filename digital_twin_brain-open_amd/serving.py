@@ -1,0 +1,168 @@
+"""Serving loop: simulation requests admitted by HBM budget and stepped
+together on one GPU (continuous batching).
+
+A request is a seeded network (generated in device memory, Engine.add_fanout)
+to be simulated for a number of steps.  Requests wait in arrival order; a
+request is admitted as soon as it has arrived and its device footprint fits
+the HBM budget.  All admitted requests are stepped by the same launches (one
+launch per step covers every resident instance); each request leaves at
+exactly its own step count, its result is read back and its memory is
+returned, and waiting requests are admitted between chunks of steps.  A
+request's trajectory does not depend on what else is resident: every
+instance has its own step counter and random streams, and a change of the
+resident set first folds every pending input exactly (Engine settle), so a
+result is the same alone and in the mix.
+"""
+import collections
+import ctypes
+import time
+
+import numpy as np
+
+from .engine import DEFAULT_G_UI, Block, Engine
+
+# Device bytes per neuron of an instance: parameter class, V, i_ou,
+# refractory counter, flag, J, pending delta, blocked flag, row offset and
+# per-wave counters; per synapse: column index, weight pair and the two
+# propagation queues (at most one 8-byte entry per synapse each).
+BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 1 + 8 + 1
+BYTES_PER_SYNAPSE = 4 + 8 + 2 * 8
+
+
+class Request:
+    __slots__ = ("rid", "arrival", "n", "k", "steps", "net_seed", "seed", "mu", "sigma", "tau", "g_ui",
+                 "admitted", "finished", "events", "spikes", "v_hash", "v_mean")
+
+    def __init__(self, rid, arrival, n, k, steps, net_seed, seed, mu, sigma=0.1, tau=4.0, g_ui=DEFAULT_G_UI):
+        self.rid, self.arrival, self.n, self.k, self.steps = rid, arrival, int(n), int(k), int(steps)
+        self.net_seed, self.seed, self.mu, self.sigma, self.tau = int(net_seed), int(seed), mu, sigma, tau
+        self.g_ui = tuple(g_ui)
+        self.admitted = self.finished = None
+        self.events = self.spikes = None
+        self.v_hash = self.v_mean = None
+
+    def device_bytes(self):
+        return self.n * BYTES_PER_NEURON + self.n * self.k * BYTES_PER_SYNAPSE
+
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def poisson_requests(count, rate, seed, n_range=(10_000, 1_000_000), k_range=(100, 1000),
+                     duration_ms=(100.0, 2000.0), dt=0.1, mu=3.0):
+    """Seeded request stream: Poisson arrivals at `rate` per second, sizes
+    log-uniform in n_range, out-degree uniform in k_range, duration uniform
+    in duration_ms of model time."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    n = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), count)).astype(np.int64)
+    k = rng.integers(k_range[0], k_range[1] + 1, count)
+    steps = np.round(rng.uniform(duration_ms[0], duration_ms[1], count) / dt).astype(np.int64)
+    return [Request(i, float(t[i]), n[i], k[i], steps[i], net_seed=int(seed) * 1_000_003 + i,
+                    seed=int(seed) * 7_919 + i, mu=mu) for i in range(count)]
+
+
+def device_free_bytes(device=0):
+    """Free device memory reported by the HIP runtime."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    hip.hipSetDevice(device)
+    if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value
+
+
+class Server:
+    """Continuous batching of simulation requests on one engine."""
+
+    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1):
+        self.engine = Engine(device)
+        self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes(device))
+        self.chunk = int(chunk)
+        self.dt = dt
+
+    def close(self):
+        self.engine.close()
+
+    def _admit(self, r):
+        eng = self.engine
+        inst = eng.add_fanout(r.n, r.k, r.net_seed, g_ui=r.g_ui, dt=self.dt, seed=r.seed)
+        b = Block(None, delta_t=self.dt, engine=eng, inst=inst)
+        b.update_ou_background_stimuli(b.subblk_id, r.tau, r.mu, r.sigma)
+        eng.configure(inst, 1, True)
+        return b
+
+    def _retire(self, r, b):
+        eng = self.engine
+        step, events, spikes = eng.counters(b.inst)
+        assert step == r.steps
+        V = eng.read_state(b.inst, r.n)["V"]
+        r.events, r.spikes = events, spikes
+        r.v_hash = int(np.bitwise_xor.reduce(V.view(np.uint32)))
+        r.v_mean = float(V.astype(np.float64).mean())
+        eng.remove(b.inst)
+
+    def serve(self, requests, realtime=True, max_resident=None):
+        """Runs the stream to completion; arrival times are honoured on the
+        wall clock when realtime (otherwise every request has arrived at
+        start).  Returns summary statistics; per-request results and times
+        are filled into the Request objects."""
+        for r in requests:
+            if r.device_bytes() > self.budget:
+                raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.device_bytes()))
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        active = {}    # inst -> (request, block, remaining steps)
+        used = 0
+        t0 = time.monotonic()
+        offset = 0.0 if realtime else min(r.arrival for r in requests)
+        steps_run = 0
+        resident_steps = 0
+        while pending or active:
+            now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
+            while (pending and pending[0].arrival - offset <= now and used + pending[0].device_bytes() <= self.budget
+                   and (max_resident is None or len(active) < max_resident)):
+                r = pending.popleft()
+                b = self._admit(r)
+                r.admitted = time.monotonic() - t0 + offset
+                active[b.inst] = [r, b, r.steps]
+                used += r.device_bytes()
+            if not active:
+                wait = pending[0].arrival - offset - (time.monotonic() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            s = min(self.chunk, min(v[2] for v in active.values()))
+            self.engine.run(s, graph_chunk=0)
+            self.engine.sync()
+            steps_run += s
+            resident_steps += s * len(active)
+            done = []
+            for inst, v in active.items():
+                v[2] -= s
+                if v[2] == 0:
+                    done.append(inst)
+            for inst in done:
+                r, b, _ = active.pop(inst)
+                self._retire(r, b)
+                r.finished = time.monotonic() - t0 + offset
+                used -= r.device_bytes()
+        wall = time.monotonic() - t0
+        lat = np.array([r.latency() for r in requests])
+        events = sum(r.events for r in requests)
+        return {"requests": len(requests), "wall_s": wall, "events": int(events),
+                "events_per_s": events / wall,
+                "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "mean_resident": resident_steps / max(steps_run, 1), "engine_steps": steps_run}
+
+
+def run_alone(request, device=0, dt=0.1):
+    """The request on an engine of its own: (events, spikes, V hash)."""
+    srv = Server(device=device, hbm_budget=1 << 62, chunk=max(1, request.steps), dt=dt)
+    try:
+        r = Request(request.rid, 0.0, request.n, request.k, request.steps, request.net_seed, request.seed,
+                    request.mu, request.sigma, request.tau, request.g_ui)
+        srv.serve([r], realtime=False)
+        return r.events, r.spikes, r.v_hash
+    finally:
+        srv.close()
