@@ -142,17 +142,19 @@ def serving_bench(dtb, nreq, rate, plain):
         stats = srv.serve(reqs, realtime=True)
     finally:
         srv.close()
-    t_plain, ev_plain, same = 0.0, 0, True
+    t_plain, ev_plain, ns_plain, same = 0.0, 0, 0, True
     for r in reqs[:plain]:
         t0 = time.perf_counter()
         ev, sp, vh = serving.run_alone(r)
         t_plain += time.perf_counter() - t0
         ev_plain += ev
+        ns_plain += r.n * r.steps
         same = same and (ev, sp, vh) == (r.events, r.spikes, r.v_hash)
     sizes = np.array([r.n for r in reqs])
     stats.update({"workload": "%d requests, Poisson %.0f/s, 10k-1M neurons log-uniform (mean %d), out-degree "
                               "100-1000, 100-2000 ms of model time, OU mu 3.0" % (nreq, rate, int(sizes.mean())),
                   "plain_requests": plain, "plain_events_per_s": ev_plain / t_plain,
+                  "plain_neuron_steps_per_s": ns_plain / t_plain,
                   "plain_same_results": bool(same)})
     return stats
 

@@ -7,6 +7,7 @@
 // weight pair (x1 >> 8, x2 >> 8) 2^-24 from Philox4x32-10 with counter
 // (r, k, 1, generator tag) and key = seed.  Row r's synapses are ordered by
 // (t, k); the channel pair is fixed by the row (inhibitory rows r >= 0.8 n).
+// Per-target in-degrees are accumulated for the queue capacities.
 // gen_fanout in dtb_gen.cpp is the same generator on the host.
 #include "dtb_devgen.hpp"
 #include "dtb_rng.hpp"
@@ -40,8 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_fanout_rowptr(uint32_t n, uint32_t K
 // first K are written out.
 __global__ __launch_bounds__(kBlock) void k_fanout_rows(uint32_t n, uint32_t K, uint32_t P, uint64_t seed,
                                                         uint32_t n_exc, uint32_t *__restrict__ col,
-                                                        float2 *__restrict__ w, uint32_t *__restrict__ indeg,
-                                                        uint32_t *__restrict__ tile_runs)
+                                                        float2 *__restrict__ w, uint32_t *__restrict__ indeg)
 {
     extern __shared__ unsigned long long keys[];
     const uint32_t tid = threadIdx.x;
@@ -74,9 +74,6 @@ __global__ __launch_bounds__(kBlock) void k_fanout_rows(uint32_t n, uint32_t K, 
             col[base + i] = t | kind;
             w[base + i] = make_float2((float)(x.y >> 8) * 0x1p-24f, (float)(x.z >> 8) * 0x1p-24f);
             atomicAdd(&indeg[t], 1u);
-            const bool start = (i == 0) || ((uint32_t)(keys[i - 1] >> 16) >> kTileBits) != (t >> kTileBits);
-            if (start)
-                atomicAdd(&tile_runs[t >> kTileBits], 1u);
         }
         __syncthreads();   // keys are reused by the next row
     }
@@ -93,7 +90,7 @@ uint32_t fanout_sort_width(uint32_t K)
 }
 
 hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *rowptr, uint32_t *col, float2 *w,
-                             uint32_t *indeg, uint32_t *tile_runs, hipStream_t stream)
+                             uint32_t *indeg, hipStream_t stream)
 {
     const uint32_t P = fanout_sort_width(K);
     const uint32_t n_exc = (uint32_t)((uint64_t)n * 8u / 10u);
@@ -103,7 +100,7 @@ hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *row
         return e;
     const uint32_t grid = n < 65536u ? n : 65536u;
     hipLaunchKernelGGL(k_fanout_rows, dim3(grid), dim3(kBlock), sizeof(unsigned long long) * P, stream, n, K, P,
-                       seed, n_exc, col, w, indeg, tile_runs);
+                       seed, n_exc, col, w, indeg);
     return hipGetLastError();
 }
 

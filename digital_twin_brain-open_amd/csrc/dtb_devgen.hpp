@@ -9,10 +9,9 @@ namespace dtb {
 uint32_t fanout_sort_width(uint32_t K);
 
 // Writes rowptr [n+1], col [n K] (channel bit in bit 31) and w [n K] of the
-// fan-out network (n, K, seed), and adds to indeg [n] (synapses per target)
-// and tile_runs [ceil(n / 256)] (runs of one row's synapses that share a
-// target tile); both must be zero-filled by the caller.  K <= 8192.
+// fan-out network (n, K, seed), and adds to indeg [n] the synapses per
+// target (zero-filled by the caller).  K <= 8192.
 hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *rowptr, uint32_t *col, float2 *w,
-                             uint32_t *indeg, uint32_t *tile_runs, hipStream_t stream);
+                             uint32_t *indeg, hipStream_t stream);
 
 }  // namespace dtb

@@ -113,7 +113,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
         throw Error(kInvalidNetwork, "too many synapses for one block");
     for (int64_t i = 0; i < net.n; ++i)
         if (net.rowptr[i + 1] < net.rowptr[i] || net.rowptr[i + 1] - net.rowptr[i] >= kMaxRowLength)
-            throw Error(kInvalidNetwork, "rowptr must be non-decreasing with rows below 2^24 synapses");
+            throw Error(kInvalidNetwork, "rowptr must be non-decreasing with rows below 2^31 synapses");
     for (int64_t j = 0; j < net.nsyn; ++j)
         if (net.col[j] >= n_)
             throw Error(kInvalidNetwork, "column index out of range");
@@ -138,7 +138,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
     }
     if (nsyn_ > 0)
         w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
-    count_fine_runs(net);
+    count_tile_synapses(net);
     DTB_HIP(hipStreamSynchronize(s));
 }
 
@@ -204,24 +204,26 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
     init_common(props.data(), dt, seed, s);
 
     const uint32_t ntile = (n_ + (1u << kTileBits) - 1) >> kTileBits;
-    DataAllocator<uint32_t> indeg(n_), runs(ntile);
+    DataAllocator<uint32_t> indeg(n_);
     DTB_HIP(launch_gen_fanout(n_, f.K, f.net_seed, rowptr_.mutable_gpu_data(), col_.mutable_gpu_data(),
-                              w_.mutable_gpu_data(), indeg.mutable_gpu_data(), runs.mutable_gpu_data(), s));
-    fine_runs_.assign(ntile, 0u);
-    runs.download(fine_runs_.data(), ntile, s);
+                              w_.mutable_gpu_data(), indeg.mutable_gpu_data(), s));
+    std::vector<uint32_t> in(n_);
+    indeg.download(in.data(), n_, s);
     DTB_HIP(hipStreamSynchronize(s));
+    tile_syn_.assign(ntile, 0u);
+    for (uint32_t i = 0; i < n_; ++i)
+        tile_syn_[i >> kTileBits] += in[i];
 }
 
 Instance::~Instance() = default;
 
-void Instance::count_fine_runs(const NetworkHost &net)
+void Instance::count_tile_synapses(const NetworkHost &net)
 {
-    // One run = maximal sequence of consecutive synapses of a row whose
-    // targets fall in the same tile.  The binning emits exactly one queue item
-    // per run, so these counts are the exact queue capacities whatever the
-    // order of synapses within a row.
+    // Synapses per target tile: a tile's queue holds at most one entry per
+    // synapse that targets it (every row fires at most once per step), so
+    // these counts are its exact capacity.
     const uint32_t nfine = (n_ + (1u << kTileBits) - 1) >> kTileBits;
-    fine_runs_.assign(nfine, 0u);
+    tile_syn_.assign(nfine, 0u);
     unsigned hw = std::thread::hardware_concurrency();
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw ? hw : 4u, 16u), nsyn_ / (1 << 20)));
     std::vector<std::vector<uint32_t>> part(nt, std::vector<uint32_t>(nfine, 0u));
@@ -230,27 +232,20 @@ void Instance::count_fine_runs(const NetworkHost &net)
         th.emplace_back([&, t]() {
             const int64_t a = (int64_t)n_ * t / nt, b = (int64_t)n_ * (t + 1) / nt;
             uint32_t *c = part[t].data();
-            for (int64_t r = a; r < b; ++r) {
-                uint32_t prev = UINT32_MAX;
-                for (int64_t j = net.rowptr[r]; j < net.rowptr[r + 1]; ++j) {
-                    const uint32_t tile = net.col[j] >> kTileBits;
-                    if (tile != prev)
-                        ++c[tile];
-                    prev = tile;
-                }
-            }
+            for (int64_t j = net.rowptr[a]; j < net.rowptr[b]; ++j)
+                ++c[net.col[j] >> kTileBits];
         });
     }
     for (auto &x : th)
         x.join();
     for (int t = 0; t < nt; ++t)
         for (uint32_t k = 0; k < nfine; ++k)
-            fine_runs_[k] += part[t][k];
+            tile_syn_[k] += part[t][k];
 }
 
-uint32_t Instance::tile_runs(uint32_t tile) const
+uint32_t Instance::tile_synapses(uint32_t tile) const
 {
-    return fine_runs_.at(tile);
+    return tile_syn_.at(tile);
 }
 
 namespace {

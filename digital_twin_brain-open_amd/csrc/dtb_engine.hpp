@@ -269,15 +269,15 @@ public:
     // Distinct parameter records (classes); n() when stored per neuron.
     uint32_t param_classes() const { return (uint32_t)pA_.count(); }
     SampleRecDev sample_rec();
-    // Runs of consecutive synapses (of one row) whose targets share a tile of
-    // 2^kTileBits neurons, per tile: the exact queue bound of the binning.
-    uint32_t tile_runs(uint32_t tile) const;
+    // Synapses whose target lies in tile `tile` (2^kTileBits neurons): the
+    // exact queue bound of the binning.
+    uint32_t tile_synapses(uint32_t tile) const;
 
 private:
     void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
     void derive_params(hipStream_t s);
     void init_state(hipStream_t s);
-    void count_fine_runs(const NetworkHost &net);
+    void count_tile_synapses(const NetworkHost &net);
     uint32_t find_pop(uint32_t brain_id) const;
 
     uint32_t n_ = 0;
@@ -316,7 +316,7 @@ private:
     int64_t rec_step0_ = 0;
     int64_t rec_len_ = 0;
     bool rec_freq_ = false, rec_samples_ = false;
-    std::vector<uint32_t> fine_runs_;
+    std::vector<uint32_t> tile_syn_;      // synapses per target tile
 };
 
 // Steps a set of instances together.  One launch per step covers every
@@ -382,7 +382,7 @@ private:
     DataAllocator<SampleRecDev> d_recs_;
     DataAllocator<TileDesc> d_tiles_;
     DataAllocator<uint32_t> d_qcnt_;       // [2][ntiles]
-    DataAllocator<QItem> d_queue_;         // [2][qtotal]
+    DataAllocator<QSyn> d_queue_;          // [2][qtotal]
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
     int graph_steps_[2] = {0, 0};
 };

@@ -109,9 +109,6 @@ void Engine::rebuild()
         Instance *p = insts_[slot].get();
         if (!p)
             continue;
-        // Both step slots hold the current step, whatever parity the engine
-        // is at.
-        p->set_step(p->step(stream_), stream_);
         const uint32_t idx = (uint32_t)table.size();
         tile_base_[slot] = (uint32_t)tiles.size();
         table.push_back(p->dev((uint32_t)tiles.size()));
@@ -126,7 +123,7 @@ void Engine::rebuild()
             d.begin = b;
             d.count = std::min<uint32_t>(kBlock, p->n() - b);
             d.pop0 = q;
-            d.qcap = p->tile_runs(t);
+            d.qcap = p->tile_synapses(t);
             d.qoff = qtotal;
             qtotal += d.qcap;
             tiles.push_back(d);

@@ -12,18 +12,19 @@
 //
 // One launch per step here (k_step).  Every workgroup of 256 threads owns a
 // tile of 256 consecutive neurons of one instance, and per tile:
-//   1. expands the tile's queued synapse runs across the lanes and adds the
-//      weights into exact Q23.40 accumulators in LDS (64-bit integer LDS
+//   1. reads the tile's queue (the synapses delivered to it: target index,
+//      channel pair, weight pair; contiguous, so the reads coalesce) and adds
+//      the weights into exact Q23.40 accumulators in LDS (64-bit integer LDS
 //      atomics, so the sum does not depend on the order);
 //   2. folds the delta into J (J = J * decay + delta), computes I_syn, runs
 //      OU + LIF for every substep;
-//   3. every wave splits the rows of its fired neurons (their synapses are
-//      ordered by target) into runs whose targets lie in one tile and
-//      appends each run to that tile's queue for the next launch (8-byte
-//      items).
+//   3. every wave reads the rows of its fired neurons (synapses ordered by
+//      target), reserves one contiguous segment per target tile with an
+//      atomic add on that tile's count and writes each synapse's entry into
+//      the target tile's queue for the next launch (12 bytes per synapse).
 // The queues are double-buffered: the launch of parity p drains set p and
-// fills set p ^ 1, so step k consumes exactly the runs queued from the spikes
-// of step k - 1.  No global atomics touch the delta.
+// fills set p ^ 1, so step k consumes exactly the synapses queued from the
+// spikes of step k - 1.  No global atomics touch the delta.
 //   k_materialize writes the pending delta of every neuron (without consuming
 //   the queues) for sampling and state read-back; k_sample gathers samples.
 #include <hip/hip_ext.h>
@@ -32,6 +33,33 @@
 #include "dtb_rng.hpp"
 
 namespace dtb {
+
+// Pointers read from the instance descriptors are generic; accessing them
+// through the global address space lets the compiler emit global (not flat)
+// memory instructions, which do not share the LDS wait counter.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+typedef float gfloat4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 ld4(const float4 *p, size_t i)
+{
+    const gfloat4 v = *(const __attribute__((address_space(1))) gfloat4 *)(p + i);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ void st4(float4 *p, size_t i, const float4 v)
+{
+    gfloat4 g;
+    g.x = v.x;
+    g.y = v.y;
+    g.z = v.z;
+    g.w = v.w;
+    *(__attribute__((address_space(1))) gfloat4 *)(p + i) = g;
+}
 
 __device__ __forceinline__ float q40_to_f32(long long q)
 {
@@ -43,20 +71,11 @@ __device__ __forceinline__ unsigned long long f32_to_q40(float w)
     return (unsigned long long)(long long)rintf(w * 0x1p40f);
 }
 
-__device__ __forceinline__ int64_t qitem_begin(QItem q)
-{
-    return (int64_t)(q & kQBeginMask);
-}
-
-__device__ __forceinline__ uint32_t qitem_len(QItem q)
-{
-    return (uint32_t)(q >> kQBeginBits);
-}
 
 // Row of the parameter records that holds neuron li's parameters.
 __device__ __forceinline__ uint32_t param_index(const InstDev &I, uint32_t li)
 {
-    return I.cls != nullptr ? (uint32_t)I.cls[li] : li;
+    return I.cls != nullptr ? (uint32_t)gp(I.cls)[li] : li;
 }
 
 // synaptic_current.cpp:39-43 in source order, no contraction.
@@ -99,68 +118,19 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, uint32_t lane
     return v;
 }
 
-// LDS scratch of the run expansion.
-struct RunScratch {
-    int64_t begin[kBlock];
-    uint32_t off[kBlock];
-    uint32_t wave_total[kBlock / kWave];
-};
-
-// Adds the queued runs of one tile into acc[count][4] (Q23.40, LDS).  The
-// workgroup takes 256 runs at a time (one per thread), scans their lengths
-// across the workgroup, and every thread then finds the run of each synapse
-// it handles by binary search over the 256 run offsets, so short runs still
-// keep every lane busy.  Called by all threads with a uniform cnt.
-__device__ __forceinline__ void accumulate_tile(unsigned long long *acc, RunScratch &rs,
-                                                const QItem *__restrict__ q, uint32_t cnt, uint32_t tile_begin,
-                                                uint32_t count, const uint32_t *__restrict__ col,
-                                                const float2 *__restrict__ wt, uint32_t tid, uint32_t lane,
-                                                uint32_t w)
+// Adds the queued synapses of one tile into acc[256][4] (Q23.40, LDS): every
+// thread takes every 256th entry of the tile's queue (contiguous, so the
+// reads coalesce) and adds its weight pair with 64-bit integer LDS atomics,
+// so the sum does not depend on the order.  Threads tid < npre already hold
+// entry tid in `pre`.
+__device__ __forceinline__ void accumulate_tile(unsigned long long *acc, const QSyn *__restrict__ q, uint32_t cnt,
+                                                const QSyn &pre, uint32_t npre, uint32_t tid)
 {
-    for (uint32_t b = 0; b < cnt; b += kBlock) {
-        const uint32_t k = b + tid;
-        int64_t beg = 0;
-        uint32_t len = 0;
-        if (k < cnt) {
-            const QItem it = q[k];
-            beg = qitem_begin(it);
-            len = qitem_len(it);
-        }
-        const uint32_t incl = wave_incl_scan_u32(len, lane);
-        if (lane == kWave - 1)
-            rs.wave_total[w] = incl;
-        __syncthreads();
-        uint32_t base = 0, total = 0;
-#pragma unroll
-        for (uint32_t x = 0; x < kBlock / kWave; ++x) {
-            const uint32_t t = rs.wave_total[x];
-            base += (x < w) ? t : 0u;
-            total += t;
-        }
-        rs.begin[tid] = beg;
-        rs.off[tid] = base + incl - len;
-        __syncthreads();
-        for (uint32_t v = tid; v < total; v += kBlock) {
-            // largest r with off[r] <= v (runs past cnt have offset total)
-            uint32_t lo = 0, hi = kBlock - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (rs.off[mid] <= v)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int64_t s = rs.begin[lo] + (int64_t)(v - rs.off[lo]);
-            const uint32_t c = col[s];
-            const float2 x = wt[s];
-            const uint32_t loc = (c & kColMask) - tile_begin;
-            if (loc < count) {
-                const uint32_t ch = (c >> 31) << 1;
-                atomicAdd(&acc[loc * 4u + ch], f32_to_q40(x.x));
-                atomicAdd(&acc[loc * 4u + ch + 1u], f32_to_q40(x.y));
-            }
-        }
-        __syncthreads();
+    for (uint32_t k = tid; k < cnt; k += kBlock) {
+        const QSyn e = (k < npre) ? pre : q[k];
+        const uint32_t slot = (e.key & 0xffu) * 4u + ((e.key & kQSynKind) ? 2u : 0u);
+        atomicAdd(&acc[slot], f32_to_q40(e.w0));
+        atomicAdd(&acc[slot + 1u], f32_to_q40(e.w1));
     }
 }
 
@@ -221,40 +191,53 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
-// Queues the synapses of one fired row for the next launch.  The row's
-// synapses are ordered by target, so they split into runs whose targets lie
-// in one tile; every run is appended (first synapse, length) to its tile's
-// queue.  Called by a whole wave with wave-uniform arguments.  Lane l holds
-// synapses c0 + 4l .. c0 + 4l + 3 of the row (256 per pass).  A run ends where
-// the next synapse's tile differs; its start is the last run start at or
-// before it (within the lane, else from a wave max-scan of the lanes' last
-// starts, else carried from the previous pass).
-__device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, int64_t row_begin, uint32_t len,
-                                        uint32_t tile_base, const TileDesc *__restrict__ tiles,
-                                        uint32_t *__restrict__ qcnt, QItem *__restrict__ queue,
-                                        StepCtl *__restrict__ ctl, uint32_t lane)
+// Queues the synapses of one fired row for the next launch: each synapse's
+// target index within its tile, channel pair and weight pair go into the
+// target tile's queue.  The row's synapses are ordered by target, so the
+// synapses of one tile form segments; each segment reserves its slots with
+// one atomic add on the tile's count and its synapses are stored
+// contiguously.  Called by a whole wave with wave-uniform arguments.  Lane l
+// holds synapses c0 + 4l .. c0 + 4l + 3 of the row (256 per pass; segments
+// end at pass boundaries).  A synapse's segment start is the last start at
+// or before it (within the lane, else from a wave max-scan of the lanes' last
+// starts); the lane holding the segment's last synapse reserves and
+// publishes the destination in seg_dst (this wave's 256 LDS entries, indexed
+// by segment start within the pass).
+__device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+                                        int64_t row_begin, uint32_t len, uint32_t tile_base,
+                                        const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt,
+                                        QSyn *__restrict__ queue, StepCtl *__restrict__ ctl,
+                                        unsigned long long *seg_dst, uint32_t lane)
 {
     constexpr uint32_t K = 4;
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
-    const uint32_t *__restrict__ col = col_all + row_begin;
-    uint32_t carry_tile = 0xfffffffcu;   // tile of the synapse before the pass
-    uint32_t carry_begin = 0;            // start of the run open at the pass boundary
+    constexpr unsigned long long kDrop = ~0ull;
+    const auto col = gp(col_all) + row_begin;
+    const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
     for (uint32_t c0 = 0; c0 < len; c0 += kWave * K) {
         const uint32_t j0 = c0 + lane * K;
-        uint32_t tl[K];
+        uint32_t cv[K], tl[K];
+        float2 wv[K];
 #pragma unroll
-        for (uint32_t k = 0; k < K; ++k)
-            tl[k] = (j0 + k < len) ? ((col[j0 + k] & kColMask) >> kTileBits) : (0xfffffff0u + k);
-        // neighbours across lanes
+        for (uint32_t k = 0; k < K; ++k) {
+            if (j0 + k < len) {
+                cv[k] = col[j0 + k];
+                const unsigned long long wb = wt[j0 + k];
+                wv[k] = make_float2(__uint_as_float((uint32_t)wb), __uint_as_float((uint32_t)(wb >> 32)));
+                tl[k] = (cv[k] & kColMask) >> kTileBits;
+            } else {
+                cv[k] = 0u;
+                wv[k] = make_float2(0.0f, 0.0f);
+                tl[k] = 0xfffffff0u + k;
+            }
+        }
+        // neighbours across lanes; the pass boundaries end segments
         uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
         uint32_t nextK = __shfl_down(tl[0], 1, kWave);
         if (lane == 0)
-            prev0 = carry_tile;
-        if (lane == kWave - 1) {
-            const uint32_t jn = c0 + kWave * K;
-            nextK = (jn < len) ? ((col[jn] & kColMask) >> kTileBits) : 0xffffffe0u;
-        }
-        // last start within this lane (absolute position), or none
+            prev0 = 0xffffffe0u;
+        if (lane == kWave - 1)
+            nextK = 0xffffffe1u;
         uint32_t last_start = kNone;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
@@ -273,12 +256,8 @@ __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, in
         uint32_t before = __shfl_up(sc, 1, kWave);
         if (lane == 0)
             before = 0u;
-        uint32_t open = (before != 0u) ? before - 1u : carry_begin;   // run open at the lane's first synapse
-        // Reserve all of this lane's queue slots first (independent atomics
-        // and loads in flight together), then store the items.
-        uint32_t pos[K], cap[K];
-        uint64_t qo[K];
-        QItem item[K];
+        uint32_t open = (before != 0u) ? before - 1u : c0;   // segment open at the lane's first synapse
+        uint32_t st[K];
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t j = j0 + k;
@@ -286,29 +265,45 @@ __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, in
             const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
             if (j < len && tl[k] != pv)
                 open = j;
-            cap[k] = 0u;
-            pos[k] = 1u;
+            st[k] = open;
             if (j < len && tl[k] != nx) {
                 const uint32_t tg = tile_base + tl[k];
-                pos[k] = atomicAdd(&qcnt[tg], 1u);
-                qo[k] = tiles[tg].qoff;
-                cap[k] = tiles[tg].qcap;
-                item[k] = ((QItem)(j + 1u - open) << kQBeginBits) | (QItem)(row_begin + (int64_t)open);
+                const uint32_t n = j + 1u - open;
+                const uint32_t base = atomicAdd(&qcnt[tg], n);
+                const TileDesc td = tiles[tg];
+                unsigned long long dst = td.qoff + base;
+                if (base + n > td.qcap) {
+                    dst = kDrop;
+                    ctl->overflow = 1u;
+                }
+                seg_dst[open - c0] = dst;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
-            if (pos[k] < cap[k])
-                queue[qo[k] + pos[k]] = item[k];
-            else if (cap[k] != 0u)
-                ctl->overflow = 1u;
+            const uint32_t j = j0 + k;
+            if (j < len) {
+                const unsigned long long dst = seg_dst[st[k] - c0];
+                if (dst != kDrop) {
+                    QSyn e;
+                    e.key = (cv[k] & 0xffu) | ((cv[k] & kKindBit) ? kQSynKind : 0u);
+                    e.w0 = wv[k].x;
+                    e.w1 = wv[k].y;
+                    queue[dst + (j - st[k])] = e;
+                }
+            }
         }
-        const uint32_t wmax = __shfl(sc, kWave - 1, kWave);
-        if (wmax != 0u)
-            carry_begin = wmax - 1u;
-        carry_tile = __shfl(tl[K - 1], kWave - 1, kWave);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
+
+// Queue entries per tile read ahead together with the count.
+constexpr uint32_t kPrefetch = 128;
 
 // The step of launch parity `par`: drains queue set par, fills set par ^ 1,
 // reads the step index from step[par] and advances it into step[par ^ 1].
@@ -319,24 +314,29 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
                                                  StepCtl *__restrict__ ctl)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ RunScratch rs;
+    __shared__ unsigned long long seg_dst[4 * kBlock];   // per wave: 64 lanes x 4 synapses
 
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
-    const QItem *__restrict__ q_in = qs.items + (size_t)par * qs.qtotal;
+    const QSyn *__restrict__ q_in = qs.items + (size_t)par * qs.qtotal;
     uint32_t *__restrict__ cnt_in = qs.count + (size_t)par * qs.ntiles;
-    QItem *__restrict__ q_out = qs.items + (size_t)(par ^ 1u) * qs.qtotal;
+    QSyn *__restrict__ q_out = qs.items + (size_t)(par ^ 1u) * qs.qtotal;
     uint32_t *__restrict__ cnt_out = qs.count + (size_t)(par ^ 1u) * qs.ntiles;
 
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
-    const int64_t step = I.step[par];
+    const int64_t step = gp(I.step)[par];
     if (d.begin == 0 && tid == 0)
-        I.step[par ^ 1u] = step + 1;   // no tile of this launch reads this slot
+        gp(I.step)[par ^ 1u] = step + 1;   // no tile of this launch reads this slot
 
+    // The queue head is read together with the count (the queue is complete
+    // when the launch starts; entries past the count are ignored).
+    QSyn head{};
+    if (tid < kPrefetch && tid < d.qcap)
+        head = q_in[d.qoff + tid];
     // Per-neuron loads first: they are independent of the accumulation.
     float4 pa, pb, g, vu, dc, J;
     float V = 0.0f, iou = 0.0f;
@@ -344,16 +344,16 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     uint32_t fl = 0;
     if (active) {
         const uint32_t c = param_index(I, li);
-        pa = I.pA[c];
-        pb = I.pB[c];
-        g = I.gU[c];
-        vu = I.vU[c];
-        dc = I.dec[c];
-        J = I.J[li];
-        V = I.V[li];
-        t = I.tcnt[li];
-        fl = I.flag[li];
-        iou = I.iou[li];
+        pa = ld4(I.pA, c);
+        pb = ld4(I.pB, c);
+        g = ld4(I.gU, c);
+        vu = ld4(I.vU, c);
+        dc = ld4(I.dec, c);
+        J = ld4(I.J, li);
+        V = gp(I.V)[li];
+        t = gp(I.tcnt)[li];
+        fl = gp(I.flag)[li];
+        iou = gp(I.iou)[li];
     }
 
     acc[tid * 4 + 0] = 0ull;
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(cnt_in[blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, q_in + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid, lane, w);
+    accumulate_tile(acc, q_in + d.qoff, cnt, head, kPrefetch, tid);
     __syncthreads();
     if (tid == 0)
         cnt_in[blockIdx.x] = 0u;   // this launch consumed the queue
@@ -433,11 +433,11 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             const float u = noise_uniform(I.seed, li, (uint64_t)step);
             emit = f || (u < pb.z);
         }
-        I.V[li] = V;
-        I.tcnt[li] = t;
-        I.flag[li] = (uint8_t)(f | (emit ? kFlagEmit : 0u));
-        I.iou[li] = iou;
-        I.J[li] = J;
+        gp(I.V)[li] = V;
+        gp(I.tcnt)[li] = t;
+        gp(I.flag)[li] = (uint8_t)(f | (emit ? kFlagEmit : 0u));
+        gp(I.iou)[li] = iou;
+        st4(I.J, li, J);
         stv[0] = stat_q(V);
     }
 
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         // stat_result.cpp:266-391: V of this step, I_syn-based statistics of
         // the previous step (its J is complete only now), blocked neurons
         // excluded.
-        const bool valid = active && !I.blocked[li];
+        const bool valid = active && !gp(I.blocked)[li];
         const int64_t row_v = step - I.rec_step0;
         if ((I.stat_mask & kStatV) && row_v >= 0 && row_v < (int64_t)I.rec_len)
             wave_pop_stats(I.rec_stat + (size_t)row_v * I.npop * kStatFields, valid, p, stv, 1u, lane);
@@ -463,12 +463,12 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         long long rb = 0;
         uint32_t len = 0;
         if (emit) {
-            rb = I.rowptr[li];
-            len = (uint32_t)(I.rowptr[li + 1] - rb);
+            rb = gp(I.rowptr)[li];
+            len = (uint32_t)(gp(I.rowptr)[li + 1] - rb);
         }
         const unsigned long long ev = wave_sum_u64((unsigned long long)len);
         if ((int)lane == leader) {
-            unsigned long long *c = I.wave_ctr + 2u * ((d.begin >> 6) + w);
+            auto c = gp(I.wave_ctr) + 2u * ((d.begin >> 6) + w);
             c[0] += ev;
             c[1] += (unsigned long long)cnt_w;
         }
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             // the wave (blocked neurons are not counted)
             const int64_t row = step - I.rec_step0;
             if (row >= 0 && row < (int64_t)I.rec_len) {
-                const bool counted = emit && !I.blocked[li];
+                const bool counted = emit && !gp(I.blocked)[li];
                 unsigned long long rest = __ballot(counted);
                 while (rest != 0ull) {
                     const int src = __ffsll((long long)rest) - 1;
@@ -495,7 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             rest &= rest - 1ull;
             const long long rb_s = __shfl(rb, src, kWave);
             const uint32_t len_s = __shfl(len, src, kWave);
-            bin_row(I.col, (int64_t)rb_s, len_s, I.tile_base, tiles, cnt_out, q_out, ctl, lane);
+            bin_row(I.col, I.w, (int64_t)rb_s, len_s, I.tile_base, tiles, cnt_out, q_out, ctl, seg_dst + w * kWave * 4u,
+                    lane);
         }
     }
 }
@@ -509,23 +510,19 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
                                                         uint32_t set)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ RunScratch rs;
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint32_t w = tid / kWave;
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(qs.count[(size_t)set * qs.ntiles + blockIdx.x], d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid,
-                    lane, w);
+    accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
     if (tid < d.count)
-        I.dlt[d.begin + tid] = acc_to_delta(acc, tid);
+        st4(I.dlt, d.begin + tid, acc_to_delta(acc, tid));
 }
 
 // Folds the pending delta of queue set `set` into J now (J = J * decay +
@@ -538,12 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
                                                    uint32_t set)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ RunScratch rs;
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
-    const uint32_t lane = __lane_id();
-    const uint32_t w = tid / kWave;
     uint32_t *cnt_p = qs.count + (size_t)set * qs.ntiles + blockIdx.x;
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
@@ -551,15 +545,14 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt = min(*cnt_p, d.qcap);
     __syncthreads();
-    accumulate_tile(acc, rs, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, d.begin, d.count, I.col, I.w, tid,
-                    lane, w);
+    accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
-        const uint32_t fl = I.flag[li];
+        const uint32_t fl = gp(I.flag)[li];
         if (!(fl & kFlagFolded)) {
-            I.J[li] = fold_delta(I.J[li], I.dec[param_index(I, li)], acc_to_delta(acc, tid));
-            I.flag[li] = (uint8_t)(fl | kFlagFolded);
+            st4(I.J, li, fold_delta(ld4(I.J, li), ld4(I.dec, param_index(I, li)), acc_to_delta(acc, tid)));
+            gp(I.flag)[li] = (uint8_t)(fl | kFlagFolded);
         }
     }
     if (tid == 0)
@@ -579,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_stat_tail(const TileDesc *__restrict
     const uint32_t cf = current_fields(I.stat_mask);
     if (I.rec_stat == nullptr || cf == 0u)
         return;
-    const int64_t row = I.step[par] - 1 - I.rec_step0;
+    const int64_t row = gp(I.step)[par] - 1 - I.rec_step0;
     if (row < 0 || row >= (int64_t)I.rec_len)
         return;
     const uint32_t tid = threadIdx.x;
@@ -593,18 +586,18 @@ __global__ __launch_bounds__(kBlock) void k_stat_tail(const TileDesc *__restrict
         while (p + 1 < I.npop && li >= I.pops[p].end)
             ++p;
         const uint32_t c = param_index(I, li);
-        const float V = I.V[li];
-        float4 J = I.J[li];
-        if (!(I.flag[li] & kFlagFolded))
-            J = fold_delta(J, I.dec[c], I.dlt[li]);
-        const float4 g = I.gU[c], vu = I.vU[c];
+        const float V = gp(I.V)[li];
+        float4 J = ld4(I.J, li);
+        if (!(gp(I.flag)[li] & kFlagFolded))
+            J = fold_delta(J, ld4(I.dec, c), ld4(I.dlt, li));
+        const float4 g = ld4(I.gU, c), vu = ld4(I.vU, c);
         const float4 rc = receptor_currents(g, vu, J, V);
         stv[1] = stat_q(syn_current(g, vu, J, V));
         stv[2] = stat_q(rc.x);
         stv[3] = stat_q(rc.y);
         stv[4] = stat_q(rc.z);
         stv[5] = stat_q(rc.w);
-        valid = !I.blocked[li];
+        valid = !gp(I.blocked)[li];
     }
     wave_pop_stats(I.rec_stat + (size_t)row * I.npop * kStatFields, valid, p, stv, cf, lane);
 }
@@ -625,23 +618,23 @@ __global__ __launch_bounds__(kBlock) void k_sample(const InstDev *__restrict__ i
     const SampleDev s = samples[i];
     const InstDev &I = insts[s.inst];
     const SampleRecDev R = recs[s.inst];
-    const int64_t row = I.step[par] - I.rec_step0;
+    const int64_t row = gp(I.step)[par] - I.rec_step0;
     if (row < 0 || row >= (int64_t)I.rec_len)
         return;
     const uint32_t li = s.neuron;
     const size_t o = (size_t)row * R.nsample + s.slot;
-    const float V = I.V[li];
+    const float V = gp(I.V)[li];
     if (R.spike)
-        R.spike[o] = (I.flag[li] & kFlagEmit) ? 1 : 0;
+        R.spike[o] = (gp(I.flag)[li] & kFlagEmit) ? 1 : 0;
     if (R.v)
         R.v[o] = V;
     if (R.isyn) {
         const uint32_t c = param_index(I, li);
-        const float4 J = fold_delta(I.J[li], I.dec[c], I.dlt[li]);   // flag bit 2 is clear here
-        R.isyn[o] = syn_current(I.gU[c], I.vU[c], J, V);
+        const float4 J = fold_delta(ld4(I.J, li), ld4(I.dec, c), ld4(I.dlt, li));   // flag bit 2 is clear here
+        R.isyn[o] = syn_current(ld4(I.gU, c), ld4(I.vU, c), J, V);
     }
     if (R.iou)
-        R.iou[o] = I.iou[li];
+        R.iou[o] = gp(I.iou)[li];
 }
 
 // Verification entry: normals of the OU stream for a range of neurons at one

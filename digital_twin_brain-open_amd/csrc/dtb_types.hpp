@@ -42,14 +42,16 @@ struct alignas(16) PopDev {
     uint32_t pad0, pad1;
 };
 
-// A run of consecutive synapses of one fired row whose targets lie in one
-// tile, queued for that tile: first synapse in bits 0..39, length in bits
-// 40..63.
-using QItem = uint64_t;
-constexpr uint32_t kQBeginBits = 40;
-constexpr uint64_t kQBeginMask = (1ull << kQBeginBits) - 1ull;
-constexpr int64_t kMaxSynapses = (int64_t)1 << kQBeginBits;
-constexpr int64_t kMaxRowLength = (int64_t)1 << (64 - kQBeginBits);
+// One delivered synapse queued for its target tile: the target's index in
+// the tile (bits 0..7), the channel pair (bit 8: 0 AMPA/NMDA, 1
+// GABA_A/GABA_B) and the weight pair.
+struct QSyn {
+    uint32_t key;
+    float w0, w1;
+};
+constexpr uint32_t kQSynKind = 1u << 8;
+constexpr int64_t kMaxSynapses = (int64_t)1 << 40;
+constexpr int64_t kMaxRowLength = (int64_t)1 << 31;
 
 // Per-population statistics (stat_result.cpp:266-391): which sums a run
 // records.  Each neuron's value enters as the exact integer rint(x * 2^24),
@@ -119,14 +121,14 @@ struct alignas(16) TileDesc {
     uint32_t count;     // neurons in the tile (<= 256)
     uint32_t pop0;      // population of the first neuron
     uint64_t qoff;      // first queue slot (within one parity's queue set)
-    uint32_t qcap;      // queue capacity: runs targeting this tile over all rows
+    uint32_t qcap;      // queue capacity: synapses targeting this tile (all rows)
     uint32_t pad;
 };
 
 // Queues of both parities: the launch of parity p drains set p and fills set
 // p ^ 1.
 struct QueueSet {
-    QItem *items;       // [2][qtotal]
+    QSyn *items;        // [2][qtotal]
     uint32_t *count;    // [2][ntiles]
     uint64_t qtotal;
     uint32_t ntiles;

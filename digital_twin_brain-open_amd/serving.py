@@ -24,9 +24,9 @@ from .engine import DEFAULT_G_UI, Block, Engine
 # Device bytes per neuron of an instance: parameter class, V, i_ou,
 # refractory counter, flag, J, pending delta, blocked flag, row offset and
 # per-wave counters; per synapse: column index, weight pair and the two
-# propagation queues (at most one 8-byte entry per synapse each).
+# propagation queues (one 12-byte entry per synapse each).
 BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 1 + 8 + 1
-BYTES_PER_SYNAPSE = 4 + 8 + 2 * 8
+BYTES_PER_SYNAPSE = 4 + 8 + 2 * 12
 
 
 class Request:
@@ -117,12 +117,15 @@ class Server:
         offset = 0.0 if realtime else min(r.arrival for r in requests)
         steps_run = 0
         resident_steps = 0
+        t_admit = t_run = t_retire = 0.0
         while pending or active:
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
             while (pending and pending[0].arrival - offset <= now and used + pending[0].device_bytes() <= self.budget
                    and (max_resident is None or len(active) < max_resident)):
                 r = pending.popleft()
+                ta = time.monotonic()
                 b = self._admit(r)
+                t_admit += time.monotonic() - ta
                 r.admitted = time.monotonic() - t0 + offset
                 active[b.inst] = [r, b, r.steps]
                 used += r.device_bytes()
@@ -132,8 +135,10 @@ class Server:
                     time.sleep(min(wait, 0.05))
                 continue
             s = min(self.chunk, min(v[2] for v in active.values()))
+            ta = time.monotonic()
             self.engine.run(s, graph_chunk=0)
             self.engine.sync()
+            t_run += time.monotonic() - ta
             steps_run += s
             resident_steps += s * len(active)
             done = []
@@ -143,17 +148,23 @@ class Server:
                     done.append(inst)
             for inst in done:
                 r, b, _ = active.pop(inst)
+                ta = time.monotonic()
                 self._retire(r, b)
+                t_retire += time.monotonic() - ta
                 r.finished = time.monotonic() - t0 + offset
                 used -= r.device_bytes()
         wall = time.monotonic() - t0
         lat = np.array([r.latency() for r in requests])
         events = sum(r.events for r in requests)
+        spikes = sum(r.spikes for r in requests)
+        nsteps = sum(r.n * r.steps for r in requests)
         return {"requests": len(requests), "wall_s": wall, "events": int(events),
-                "events_per_s": events / wall,
+                "events_per_s": events / wall, "neuron_steps_per_s": nsteps / wall,
+                "mean_rate_hz": spikes / (nsteps * self.dt * 1e-3),
                 "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
-                "mean_resident": resident_steps / max(steps_run, 1), "engine_steps": steps_run}
+                "mean_resident": resident_steps / max(steps_run, 1), "engine_steps": steps_run,
+                "admit_s": t_admit, "step_s": t_run, "retire_s": t_retire}
 
 
 def run_alone(request, device=0, dt=0.1):
