@@ -83,6 +83,9 @@ _SIGS = {
     "dtb_version": (c_int, []),
     "dtb_last_error": (c_char_p, []),
     "dtb_device_count": (c_int, [POINTER(c_int)]),
+    "dtb_device_memory_cached": (c_int, [c_int, POINTER(c_uint64)]),
+    "dtb_device_memory_trim": (c_int, []),
+    "dtb_device_memory_reserve": (c_int, [c_int, c_uint64]),
     "dtb_engine_create": (c_int, [c_int, POINTER(c_void_p)]),
     "dtb_engine_destroy": (c_int, [c_void_p]),
     "dtb_instance_add": (c_int, [c_void_p, POINTER(Network), c_float, c_uint64, POINTER(c_int)]),

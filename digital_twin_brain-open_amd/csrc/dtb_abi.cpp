@@ -79,6 +79,31 @@ int dtb_device_count(int *count)
     });
 }
 
+int dtb_device_memory_cached(int device, uint64_t *bytes)
+{
+    return guarded([&] {
+        need(bytes != nullptr, "bytes is NULL");
+        *bytes = (uint64_t)dtb::devmem::cached_bytes(device);
+    });
+}
+
+int dtb_device_memory_reserve(int device, uint64_t bytes)
+{
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            throw dtb::Error(DTB_NO_DEVICE, "no HIP device");
+        need(device >= 0 && device < count, "device out of range");
+        DTB_HIP(hipSetDevice(device));
+        dtb::devmem::reserve((size_t)bytes);
+    });
+}
+
+int dtb_device_memory_trim(void)
+{
+    return guarded([&] { dtb::devmem::trim(); });
+}
+
 int dtb_engine_create(int device, dtb_engine_t *out)
 {
     return guarded([&] {

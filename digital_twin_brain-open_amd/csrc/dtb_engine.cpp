@@ -213,6 +213,8 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
     tile_syn_.assign(ntile, 0u);
     for (uint32_t i = 0; i < n_; ++i)
         tile_syn_[i >> kTileBits] += in[i];
+    // expected distinct tiles among K uniform targets, as a fraction
+    row_coverage_ = 1.0 - std::pow(1.0 - 1.0 / ntile, (double)f.K);
 }
 
 Instance::~Instance() = default;
@@ -227,20 +229,34 @@ void Instance::count_tile_synapses(const NetworkHost &net)
     unsigned hw = std::thread::hardware_concurrency();
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw ? hw : 4u, 16u), nsyn_ / (1 << 20)));
     std::vector<std::vector<uint32_t>> part(nt, std::vector<uint32_t>(nfine, 0u));
+    std::vector<uint64_t> segs(nt, 0);
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t) {
         th.emplace_back([&, t]() {
             const int64_t a = (int64_t)n_ * t / nt, b = (int64_t)n_ * (t + 1) / nt;
             uint32_t *c = part[t].data();
-            for (int64_t j = net.rowptr[a]; j < net.rowptr[b]; ++j)
-                ++c[net.col[j] >> kTileBits];
+            uint64_t sg = 0;
+            for (int64_t r = a; r < b; ++r) {
+                uint32_t prev = 0xffffffffu;
+                for (int64_t j = net.rowptr[r]; j < net.rowptr[r + 1]; ++j) {
+                    const uint32_t tl = net.col[j] >> kTileBits;
+                    ++c[tl];
+                    sg += tl != prev;
+                    prev = tl;
+                }
+            }
+            segs[t] = sg;
         });
     }
     for (auto &x : th)
         x.join();
-    for (int t = 0; t < nt; ++t)
+    uint64_t sg = 0;
+    for (int t = 0; t < nt; ++t) {
+        sg += segs[t];
         for (uint32_t k = 0; k < nfine; ++k)
             tile_syn_[k] += part[t][k];
+    }
+    row_coverage_ = (double)sg / ((double)n_ * (double)nfine);
 }
 
 uint32_t Instance::tile_synapses(uint32_t tile) const
@@ -328,7 +344,7 @@ void Instance::derive_params(hipStream_t s)
     gU_.resize(nrec);
     vU_.resize(nrec);
     dec_.resize(nrec);
-    cls_.resize(dense ? 0 : n_);
+    cls_.resize(n_);
     float4 *pa = pA_.mutable_cpu_data();
     float4 *pb = pB_.mutable_cpu_data();
     float4 *gu = gU_.mutable_cpu_data();
@@ -343,8 +359,11 @@ void Instance::derive_params(hipStream_t s)
     };
     if (dense) {
         prev = nullptr;
-        for (uint32_t i = 0; i < n_; ++i)
+        uint16_t *cl = cls_.mutable_cpu_data();
+        for (uint32_t i = 0; i < n_; ++i) {
             put(i, derive(i));
+            cl[i] = (uint16_t)(i & (kBlock - 1));
+        }
     } else {
         for (size_t k = 0; k < recs.size(); ++k)
             put(k, recs[k]);
@@ -640,9 +659,10 @@ void Instance::read_state(float *V, int32_t *t, uint8_t *f, uint8_t *emitted, fl
         const float4 *dcs = dec_.cpu_data();
         const float4 *gs = gU_.cpu_data();
         const float4 *vus = vU_.cpu_data();
-        const uint16_t *cl = dense_params_ ? nullptr : cls_.cpu_data();
+        const uint16_t *cl = cls_.cpu_data();
+        const uint32_t pmask = dense_params_ ? ~(kBlock - 1) : 0u;
         for (uint32_t i = 0; i < n_; ++i) {
-            const size_t c = cl ? cl[i] : i;
+            const size_t c = (i & pmask) + cl[i];
             const float4 *dc = dcs + c, *g = gs + c, *vu = vus + c;
             float4 r = j[i];
             if (!(fl[i] & kFlagFolded)) {
@@ -710,7 +730,8 @@ InstDev Instance::dev(uint32_t tile_base)
     d.gU = gU_.gpu_data();
     d.vU = vU_.gpu_data();
     d.dec = dec_.gpu_data();
-    d.cls = dense_params_ ? nullptr : cls_.gpu_data();
+    d.cls = cls_.gpu_data();
+    d.pmask = dense_params_ ? ~(kBlock - 1) : 0u;
     d.blocked = blocked_.gpu_data();
     d.V = V_.mutable_gpu_data();
     d.tcnt = tcnt_.mutable_gpu_data();

@@ -94,6 +94,15 @@ int prop_type_column(int prop);   // PropType -> property column, -1 if unknown
 enum DataType : int { DOUBLE = 0, FLOAT = 1, FLOAT16 = 2, INT32 = 3, INT8 = 4 };
 size_t data_size(DataType t);
 
+// Device memory that stays with the library once freed (dtb_devmem.cpp).
+namespace devmem {
+void *alloc(size_t bytes);           // on the current device
+void release(void *p);               // block from alloc; may still be in use by queued work
+void reserve(size_t bytes);          // keeps at least this much for blocks of 1 MB and more
+void trim();                         // returns all unused memory to the runtime
+size_t cached_bytes(int device);     // bytes held in the cache
+}  // namespace devmem
+
 // Device buffer with a lazily created, zero-filled host mirror.
 template <typename T>
 class DataAllocator {
@@ -117,7 +126,7 @@ public:
     T *mutable_gpu_data()
     {
         if (gpu_ == nullptr && count_ > 0) {
-            DTB_HIP(hipMalloc(reinterpret_cast<void **>(&gpu_), size()));
+            gpu_ = static_cast<T *>(devmem::alloc(size()));
             // The zero fill runs on the null stream, which does not order
             // with the engine's non-blocking stream: wait for it here so a
             // later upload on the engine stream cannot be overwritten.
@@ -166,8 +175,7 @@ public:
 private:
     void release()
     {
-        if (gpu_ != nullptr)
-            (void)hipFree(gpu_);
+        devmem::release(gpu_);
         gpu_ = nullptr;
         std::vector<T>().swap(cpu_);
         count_ = 0;
@@ -272,6 +280,9 @@ public:
     // Synapses whose target lies in tile `tile` (2^kTileBits neurons): the
     // exact queue bound of the binning.
     uint32_t tile_synapses(uint32_t tile) const;
+    // Mean over rows of the fraction of this instance's tiles that the row's
+    // synapses reach (tile segments of the row / tiles).
+    double row_coverage() const { return row_coverage_; }
 
 private:
     void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
@@ -295,7 +306,7 @@ private:
     std::vector<uint32_t> samples_;
 
     DataAllocator<float4> pA_, pB_, gU_, vU_, dec_;    // parameter records per class
-    DataAllocator<uint16_t> cls_;                      // parameter class per neuron (empty if dense)
+    DataAllocator<uint16_t> cls_;                      // parameter class per neuron
     bool dense_params_ = false;                        // one class per neuron
     DataAllocator<float4> J_;
     DataAllocator<uint8_t> blocked_, flag_;
@@ -317,6 +328,7 @@ private:
     int64_t rec_len_ = 0;
     bool rec_freq_ = false, rec_samples_ = false;
     std::vector<uint32_t> tile_syn_;      // synapses per target tile
+    double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
 };
 
 // Steps a set of instances together.  One launch per step covers every
@@ -376,12 +388,13 @@ private:
     uint32_t par_ = 0;                     // parity of the next step launch (= pending queue set)
     uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;
     uint64_t qtotal_ = 0;                  // queue slots per parity
+    uint32_t cshift_ = kCountShiftDense;   // queue count layout (count_slot)
     DataAllocator<InstDev> d_insts_;
     DataAllocator<StepCtl> d_ctl_;
     DataAllocator<SampleDev> d_samples_;
     DataAllocator<SampleRecDev> d_recs_;
     DataAllocator<TileDesc> d_tiles_;
-    DataAllocator<uint32_t> d_qcnt_;       // [2][ntiles]
+    DataAllocator<uint32_t> d_qcnt_;       // count_words(2 * ntiles)
     DataAllocator<QSyn> d_queue_;          // [2][qtotal]
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
     int graph_steps_[2] = {0, 0};

@@ -92,6 +92,7 @@ QueueSet Engine::queues()
     q.count = d_qcnt_.mutable_gpu_data();
     q.qtotal = qtotal_;
     q.ntiles = ntiles_;
+    q.cshift = cshift_;
     return q;
 }
 
@@ -105,6 +106,7 @@ void Engine::rebuild()
     std::vector<SampleRecDev> recs;
     tile_base_.assign(insts_.size(), 0u);
     uint64_t total = 0, qtotal = 0;
+    double coverage = 0.0;
     for (size_t slot = 0; slot < insts_.size(); ++slot) {
         Instance *p = insts_[slot].get();
         if (!p)
@@ -134,6 +136,7 @@ void Engine::rebuild()
                 samples.push_back(SampleDev{idx, ids[k], k, 0});
         }
         total += p->n();
+        coverage += p->row_coverage() * p->n();
     }
     if (total > 0xffffffffull || tiles.size() > 0xffffffffull)
         throw Error(kInvalidParameter, "too many neurons in one engine");
@@ -145,11 +148,14 @@ void Engine::rebuild()
     d_tiles_.resize(std::max<size_t>(tiles.size(), 1));
     d_samples_.resize(std::max<size_t>(samples.size(), 1));
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
-    d_qcnt_.resize(2 * std::max<size_t>(tiles.size(), 1));
+    // Rows that reach more than a tenth of their instance's tiles make many
+    // waves reserve in the same counts at once: give each count its own line.
+    cshift_ = (total > 0 && coverage / (double)total > 0.1) ? kCountShiftSpread : kCountShiftDense;
+    d_qcnt_.resize(count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
     // The queues only grow: when instances come and go (serving) the buffer
     // is reused instead of being freed, reallocated and zero-filled.
-    if (d_queue_.count() < (size_t)(2 * qtotal_))
-        d_queue_.resize((size_t)(2 * qtotal_));
+    if (d_queue_.count() < (size_t)(2 * qtotal_ + kQueuePrefetch))
+        d_queue_.resize((size_t)(2 * qtotal_ + kQueuePrefetch));
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
     d_qcnt_.mutable_gpu_data();
@@ -256,9 +262,12 @@ void Engine::clear_queues(int id)
     if (dirty_)
         rebuild();
     const uint32_t ntile = (get(id)->n() + kBlock - 1) / kBlock;
-    for (uint32_t p = 0; p < 2; ++p)
-        DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + (size_t)p * ntiles_ + tile_base_[(size_t)id], 0,
-                               ntile * sizeof(uint32_t), stream_));
+    for (uint32_t p = 0; p < 2; ++p) {
+        // the counts of these tiles and the unused words between them
+        const size_t first = count_slot((size_t)p * ntiles_ + tile_base_[(size_t)id], cshift_);
+        const size_t last = count_slot((size_t)p * ntiles_ + tile_base_[(size_t)id] + ntile - 1, cshift_);
+        DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + first, 0, (last + 1 - first) * sizeof(uint32_t), stream_));
+    }
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 

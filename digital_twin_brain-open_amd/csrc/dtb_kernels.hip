@@ -75,7 +75,7 @@ __device__ __forceinline__ unsigned long long f32_to_q40(float w)
 // Row of the parameter records that holds neuron li's parameters.
 __device__ __forceinline__ uint32_t param_index(const InstDev &I, uint32_t li)
 {
-    return I.cls != nullptr ? (uint32_t)gp(I.cls)[li] : li;
+    return (li & I.pmask) + (uint32_t)gp(I.cls)[li];
 }
 
 // synaptic_current.cpp:39-43 in source order, no contraction.
@@ -205,7 +205,7 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
 // by segment start within the pass).
 __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
                                         int64_t row_begin, uint32_t len, uint32_t tile_base,
-                                        const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt,
+                                        const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, StepCtl *__restrict__ ctl,
                                         unsigned long long *seg_dst, uint32_t lane)
 {
@@ -269,7 +269,7 @@ __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, co
             if (j < len && tl[k] != nx) {
                 const uint32_t tg = tile_base + tl[k];
                 const uint32_t n = j + 1u - open;
-                const uint32_t base = atomicAdd(&qcnt[tg], n);
+                const uint32_t base = atomicAdd(&qcnt[count_slot(qset + tg, cshift)], n);
                 const TileDesc td = tiles[tg];
                 unsigned long long dst = td.qoff + base;
                 if (base + n > td.qcap) {
@@ -302,8 +302,6 @@ __device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, co
     }
 }
 
-// Queue entries per tile read ahead together with the count.
-constexpr uint32_t kPrefetch = 128;
 
 // The step of launch parity `par`: drains queue set par, fills set par ^ 1,
 // reads the step index from step[par] and advances it into step[par ^ 1].
@@ -322,50 +320,46 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
     const QSyn *__restrict__ q_in = qs.items + (size_t)par * qs.qtotal;
-    uint32_t *__restrict__ cnt_in = qs.count + (size_t)par * qs.ntiles;
+    const size_t cslot_in = count_slot((size_t)par * qs.ntiles + blockIdx.x, qs.cshift);
     QSyn *__restrict__ q_out = qs.items + (size_t)(par ^ 1u) * qs.qtotal;
-    uint32_t *__restrict__ cnt_out = qs.count + (size_t)(par ^ 1u) * qs.ntiles;
 
+    // Loads are issued in order of their dependence chains: the queue count
+    // (kernel arguments only), the parameter class, the neuron state, the
+    // parameter records (need the class), the queue head.
+    const uint32_t cnt_raw = qs.count[cslot_in];
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
+    const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
+    const uint32_t cls = gp(I.cls)[lc];
+    float4 J = ld4(I.J, lc);
+    float V = gp(I.V)[lc];
+    int32_t t = gp(I.tcnt)[lc];
+    uint32_t fl = gp(I.flag)[lc];
+    float iou = gp(I.iou)[lc];
+    const uint32_t c = (lc & I.pmask) + cls;
+    const float4 pa = ld4(I.pA, c);
+    const float4 pb = ld4(I.pB, c);
+    const float4 g = ld4(I.gU, c);
+    const float4 vu = ld4(I.vU, c);
+    const float4 dc = ld4(I.dec, c);
+    // The queue is complete when the launch starts; entries past the count
+    // are ignored (the queue buffer has kQueuePrefetch slots of slack at its
+    // end, so the read needs no branch).
+    const QSyn head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
     const int64_t step = gp(I.step)[par];
-    if (d.begin == 0 && tid == 0)
-        gp(I.step)[par ^ 1u] = step + 1;   // no tile of this launch reads this slot
-
-    // The queue head is read together with the count (the queue is complete
-    // when the launch starts; entries past the count are ignored).
-    QSyn head{};
-    if (tid < kPrefetch && tid < d.qcap)
-        head = q_in[d.qoff + tid];
-    // Per-neuron loads first: they are independent of the accumulation.
-    float4 pa, pb, g, vu, dc, J;
-    float V = 0.0f, iou = 0.0f;
-    int32_t t = 0;
-    uint32_t fl = 0;
-    if (active) {
-        const uint32_t c = param_index(I, li);
-        pa = ld4(I.pA, c);
-        pb = ld4(I.pB, c);
-        g = ld4(I.gU, c);
-        vu = ld4(I.vU, c);
-        dc = ld4(I.dec, c);
-        J = ld4(I.J, li);
-        V = gp(I.V)[li];
-        t = gp(I.tcnt)[li];
-        fl = gp(I.flag)[li];
-        iou = gp(I.iou)[li];
-    }
 
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(cnt_in[blockIdx.x], d.qcap);
+    const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
-    accumulate_tile(acc, q_in + d.qoff, cnt, head, kPrefetch, tid);
+    accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
     __syncthreads();
     if (tid == 0)
-        cnt_in[blockIdx.x] = 0u;   // this launch consumed the queue
+        qs.count[cslot_in] = 0u;   // this launch consumed the queue
+    if (d.begin == 0 && tid == 0)
+        gp(I.step)[par ^ 1u] = step + 1;   // no tile of this launch reads this slot
 
     bool emit = false;
     uint32_t p = d.pop0;
@@ -495,8 +489,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             rest &= rest - 1ull;
             const long long rb_s = __shfl(rb, src, kWave);
             const uint32_t len_s = __shfl(len, src, kWave);
-            bin_row(I.col, I.w, (int64_t)rb_s, len_s, I.tile_base, tiles, cnt_out, q_out, ctl, seg_dst + w * kWave * 4u,
-                    lane);
+            bin_row(I.col, I.w, (int64_t)rb_s, len_s, I.tile_base, tiles, qs.count, (size_t)(par ^ 1u) * qs.ntiles,
+                    qs.cshift, q_out, ctl, seg_dst + w * kWave * 4u, lane);
         }
     }
 }
@@ -517,7 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(qs.count[(size_t)set * qs.ntiles + blockIdx.x], d.qcap);
+    const uint32_t cnt = min(qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)], d.qcap);
     __syncthreads();
     accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
@@ -538,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
-    uint32_t *cnt_p = qs.count + (size_t)set * qs.ntiles + blockIdx.x;
+    uint32_t *cnt_p = qs.count + count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift);
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;

@@ -73,10 +73,12 @@ struct alignas(16) InstDev {
     //   gU, vU, dec = (AMPA, NMDA, GABA_A, GABA_B)
     // Neurons with bit-identical records share a class (in practice one per
     // population), so the step reads 2 bytes of class per neuron instead of
-    // 80 bytes of parameters.  cls == nullptr: one class per neuron (class =
-    // neuron index), used when the records have more than 65536 values.
+    // 80 bytes of parameters.  The record of neuron i is
+    // (i & pmask) + cls[i]: pmask = 0 with shared classes; when the records
+    // have more than 65536 values every neuron has its own (pmask = ~0xff,
+    // cls[i] = i & 0xff, record = i).
     const float4 *pA, *pB, *gU, *vU, *dec;
-    const uint16_t *cls;        // [n] parameter class, or nullptr
+    const uint16_t *cls;        // [n] parameter class
     const uint8_t *blocked;     // excluded from population statistics
     // state
     float *V;
@@ -111,10 +113,12 @@ struct alignas(16) InstDev {
     float dt_sub;
     uint32_t tile_base;         // engine-wide index of this instance's first tile
     uint32_t stat_mask;         // kStatV | kStatI | kStatReceptor: statistics recorded in rec_stat
+    uint32_t pmask;             // parameter record of neuron i: (i & pmask) + cls[i]
+    uint32_t pad;
 };
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
-// of one instance, and the queue of synapse runs that target them.
+// of one instance, and the queue of the synapses delivered to them.
 struct alignas(16) TileDesc {
     uint32_t inst;
     uint32_t begin;     // first instance-local neuron
@@ -127,12 +131,36 @@ struct alignas(16) TileDesc {
 
 // Queues of both parities: the launch of parity p drains set p and fills set
 // p ^ 1.
+// Queue entries per tile that the step kernel reads ahead together with the
+// count; the queue buffer has this many slots of slack after both sets.
+constexpr uint32_t kQueuePrefetch = 128;
+
+// Word of queue count i (i = set * ntiles + tile): groups of 2^shift
+// consecutive counts share one 128-byte line (32 words).  shift = 5 packs the
+// counts densely: the reservations of one wave, which mostly hit neighbouring
+// tiles, touch few lines.  shift = 0 gives every count a line of its own:
+// when every fired row reaches most tiles, many waves reserve in the same
+// counts at once, and counts sharing a line would serialise on it.
+constexpr uint32_t kCountLineWords = 32;
+constexpr uint32_t kCountShiftDense = 5;
+constexpr uint32_t kCountShiftSpread = 0;
+
+__host__ __device__ inline size_t count_slot(size_t i, uint32_t shift)
+{
+    return ((i >> shift) * kCountLineWords) + (i & ((1u << shift) - 1u));
+}
+
+__host__ __device__ inline size_t count_words(size_t n, uint32_t shift)
+{
+    return ((n + (1u << shift) - 1u) >> shift) * kCountLineWords;
+}
+
 struct QueueSet {
     QSyn *items;        // [2][qtotal]
-    uint32_t *count;    // [2][ntiles]
+    uint32_t *count;    // [2][ntiles], tile t of set p at word count_slot(p * ntiles + t, cshift)
     uint64_t qtotal;
     uint32_t ntiles;
-    uint32_t pad;
+    uint32_t cshift;    // kCountShiftDense or kCountShiftSpread
 };
 
 // Sample request (reference stat_samples_kernel, stat_result.cpp:393-433).

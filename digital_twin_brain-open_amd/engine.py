@@ -200,6 +200,24 @@ def device_count():
     return c.value
 
 
+def device_memory_cached(device=0):
+    """Bytes of freed device memory the library keeps for reuse on `device`."""
+    b = c_uint64(0)
+    _lib.check(_lib.load().dtb_device_memory_cached(device, byref(b)))
+    return b.value
+
+
+def device_memory_reserve(nbytes, device=0):
+    """Keeps at least nbytes of device memory for the library's large buffers
+    (one allocation now; instances added later are carved from it)."""
+    _lib.check(_lib.load().dtb_device_memory_reserve(device, int(nbytes)))
+
+
+def device_memory_trim():
+    """Returns every cached device block to the HIP runtime."""
+    _lib.check(_lib.load().dtb_device_memory_trim())
+
+
 def ou_normals(seed, neuron0, substep, count, device=0):
     out = np.empty(count, dtype=np.float32)
     _lib.check(_lib.load().dtb_ou_normals(device, seed, neuron0, substep, count, _ptr(out)))

@@ -19,7 +19,7 @@ import time
 
 import numpy as np
 
-from .engine import DEFAULT_G_UI, Block, Engine
+from .engine import DEFAULT_G_UI, Block, Engine, device_memory_reserve
 
 # Device bytes per neuron of an instance: parameter class, V, i_ou,
 # refractory counter, flag, J, pending delta, blocked flag, row offset and
@@ -75,11 +75,15 @@ def device_free_bytes(device=0):
 class Server:
     """Continuous batching of simulation requests on one engine."""
 
-    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1):
+    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1, reserve=True):
         self.engine = Engine(device)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes(device))
         self.chunk = int(chunk)
         self.dt = dt
+        if reserve:
+            # one device allocation for the whole budget: admitting and
+            # retiring requests then carves and returns ranges of it
+            device_memory_reserve(self.budget, device)
 
     def close(self):
         self.engine.close()
@@ -169,7 +173,7 @@ class Server:
 
 def run_alone(request, device=0, dt=0.1):
     """The request on an engine of its own: (events, spikes, V hash)."""
-    srv = Server(device=device, hbm_budget=1 << 62, chunk=max(1, request.steps), dt=dt)
+    srv = Server(device=device, hbm_budget=1 << 62, chunk=max(1, request.steps), dt=dt, reserve=False)
     try:
         r = Request(request.rid, 0.0, request.n, request.k, request.steps, request.net_seed, request.seed,
                     request.mu, request.sigma, request.tau, request.g_ui)

@@ -107,6 +107,16 @@ int dtb_version(void);
 const char *dtb_last_error(void);
 /* Number of visible HIP devices (0 without a GPU). */
 int dtb_device_count(int *count);
+/* No counterpart in the reference (its blocks live for the whole run).
+ * Device memory of removed instances and resized buffers stays with the
+ * library and is reused (blocks of 1 MB and more are carved best-fit from
+ * larger allocations); a server reserves its budget once with
+ * dtb_device_memory_reserve so that admitting and retiring instances makes
+ * no HIP allocation.  dtb_device_memory_cached reports the unused bytes held
+ * on `device`; dtb_device_memory_trim returns them to the HIP runtime. */
+int dtb_device_memory_reserve(int device, uint64_t bytes);
+int dtb_device_memory_cached(int device, uint64_t *bytes);
+int dtb_device_memory_trim(void);
 
 /* ---- engine: one GPU, one stream, any number of instances ---------------- */
 
