@@ -38,27 +38,39 @@ IN_DEGREE = 500
 OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
 NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of one k_step launch on this workload,
-# scaled by a device-copy calibration (the file gives commands and factors).
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md")
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of one k_step launch per workload (at
+# its default size), scaled by a device-copy calibration (each file gives
+# its commands and factors).
+TRAFFIC_PROFILE = {"block": os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md"),
+                   "stress": os.path.join(ROOT, "profiles", "k_step_uniform500k_in5000.md")}
 
-# Algorithmic HBM bytes of one step (k_step):
-# per neuron: parameter class (2 B) read; J float4 (16 B) read + write; V,
-# refractory counter, i_ou 3 x 4 B read + write; flag byte read + write
-# (= 60 B; the 400 parameter records, 80 B each, stay in cache); per fired
-# row: its two row offsets (16 B); per delivered synapse: column index +
-# weight pair (12 B).
-STEP_NEURON_BYTES = 2 + 2 * 16 + 2 * 12 + 2 * 1
-STEP_ROW_BYTES = 16
-STEP_SYN_BYTES = 12
+# Algorithmic HBM bytes of one step, as the step is defined on per-neuron
+# arrays: per neuron 165 B (parameters 76 B: 7 scalars and g, V_u, decay of
+# 4 channels; state 56 B read + write: V, refractory counter, i_ou, J x 4;
+# delta accumulators 32 B read + reset; fired flag 1 B); per fired row 16 B
+# (row offsets, fired-list entry); per delivered synapse 12 B (column index,
+# weight pair).
+ALGO_NEURON_BYTES, ALGO_ROW_BYTES, ALGO_SYN_BYTES = 165, 16, 12
+
+# Bytes this layout has to move per step: per neuron 60 B (parameter class
+# 2 B read; J 16 B, V, refractory counter, i_ou 3 x 4 B read + write; flag
+# 1 B read + write; the shared parameter records and the delta accumulators
+# stay on chip); per fired row 16 B (row offsets); per delivered synapse
+# 36 B (column index + weight pair read, its 12-byte queue entry written and
+# read).
+LAYOUT_NEURON_BYTES = 2 + 2 * 16 + 2 * 12 + 2 * 1
+LAYOUT_ROW_BYTES = 16
+LAYOUT_SYN_BYTES = 12 + 2 * 12
 
 
-def profiled_traffic(neurons):
+def profiled_traffic(workload, neurons):
     """HBM bytes per k_step launch measured with rocprofv3 counters on the
-    default workload, or None for another size."""
-    if neurons != N_NEURONS or not os.path.exists(TRAFFIC_PROFILE):
+    workload at its default size, or None."""
+    path = TRAFFIC_PROFILE.get(workload)
+    default = N_NEURONS if workload == "block" else STRESS_NEURONS
+    if path is None or neurons != default or not os.path.exists(path):
         return None
-    with open(TRAFFIC_PROFILE) as f:
+    with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
@@ -309,8 +321,10 @@ def main():
 
     ev_step, sp_step = events / args.steps, spikes / args.steps
     us = ms_step * 1e3 / args.steps
-    algo_bytes = STEP_NEURON_BYTES * net.n + STEP_ROW_BYTES * sp_step + STEP_SYN_BYTES * ev_step
+    algo_bytes = ALGO_NEURON_BYTES * net.n + ALGO_ROW_BYTES * sp_step + ALGO_SYN_BYTES * ev_step
+    layout_bytes = LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step + LAYOUT_SYN_BYTES * ev_step
     achieved = algo_bytes / (us * 1e-6) / 1e9
+    traffic = profiled_traffic(args.workload, args.neurons)
 
     res = {
         "metric": "synaptic events/s (%s)" % ("2M-neuron block" if args.workload == "block" else
@@ -333,8 +347,12 @@ def main():
         "kernel_us_per_step": {"k_step": us},
         "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
         "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args.neurons) if args.workload == "block" else None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algo_bytes_per_launch": algo_bytes,
+                     "layout_bytes_per_launch": layout_bytes,
+                     "layout_achieved": layout_bytes / (us * 1e-6) / 1e9,
+                     "layout_frac": layout_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                     "traffic_achieved": None if traffic is None else traffic / (us * 1e-6) / 1e9},
     }
     if args.cpu_steps > 0:
         t0 = time.perf_counter()
