@@ -191,30 +191,31 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
-// Queues the synapses of one fired row for the next launch: each synapse's
-// target index within its tile, channel pair and weight pair go into the
-// target tile's queue.  The row's synapses are ordered by target, so the
-// synapses of one tile form segments; each segment reserves its slots with
-// one atomic add on the tile's count and its synapses are stored
-// contiguously.  Called by a whole wave with wave-uniform arguments.  Lane l
-// holds synapses c0 + 4l .. c0 + 4l + 3 of the row (256 per pass; segments
-// end at pass boundaries).  A synapse's segment start is the last start at
+// Queues synapses c0 .. c0 + 255 of one fired row (one pass) for the next
+// launch: each synapse's target index within its tile, channel pair and
+// weight pair go into the target tile's queue.  The row's synapses are
+// ordered by target, so the synapses of one tile form segments; each segment
+// reserves its slots with one atomic add on the tile's count and its
+// synapses are stored contiguously.  Called by a whole wave with
+// wave-uniform arguments.  Lane l holds synapses c0 + 4l .. c0 + 4l + 3 of
+// the row (segments end at pass boundaries).  A synapse's segment start is the last start at
 // or before it (within the lane, else from a wave max-scan of the lanes' last
 // starts); the lane holding the segment's last synapse reserves and
 // publishes the destination in seg_dst (this wave's 256 LDS entries, indexed
 // by segment start within the pass).
-__device__ __forceinline__ void bin_row(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
-                                        int64_t row_begin, uint32_t len, uint32_t tile_base,
+__device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+                                        int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, StepCtl *__restrict__ ctl,
                                         unsigned long long *seg_dst, uint32_t lane)
 {
     constexpr uint32_t K = 4;
+    static_assert(kWave * K == kBlock, "one pass is 256 synapses");
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     constexpr unsigned long long kDrop = ~0ull;
     const auto col = gp(col_all) + row_begin;
     const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
-    for (uint32_t c0 = 0; c0 < len; c0 += kWave * K) {
+    {
         const uint32_t j0 = c0 + lane * K;
         uint32_t cv[K], tl[K];
         float2 wv[K];
@@ -313,6 +314,9 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
 {
     __shared__ unsigned long long acc[kBlock * 4];
     __shared__ unsigned long long seg_dst[4 * kBlock];   // per wave: 64 lanes x 4 synapses
+    __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
+    __shared__ uint32_t row_len[kBlock];                  // and length
+    __shared__ uint32_t nrows;
 
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
@@ -352,6 +356,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
+    if (tid == 0)
+        nrows = 0u;
     const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
@@ -448,8 +454,10 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             wave_pop_stats(I.rec_stat + (size_t)(row_v - 1) * I.npop * kStatFields, valid, p, stv, cf, lane);
     }
 
-    // Spike propagation (presynaptic_voltage.cpp:122-225): every wave queues
-    // the rows of its own fired neurons for the next launch.
+    // Spike propagation (presynaptic_voltage.cpp:122-225): the tile's fired
+    // rows are listed in LDS, and their passes of 256 synapses are dealt
+    // round-robin to the four waves, so one long row does not keep a single
+    // wave busy while the others idle.
     const unsigned long long m = __ballot(emit);
     if (m != 0ull) {
         const uint32_t cnt_w = (uint32_t)__popcll(m);
@@ -461,10 +469,18 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
             len = (uint32_t)(gp(I.rowptr)[li + 1] - rb);
         }
         const unsigned long long ev = wave_sum_u64((unsigned long long)len);
+        uint32_t base = 0;
         if ((int)lane == leader) {
             auto c = gp(I.wave_ctr) + 2u * ((d.begin >> 6) + w);
             c[0] += ev;
             c[1] += (unsigned long long)cnt_w;
+            base = atomicAdd(&nrows, cnt_w);
+        }
+        base = __shfl(base, leader, kWave);
+        if (emit) {
+            const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            row_rb[k] = rb;
+            row_len[k] = len;
         }
         if (I.rec_counts != nullptr) {
             // stat_result.cpp:161-206: one atomic per population present in
@@ -483,15 +499,17 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
                 }
             }
         }
-        unsigned long long rest = m;
-        while (rest != 0ull) {
-            const int src = __ffsll((long long)rest) - 1;
-            rest &= rest - 1ull;
-            const long long rb_s = __shfl(rb, src, kWave);
-            const uint32_t len_s = __shfl(len, src, kWave);
-            bin_row(I.col, I.w, (int64_t)rb_s, len_s, I.tile_base, tiles, qs.count, (size_t)(par ^ 1u) * qs.ntiles,
-                    qs.cshift, q_out, ctl, seg_dst + w * kWave * 4u, lane);
-        }
+    }
+    __syncthreads();
+    const uint32_t nr = nrows;
+    uint32_t pass = 0;
+    for (uint32_t r = 0; r < nr; ++r) {
+        const long long rb_s = row_rb[r];
+        const uint32_t len_s = row_len[r];
+        for (uint32_t c0 = 0; c0 < len_s; c0 += kBlock, ++pass)
+            if ((pass & 3u) == w)
+                bin_pass(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, ctl, seg_dst + w * kWave * 4u, lane);
     }
 }
 
