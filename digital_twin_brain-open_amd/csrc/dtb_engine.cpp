@@ -369,6 +369,17 @@ void Instance::derive_params(hipStream_t s)
             put(k, recs[k]);
         std::memcpy(cls_.mutable_cpu_data(), cls.data(), sizeof(uint16_t) * n_);
     }
+    const uint32_t ntile = (n_ + kBlock - 1) / kBlock;
+    tile_cls_.assign(ntile, 0u);
+    if (!dense) {
+        for (uint32_t t = 0; t < ntile; ++t) {
+            const uint32_t b = t * kBlock, e = std::min<uint32_t>(n_, b + kBlock);
+            bool same = true;
+            for (uint32_t i = b + 1; i < e && same; ++i)
+                same = cls[i] == cls[b];
+            tile_cls_[t] = same ? (uint32_t)cls[b] + 1u : 0u;
+        }
+    }
     uint8_t *bl = blocked_.mutable_cpu_data();
     for (uint32_t i = 0; i < n_; ++i)
         bl[i] = props_[(size_t)i * kNumCols + kColBlocked] != 0.0f;

@@ -280,6 +280,9 @@ public:
     // Synapses whose target lies in tile `tile` (2^kTileBits neurons): the
     // exact queue bound of the binning.
     uint32_t tile_synapses(uint32_t tile) const;
+    // 1 + the parameter class of every neuron of tile `tile` if they share
+    // one, else 0.
+    uint32_t tile_class(uint32_t tile) const { return tile_cls_.at(tile); }
     // Mean over rows of the fraction of this instance's tiles that the row's
     // synapses reach (tile segments of the row / tiles).
     double row_coverage() const { return row_coverage_; }
@@ -329,6 +332,7 @@ private:
     bool rec_freq_ = false, rec_samples_ = false;
     std::vector<uint32_t> tile_syn_;      // synapses per target tile
     double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
+    std::vector<uint32_t> tile_cls_;      // per tile: 1 + shared parameter class, or 0
 };
 
 // Steps a set of instances together.  One launch per step covers every

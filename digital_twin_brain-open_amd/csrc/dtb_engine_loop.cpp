@@ -126,6 +126,7 @@ void Engine::rebuild()
             d.count = std::min<uint32_t>(kBlock, p->n() - b);
             d.pop0 = q;
             d.qcap = p->tile_synapses(t);
+            d.pcls = p->tile_class(t);
             d.qoff = qtotal;
             qtotal += d.qcap;
             tiles.push_back(d);

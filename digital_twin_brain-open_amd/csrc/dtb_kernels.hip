@@ -18,10 +18,12 @@
 //      atomics, so the sum does not depend on the order);
 //   2. folds the delta into J (J = J * decay + delta), computes I_syn, runs
 //      OU + LIF for every substep;
-//   3. every wave reads the rows of its fired neurons (synapses ordered by
-//      target), reserves one contiguous segment per target tile with an
-//      atomic add on that tile's count and writes each synapse's entry into
-//      the target tile's queue for the next launch (12 bytes per synapse).
+//   3. lists the tile's fired rows in LDS and deals their passes of 128
+//      synapses round-robin to its four waves; a wave reads a pass (synapses
+//      ordered by target), reserves one contiguous segment per target tile
+//      with an atomic add on that tile's count and writes each synapse's
+//      entry into the target tile's queue for the next launch (12 bytes per
+//      synapse).
 // The queues are double-buffered: the launch of parity p drains set p and
 // fills set p ^ 1, so step k consumes exactly the synapses queued from the
 // spikes of step k - 1.  No global atomics touch the delta.
@@ -191,26 +193,29 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
-// Queues synapses c0 .. c0 + 255 of one fired row (one pass) for the next
+// Synapses per lane and per pass of bin_pass (one pass = one wave).
+constexpr uint32_t kPassK = 2;
+constexpr uint32_t kPass = kWave * kPassK;
+
+// Queues synapses c0 .. c0 + kPass - 1 of one fired row (one pass) for the next
 // launch: each synapse's target index within its tile, channel pair and
 // weight pair go into the target tile's queue.  The row's synapses are
 // ordered by target, so the synapses of one tile form segments; each segment
 // reserves its slots with one atomic add on the tile's count and its
 // synapses are stored contiguously.  Called by a whole wave with
-// wave-uniform arguments.  Lane l holds synapses c0 + 4l .. c0 + 4l + 3 of
+// wave-uniform arguments.  Lane l holds synapses c0 + 2l and c0 + 2l + 1 of
 // the row (segments end at pass boundaries).  A synapse's segment start is the last start at
 // or before it (within the lane, else from a wave max-scan of the lanes' last
 // starts); the lane holding the segment's last synapse reserves and
-// publishes the destination in seg_dst (this wave's 256 LDS entries, indexed
-// by segment start within the pass).
+// publishes the destination in seg_dst (this wave's kPass LDS entries,
+// indexed by segment start within the pass).
 __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, StepCtl *__restrict__ ctl,
                                         unsigned long long *seg_dst, uint32_t lane)
 {
-    constexpr uint32_t K = 4;
-    static_assert(kWave * K == kBlock, "one pass is 256 synapses");
+    constexpr uint32_t K = kPassK;
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     constexpr unsigned long long kDrop = ~0ull;
     const auto col = gp(col_all) + row_begin;
@@ -313,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
                                                  StepCtl *__restrict__ ctl)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ unsigned long long seg_dst[4 * kBlock];   // per wave: 64 lanes x 4 synapses
+    __shared__ unsigned long long seg_dst[4 * kPass];    // per wave: one pass
     __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
     __shared__ uint32_t row_len[kBlock];                  // and length
     __shared__ uint32_t nrows;
@@ -334,13 +339,16 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
     const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
-    const uint32_t cls = gp(I.cls)[lc];
+    // A tile whose neurons share one parameter class (the usual case: a
+    // population spans many tiles) reads its records without the class load.
+    uint32_t c = d.pcls - 1u;
+    if (d.pcls == 0u)
+        c = (lc & I.pmask) + (uint32_t)gp(I.cls)[lc];
     float4 J = ld4(I.J, lc);
     float V = gp(I.V)[lc];
     int32_t t = gp(I.tcnt)[lc];
     uint32_t fl = gp(I.flag)[lc];
     float iou = gp(I.iou)[lc];
-    const uint32_t c = (lc & I.pmask) + cls;
     const float4 pa = ld4(I.pA, c);
     const float4 pb = ld4(I.pB, c);
     const float4 g = ld4(I.gU, c);
@@ -455,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     }
 
     // Spike propagation (presynaptic_voltage.cpp:122-225): the tile's fired
-    // rows are listed in LDS, and their passes of 256 synapses are dealt
+    // rows are listed in LDS, and their passes of kPass synapses are dealt
     // round-robin to the four waves, so one long row does not keep a single
     // wave busy while the others idle.
     const unsigned long long m = __ballot(emit);
@@ -506,10 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     for (uint32_t r = 0; r < nr; ++r) {
         const long long rb_s = row_rb[r];
         const uint32_t len_s = row_len[r];
-        for (uint32_t c0 = 0; c0 < len_s; c0 += kBlock, ++pass)
+        for (uint32_t c0 = 0; c0 < len_s; c0 += kPass, ++pass)
             if ((pass & 3u) == w)
                 bin_pass(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
-                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, ctl, seg_dst + w * kWave * 4u, lane);
+                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, ctl, seg_dst + w * kPass, lane);
     }
 }
 

@@ -126,7 +126,7 @@ struct alignas(16) TileDesc {
     uint32_t pop0;      // population of the first neuron
     uint64_t qoff;      // first queue slot (within one parity's queue set)
     uint32_t qcap;      // queue capacity: synapses targeting this tile (all rows)
-    uint32_t pad;
+    uint32_t pcls;      // 1 + the parameter class shared by every neuron of the tile, or 0
 };
 
 // Queues of both parities: the launch of parity p drains set p and fills set
