@@ -100,6 +100,7 @@ _SIGS = {
     "dtb_engine_run": (c_int, [c_void_p, c_int64, c_int32]),
     "dtb_configure": (c_int, [c_void_p, c_int, c_int32, c_int32]),
     "dtb_engine_sync": (c_int, [c_void_p]),
+    "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),
     "dtb_engine_time_kernels": (c_int, [c_void_p, c_int64, POINTER(c_float)]),
     "dtb_engine_check": (c_int, [c_void_p]),
     "dtb_read_state": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -319,6 +319,14 @@ int dtb_engine_sync(dtb_engine_t h)
     return guarded([&] { engine(h).sync(); });
 }
 
+int dtb_engine_run_span(dtb_engine_t h, int64_t steps, int32_t graph_chunk, float *ms)
+{
+    return guarded([&] {
+        need(steps >= 0, "negative step count");
+        engine(h).run_span(steps, graph_chunk, ms);
+    });
+}
+
 int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step)
 {
     return guarded([&] {

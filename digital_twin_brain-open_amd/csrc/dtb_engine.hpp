@@ -374,6 +374,9 @@ public:
     // Kernel-time measurement: events recorded around every step kernel
     // (total in ms over `steps` steps).
     void run_timed(int64_t steps, float *ms_step);
+    // run(steps, graph_chunk) between two events on the engine stream; *ms =
+    // the GPU time between them (the step kernels and the gaps between).
+    void run_span(int64_t steps, int graph_chunk, float *ms);
 
 private:
     void rebuild();

@@ -323,6 +323,26 @@ void Engine::run_timed(int64_t steps, float *ms_step)
         *ms_step = (float)ts;
 }
 
+void Engine::run_span(int64_t steps, int graph_chunk, float *ms)
+{
+    DTB_HIP(hipSetDevice(device_));
+    if (dirty_)
+        rebuild();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DTB_HIP(hipEventCreate(&e0));
+    DTB_HIP(hipEventCreate(&e1));
+    DTB_HIP(hipEventRecord(e0, stream_));
+    run(steps, graph_chunk);
+    DTB_HIP(hipEventRecord(e1, stream_));
+    DTB_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    DTB_HIP(hipEventElapsedTime(&t, e0, e1));
+    DTB_HIP(hipEventDestroy(e0));
+    DTB_HIP(hipEventDestroy(e1));
+    if (ms)
+        *ms = t;
+}
+
 void Engine::sync()
 {
     DTB_HIP(hipSetDevice(device_));

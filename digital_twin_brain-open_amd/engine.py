@@ -285,6 +285,13 @@ class Engine:
     def check(self):
         _lib.check(self._lib.dtb_engine_check(self._h))
 
+    def run_span(self, steps, graph_chunk=0):
+        """Runs `steps` steps like run(); returns the GPU milliseconds between
+        events recorded on the engine stream before and after them."""
+        ms = c_float(0.0)
+        _lib.check(self._lib.dtb_engine_run_span(self._h, steps, graph_chunk, byref(ms)))
+        return ms.value
+
     def time_kernels(self, steps):
         """Runs `steps` steps; returns the total ms of the step kernel
         measured with HIP events on the engine stream."""

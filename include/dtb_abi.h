@@ -173,6 +173,10 @@ int dtb_engine_run(dtb_engine_t h, int64_t steps, int32_t graph_chunk);
 int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou);
 /* Waits for all work of the engine. */
 int dtb_engine_sync(dtb_engine_t h);
+/* No counterpart in the reference: dtb_engine_run between two HIP events on
+ * the engine stream; *ms = the GPU time between them (step kernels and the
+ * gaps between them), the engine synchronised on return. */
+int dtb_engine_run_span(dtb_engine_t h, int64_t steps, int32_t graph_chunk, float *ms);
 /* Times `steps` steps with HIP events around each step kernel (delta
  * accumulation + neuron update + spike binning); total in ms. */
 int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step);
