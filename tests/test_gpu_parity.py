@@ -410,3 +410,65 @@ def test_device_generated_network_equals_host(dtb, orc, gpu):
     ref, sim = run_oracle(orc, host, steps, seed, mu=2.0)
     assert_state_equal(sd, sim)
     assert ref["raster"].sum() > 1000
+
+
+def _edge_network(dtb, case):
+    """Hand-built networks at the edges of the layout (see the cases below);
+    weights k * 2^-24, channel bit = presynaptic neuron inhibitory."""
+    rng = np.random.default_rng(77)
+    if case == "single":
+        n = 1
+        pre = post = np.zeros(0, dtype=np.int64)
+    elif case == "ragged":
+        # two tiles, the second partial; empty rows, a one-synapse row, a
+        # 700-synapse row into one tile (a segment across passes), a row onto
+        # every neuron, repeated targets and a self-connection
+        n = 300
+        pre, post = [], []
+        pre += [5] * 700
+        post += list(rng.integers(0, 256, 700))
+        pre += [17] * n
+        post += list(range(n))
+        pre += [40]
+        post += [299]
+        pre += [41] * 6
+        post += [3, 3, 3, 280, 280, 41]
+        for r in range(60, 300, 3):
+            k = int(rng.integers(1, 90))
+            pre += [r] * k
+            post += list(rng.integers(0, n, k))
+        pre, post = np.asarray(pre), np.asarray(post)
+    elif case == "hot_tile":
+        # every synapse targets the first tile: its queue takes them all
+        n = 2048
+        pre = np.repeat(np.arange(n), 64)
+        post = rng.integers(0, 256, pre.size)
+    else:
+        raise ValueError(case)
+    props = dtb.gen_uniform(max(n, 2), 1, seed=3).props[:n]
+    if n == 1:
+        props = props.copy()
+        props[:, 3] = 0
+    kind = (pre >= int(n * 0.8)).astype(np.uint8)
+    w = rng.integers(0, 1 << 24, (pre.size, 2)).astype(np.float32) * np.float32(2.0 ** -24)
+    order = np.lexsort((post, pre))   # rows ordered by target, as the block files are
+    return dtb.NetworkArrays.from_edges(props, pre[order], post[order], w[order], kind[order])
+
+
+@pytest.mark.parametrize("case,mu", [("single", 3.0), ("ragged", 2.0), ("hot_tile", 2.0)])
+def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu):
+    """A single neuron without synapses, ragged rows over a partial tile, and
+    all synapses into one tile: raster, V trace and final state bit-exact."""
+    net = _edge_network(dtb, case)
+    steps, seed = 400, 99
+    ids = np.arange(net.n, dtype=np.uint32)
+    out, st, cnt = run_gpu(dtb, net, steps, seed, mu, sample=ids)
+    ref, sim = run_oracle(orc, net, steps, seed, mu, vtrace=True)
+    print(case, "n", net.n, "synapses", net.nsyn, "spikes", int(ref["raster"].sum()))
+    assert ref["raster"].sum() > 0
+    assert np.array_equal(out["spike"], ref["raster"]), "raster differs"
+    assert np.array_equal(bits(out["vmemb"]), bits(ref["V"])), "V trace differs"
+    assert_state_equal(st, sim)
+    deg = np.diff(net.rowptr)
+    assert cnt[1] == int((ref["raster"].astype(np.int64) * deg[None, :]).sum())
+    assert cnt[2] == int(ref["raster"].sum())
