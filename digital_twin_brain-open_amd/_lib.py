@@ -100,6 +100,8 @@ _SIGS = {
     "dtb_engine_run": (c_int, [c_void_p, c_int64, c_int32]),
     "dtb_configure": (c_int, [c_void_p, c_int, c_int32, c_int32]),
     "dtb_engine_sync": (c_int, [c_void_p]),
+    "dtb_engine_queue_capacity": (c_int, [c_void_p, c_uint32, c_uint32]),
+    "dtb_engine_spilled": (c_int, [c_void_p, POINTER(c_uint64)]),
     "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),
     "dtb_engine_time_kernels": (c_int, [c_void_p, c_int64, POINTER(c_float)]),
     "dtb_engine_check": (c_int, [c_void_p]),

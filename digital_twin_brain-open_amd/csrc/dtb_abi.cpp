@@ -340,8 +340,19 @@ int dtb_engine_check(dtb_engine_t h)
     return guarded([&] {
         dtb::Engine &e = engine(h);
         e.sync();
-        if (e.queue_overflow())
-            throw dtb::Error(dtb::kInvalidNetwork, "a propagation queue bound was exceeded");
+    });
+}
+
+int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift)
+{
+    return guarded([&] { engine(h).set_queue_capacity(floor, shift); });
+}
+
+int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses)
+{
+    return guarded([&] {
+        need(synapses != nullptr, "synapses is NULL");
+        *synapses = engine(h).spilled();
     });
 }
 

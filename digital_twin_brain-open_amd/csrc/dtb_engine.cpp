@@ -179,6 +179,7 @@ void Instance::init_common(const float *props, float dt, uint64_t seed, hipStrea
     iou_.resize(n_);
     tcnt_.resize(n_);
     dlt_.resize(n_);
+    spill_.resize((size_t)n_ * 8);
     rowptr_.resize((size_t)n_ + 1);
     col_.resize((size_t)std::max<int64_t>(nsyn_, 1));
     w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
@@ -422,6 +423,7 @@ void Instance::init_state(hipStream_t s)
     DTB_HIP(hipMemsetAsync(flag_.mutable_gpu_data(), 0, flag_.size(), s));
     DTB_HIP(hipMemsetAsync(J_.mutable_gpu_data(), 0, J_.size(), s));
     DTB_HIP(hipMemsetAsync(dlt_.mutable_gpu_data(), 0, dlt_.size(), s));
+    DTB_HIP(hipMemsetAsync(spill_.mutable_gpu_data(), 0, spill_.size(), s));
     DTB_HIP(hipMemsetAsync(step_.mutable_gpu_data(), 0, step_.size(), s));
     DTB_HIP(hipMemsetAsync(counters_.mutable_gpu_data(), 0, counters_.size(), s));
     DTB_HIP(hipStreamSynchronize(s));
@@ -733,6 +735,11 @@ void Instance::write_state(const float *V, const int32_t *t, const uint8_t *f, c
     DTB_HIP(hipStreamSynchronize(s));
 }
 
+void Instance::clear_spill(hipStream_t s)
+{
+    DTB_HIP(hipMemsetAsync(spill_.mutable_gpu_data(), 0, spill_.size(), s));
+}
+
 InstDev Instance::dev(uint32_t tile_base)
 {
     InstDev d{};
@@ -750,6 +757,7 @@ InstDev Instance::dev(uint32_t tile_base)
     d.iou = iou_.mutable_gpu_data();
     d.J = J_.mutable_gpu_data();
     d.dlt = dlt_.mutable_gpu_data();
+    d.spill = spill_.mutable_gpu_data();
     d.rowptr = rowptr_.gpu_data();
     d.col = col_.gpu_data();
     d.w = w_.gpu_data();

@@ -282,6 +282,9 @@ public:
     uint32_t tile_synapses(uint32_t tile) const;
     // 1 + the parameter class of every neuron of tile `tile` if they share
     // one, else 0.
+    // Zeroes both spill accumulators (the queues of this instance were
+    // cleared).
+    void clear_spill(hipStream_t s);
     uint32_t tile_class(uint32_t tile) const { return tile_cls_.at(tile); }
     // Mean over rows of the fraction of this instance's tiles that the row's
     // synapses reach (tile segments of the row / tiles).
@@ -316,6 +319,7 @@ private:
     DataAllocator<float> V_, iou_;
     DataAllocator<int32_t> tcnt_;
     DataAllocator<float4> dlt_;
+    DataAllocator<unsigned long long> spill_;      // [2][n][4] Q23.40
     DataAllocator<int64_t> rowptr_;
     DataAllocator<uint32_t> col_;
     DataAllocator<float2> w_;
@@ -355,7 +359,13 @@ public:
     // that many steps into one HIP graph and replays it.
     void run(int64_t steps, int graph_chunk);
     // True if a queue bound was ever exceeded (a bug, never expected).
-    bool queue_overflow();
+    // Synapses delivered through the spill accumulators since the engine was
+    // made (queues past their capacity; not an error).
+    uint64_t spilled();
+    // Queue capacity per tile: min(synapses targeting it, floor +
+    // (synapses >> shift)); shift 0 keeps the exact worst case.
+    void set_queue_capacity(uint32_t floor, uint32_t shift);
+    uint32_t queue_capacity(uint32_t synapses) const;
     void sync();
     void mark_dirty() { dirty_ = true; }
     // Writes the pending delta of every instance into its dlt buffer (the
@@ -396,6 +406,7 @@ private:
     uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;
     uint64_t qtotal_ = 0;                  // queue slots per parity
     uint32_t cshift_ = kCountShiftDense;   // queue count layout (count_slot)
+    uint32_t queue_floor_ = 4096, queue_shift_ = 3;   // queue_capacity
     DataAllocator<InstDev> d_insts_;
     DataAllocator<StepCtl> d_ctl_;
     DataAllocator<SampleDev> d_samples_;

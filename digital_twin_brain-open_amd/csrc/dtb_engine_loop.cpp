@@ -67,12 +67,30 @@ size_t Engine::active() const
     return k;
 }
 
-bool Engine::queue_overflow()
+uint64_t Engine::spilled()
 {
     StepCtl c{};
     d_ctl_.download(&c, 1, stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
-    return c.overflow != 0;
+    return c.spilled;
+}
+
+void Engine::set_queue_capacity(uint32_t floor, uint32_t shift)
+{
+    if (shift > 31)
+        throw Error(kInvalidParameter, "queue capacity shift must be at most 31");
+    queue_floor_ = floor;
+    queue_shift_ = shift;
+    dirty_ = true;
+}
+
+uint32_t Engine::queue_capacity(uint32_t synapses) const
+{
+    // Every row fires at most once per step, so `synapses` entries always
+    // suffice; fewer keep the queues small, and the synapses past the
+    // capacity take the spill path.
+    const uint64_t c = (uint64_t)queue_floor_ + (synapses >> queue_shift_);
+    return (uint32_t)std::min<uint64_t>(synapses, c);
 }
 
 void Engine::drop_graphs()
@@ -125,7 +143,7 @@ void Engine::rebuild()
             d.begin = b;
             d.count = std::min<uint32_t>(kBlock, p->n() - b);
             d.pop0 = q;
-            d.qcap = p->tile_synapses(t);
+            d.qcap = queue_capacity(p->tile_synapses(t));
             d.pcls = p->tile_class(t);
             d.qoff = qtotal;
             qtotal += d.qcap;
@@ -269,6 +287,7 @@ void Engine::clear_queues(int id)
         const size_t last = count_slot((size_t)p * ntiles_ + tile_base_[(size_t)id] + ntile - 1, cshift_);
         DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + first, 0, (last + 1 - first) * sizeof(uint32_t), stream_));
     }
+    get(id)->clear_spill(stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 

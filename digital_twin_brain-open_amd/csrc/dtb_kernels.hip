@@ -136,6 +136,21 @@ __device__ __forceinline__ void accumulate_tile(unsigned long long *acc, const Q
     }
 }
 
+// Adds neuron li's spill accumulator of queue set `set` into the neuron's own
+// LDS sums (row tid of acc); `consume` clears it for the set's next use.
+// Called by the neuron's thread after the queue accumulation.
+__device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDev &I, uint32_t set, uint32_t li,
+                                           uint32_t tid, bool consume)
+{
+    const auto s = gp(I.spill) + ((size_t)set * I.n + li) * 4u;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c) {
+        acc[tid * 4u + c] += s[c];
+        if (consume)
+            s[c] = 0ull;
+    }
+}
+
 // A statistic's value as the exact integer rint(x * 2^24).
 __device__ __forceinline__ long long stat_q(float x)
 {
@@ -204,20 +219,24 @@ constexpr uint32_t kPass = kWave * kPassK;
 // reserves its slots with one atomic add on the tile's count and its
 // synapses are stored contiguously.  Called by a whole wave with
 // wave-uniform arguments.  Lane l holds synapses c0 + 2l and c0 + 2l + 1 of
-// the row (segments end at pass boundaries).  A synapse's segment start is the last start at
-// or before it (within the lane, else from a wave max-scan of the lanes' last
-// starts); the lane holding the segment's last synapse reserves and
-// publishes the destination in seg_dst (this wave's kPass LDS entries,
-// indexed by segment start within the pass).
+// the row (segments end at pass boundaries).  A synapse's segment start is
+// the last start at or before it (within the lane, else from a wave max-scan
+// of the lanes' last starts); the lane holding the segment's last synapse
+// reserves and publishes the destination and the number of the segment's
+// synapses that fit the queue in seg_dst / seg_room (this wave's kPass LDS
+// entries, indexed by segment start within the pass).  Synapses past the
+// queue's capacity add their weights into the target's spill accumulator
+// instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
+// does not depend on which path a synapse took).
 __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
-                                        QSyn *__restrict__ queue, StepCtl *__restrict__ ctl,
-                                        unsigned long long *seg_dst, uint32_t lane)
+                                        QSyn *__restrict__ queue, unsigned long long *__restrict__ spill,
+                                        StepCtl *__restrict__ ctl, unsigned long long *seg_dst, uint32_t *seg_room,
+                                        uint32_t lane)
 {
     constexpr uint32_t K = kPassK;
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
-    constexpr unsigned long long kDrop = ~0ull;
     const auto col = gp(col_all) + row_begin;
     const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
     {
@@ -277,12 +296,11 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
                 const uint32_t n = j + 1u - open;
                 const uint32_t base = atomicAdd(&qcnt[count_slot(qset + tg, cshift)], n);
                 const TileDesc td = tiles[tg];
-                unsigned long long dst = td.qoff + base;
-                if (base + n > td.qcap) {
-                    dst = kDrop;
-                    ctl->overflow = 1u;
-                }
-                seg_dst[open - c0] = dst;
+                const uint32_t room = td.qcap > base ? min(n, td.qcap - base) : 0u;
+                if (room < n)
+                    atomicAdd(&ctl->spilled, (unsigned long long)(n - room));
+                seg_dst[open - c0] = td.qoff + base;
+                seg_room[open - c0] = room;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -292,13 +310,17 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t j = j0 + k;
             if (j < len) {
-                const unsigned long long dst = seg_dst[st[k] - c0];
-                if (dst != kDrop) {
+                const uint32_t at = j - st[k];
+                if (at < seg_room[st[k] - c0]) {
                     QSyn e;
                     e.key = (cv[k] & 0xffu) | ((cv[k] & kKindBit) ? kQSynKind : 0u);
                     e.w0 = wv[k].x;
                     e.w1 = wv[k].y;
-                    queue[dst + (j - st[k])] = e;
+                    queue[seg_dst[st[k] - c0] + at] = e;
+                } else {
+                    unsigned long long *a = spill + (size_t)(cv[k] & kColMask) * 4u + ((cv[k] & kKindBit) ? 2u : 0u);
+                    atomicAdd(a, f32_to_q40(wv[k].x));
+                    atomicAdd(a + 1, f32_to_q40(wv[k].y));
                 }
             }
         }
@@ -319,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
 {
     __shared__ unsigned long long acc[kBlock * 4];
     __shared__ unsigned long long seg_dst[4 * kPass];    // per wave: one pass
+    __shared__ uint32_t seg_room[4 * kPass];
     __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
     __shared__ uint32_t row_len[kBlock];                  // and length
     __shared__ uint32_t nrows;
@@ -370,6 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     __syncthreads();
     accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
     __syncthreads();
+    if (cnt_raw > d.qcap && active)
+        take_spill(acc, I, par, li, tid, true);   // synapses past the queue's capacity
     if (tid == 0)
         qs.count[cslot_in] = 0u;   // this launch consumed the queue
     if (d.begin == 0 && tid == 0)
@@ -517,7 +542,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         for (uint32_t c0 = 0; c0 < len_s; c0 += kPass, ++pass)
             if ((pass & 3u) == w)
                 bin_pass(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
-                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, ctl, seg_dst + w * kPass, lane);
+                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, I.spill + (size_t)(par ^ 1u) * I.n * 4u,
+                         ctl, seg_dst + w * kPass, seg_room + w * kPass, lane);
     }
 }
 
@@ -537,12 +563,16 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)], d.qcap);
+    const uint32_t cnt_raw = qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)];
+    const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
-    if (tid < d.count)
+    if (tid < d.count) {
+        if (cnt_raw > d.qcap)
+            take_spill(acc, I, set, d.begin + tid, tid, false);
         st4(I.dlt, d.begin + tid, acc_to_delta(acc, tid));
+    }
 }
 
 // Folds the pending delta of queue set `set` into J now (J = J * decay +
@@ -563,12 +593,15 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
-    const uint32_t cnt = min(*cnt_p, d.qcap);
+    const uint32_t cnt_raw = *cnt_p;
+    const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
+        if (cnt_raw > d.qcap)
+            take_spill(acc, I, set, li, tid, true);
         const uint32_t fl = gp(I.flag)[li];
         if (!(fl & kFlagFolded)) {
             st4(I.J, li, fold_delta(ld4(I.J, li), ld4(I.dec, param_index(I, li)), acc_to_delta(acc, tid)));

@@ -87,6 +87,7 @@ struct alignas(16) InstDev {
     float *iou;                 // OU current (reference i_ou_background_stimuli_)
     float4 *J;                  // J_u after the previous step's decay (delta below pending)
     float4 *dlt;                // materialised pending delta (sampling / read-back only)
+    unsigned long long *spill;  // [2][n][4] Q23.40 sums of the synapses past a queue's capacity
     // connectivity: presynaptic-major CSR, 64-bit row offsets
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
@@ -183,8 +184,8 @@ struct alignas(16) SampleRecDev {
 
 // Shared per-engine diagnostics in device memory.
 struct alignas(16) StepCtl {
-    uint32_t overflow;          // set if a queue bound was ever exceeded (never expected)
-    uint32_t pad[3];
+    unsigned long long spilled; // synapses that went to the spill accumulators (not an error)
+    unsigned long long pad;
 };
 
 }  // namespace dtb

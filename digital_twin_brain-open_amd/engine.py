@@ -285,6 +285,17 @@ class Engine:
     def check(self):
         _lib.check(self._lib.dtb_engine_check(self._h))
 
+    def queue_capacity(self, floor, shift):
+        """Per-tile queue capacity min(S_t, floor + (S_t >> shift)); synapses
+        past it go through the exact spill accumulators."""
+        _lib.check(self._lib.dtb_engine_queue_capacity(self._h, int(floor), int(shift)))
+
+    def spilled(self):
+        """Synapses delivered through the spill accumulators so far."""
+        v = c_uint64(0)
+        _lib.check(self._lib.dtb_engine_spilled(self._h, byref(v)))
+        return v.value
+
     def run_span(self, steps, graph_chunk=0):
         """Runs `steps` steps like run(); returns the GPU milliseconds between
         events recorded on the engine stream before and after them."""

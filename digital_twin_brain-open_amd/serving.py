@@ -22,11 +22,14 @@ import numpy as np
 from .engine import DEFAULT_G_UI, Block, Engine, device_memory_reserve
 
 # Device bytes per neuron of an instance: parameter class, V, i_ou,
-# refractory counter, flag, J, pending delta, blocked flag, row offset and
-# per-wave counters; per synapse: column index, weight pair and the two
-# propagation queues (one 12-byte entry per synapse each).
-BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 1 + 8 + 1
-BYTES_PER_SYNAPSE = 4 + 8 + 2 * 12
+# refractory counter, flag, J, pending delta, spill accumulators (2 x 32),
+# blocked flag, row offset and per-wave counters; per synapse: column index
+# and weight pair; the two propagation queues hold 12-byte entries, per tile
+# min(S_t, QUEUE_FLOOR + S_t / 2^QUEUE_SHIFT) of its S_t synapses (the
+# engine's default capacity).
+BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 64 + 1 + 8 + 1
+BYTES_PER_SYNAPSE = 4 + 8
+QUEUE_ENTRY_BYTES, QUEUE_FLOOR, QUEUE_SHIFT = 12, 4096, 3
 
 
 class Request:
@@ -42,7 +45,10 @@ class Request:
         self.v_hash = self.v_mean = None
 
     def device_bytes(self):
-        return self.n * BYTES_PER_NEURON + self.n * self.k * BYTES_PER_SYNAPSE
+        syn = self.n * self.k
+        tiles = (self.n + 255) // 256
+        queue = min(syn, tiles * QUEUE_FLOOR + (syn >> QUEUE_SHIFT))   # bound on the sum over tiles
+        return self.n * BYTES_PER_NEURON + syn * BYTES_PER_SYNAPSE + 2 * QUEUE_ENTRY_BYTES * queue
 
     def latency(self):
         return self.finished - self.arrival

@@ -180,9 +180,17 @@ int dtb_engine_run_span(dtb_engine_t h, int64_t steps, int32_t graph_chunk, floa
 /* Times `steps` steps with HIP events around each step kernel (delta
  * accumulation + neuron update + spike binning); total in ms. */
 int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step);
-/* Waits for the engine and reports DTB_INVALID_NETWORK if a propagation queue
- * bound was ever exceeded (a defect; never expected). */
+/* Waits for the engine and reports any runtime failure of its work. */
 int dtb_engine_check(dtb_engine_t h);
+/* No counterpart in the reference (it scatters into per-neuron buffers).
+ * Spikes travel through per-tile queues; a tile's queue holds
+ * min(S_t, floor + (S_t >> shift)) entries, S_t = synapses targeting the tile
+ * (default floor 4096, shift 3; shift 0 = always room for every synapse).
+ * Synapses past the capacity are added into a per-neuron spill accumulator
+ * with the same exact integer arithmetic, so results do not depend on the
+ * capacity; dtb_engine_spilled counts them.  Takes effect at the next run. */
+int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
+int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses);
 
 /* Checkpoint: reference-visible state after the last step (v_membranes_,
  * t_actives_, f_inner_actives_, emitted flags, i_ou_background_stimuli_,

@@ -472,3 +472,47 @@ def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu):
     deg = np.diff(net.rowptr)
     assert cnt[1] == int((ref["raster"].astype(np.int64) * deg[None, :]).sum())
     assert cnt[2] == int(ref["raster"].sum())
+
+
+@pytest.mark.parametrize("floor,shift", [(0, 31), (16, 31), (64, 4)])
+def test_queue_capacity_spill_bit_exact(dtb, orc, gpu, floor, shift):
+    """Queues smaller than their worst case: the synapses past a tile's
+    capacity take the spill accumulators (every synapse with capacity 0; a
+    segment split between queue and spill with 16 entries).  Raster, sampled
+    V and I_syn, and the state stay bit-exact, also across a parameter change
+    with spikes pending (settle) and a second run."""
+    n, steps, seed = 10000, 300, 31
+    net = dtb.gen_uniform(n, 100, seed=1)
+    ids = np.arange(0, n, 7, dtype=np.uint32)
+    eng = dtb.Engine(0)
+    eng.queue_capacity(floor, shift)
+    try:
+        blk = dtb.Block(net, delta_t=DT, seed=seed, engine=eng)
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 1.5, 0.1)
+        blk.set_samples(ids)
+        a = blk.run(steps, freqs=True, output_sample_spike=True, output_sample_vmemb=True,
+                    output_sample_isynaptic=True)
+        blk.assign_property_by_subblk([1], [5], [-51.0])   # settles the pending delta
+        b = blk.run(steps, freqs=True, output_sample_spike=True, output_sample_vmemb=True,
+                    output_sample_isynaptic=True)
+        st = blk.state()
+        spilled = eng.spilled()
+        eng.check()
+    finally:
+        eng.close()
+    on = orc.OracleNet.from_network(net, dt=DT, seed=seed)
+    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
+    sim = orc.Sim(on, 32)
+    ra = sim.run(steps, raster=True, vtrace=True, pop_counts=True)
+    on.props[on.props[:, 3] == 1, 8] = -51.0
+    rb = sim.run(steps, raster=True, vtrace=True, pop_counts=True)
+    print("floor", floor, "shift", shift, "spilled", spilled, "spikes", int(ra["raster"].sum() + rb["raster"].sum()))
+    assert spilled > 0
+    for out, ref in ((a, ra), (b, rb)):
+        assert np.array_equal(out["spike"], ref["raster"][:, ids]), "raster differs"
+        assert np.array_equal(bits(out["vmemb"]), bits(ref["V"][:, ids])), "V trace differs"
+        assert np.array_equal(out["freq"], ref["freq"])
+    assert_state_equal(st, sim)
+    # the sampled I_syn of the last step (pending delta materialised with the
+    # spill) is the state's I_syn
+    assert np.array_equal(bits(b["isynaptic"][-1]), bits(st["I_syn"][ids]))
