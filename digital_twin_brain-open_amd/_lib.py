@@ -100,6 +100,9 @@ _SIGS = {
     "dtb_engine_run": (c_int, [c_void_p, c_int64, c_int32]),
     "dtb_configure": (c_int, [c_void_p, c_int, c_int32, c_int32]),
     "dtb_engine_sync": (c_int, [c_void_p]),
+    "dtb_gen_gamma": (c_int, [c_uint64, c_uint32, c_uint32, c_uint32, c_float, c_float, c_void_p]),
+    "dtb_gamma_prop_cols": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int32]),
+    "dtb_read_props": (c_int, [c_void_p, c_int, c_void_p]),
     "dtb_engine_queue_capacity": (c_int, [c_void_p, c_uint32, c_uint32]),
     "dtb_engine_spilled": (c_int, [c_void_p, POINTER(c_uint64)]),
     "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),

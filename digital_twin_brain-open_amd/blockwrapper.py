@@ -174,6 +174,18 @@ class BlockWrapper:
         self._block.engine.update_prop_cols(self._block.inst, self._prop_ids(idx[:, 1]),
                                             idx[:, 0].astype(np.uint32), hp, assign=True)
 
+    def gamma_property_by_subblk(self, property_idx, gamma_concentration, gamma_rate):
+        """dist_blockwrapper.py:529-554: every neuron of each listed
+        population draws the property from Gamma(concentration, rate);
+        concentration None means 1."""
+        idx = _np(property_idx, np.int64).reshape(-1, 2)
+        rate = _np(gamma_rate, np.float32).reshape(-1)
+        conc = np.ones_like(rate) if gamma_concentration is None else _np(gamma_concentration, np.float32).reshape(-1)
+        if not ((conc >= 0).all() and (rate >= 0).all()):
+            raise ValueError("gamma hyperparameters must be non-negative")
+        self._block.engine.gamma_prop_cols(self._block.inst, self._prop_ids(idx[:, 1]),
+                                           idx[:, 0].astype(np.uint32), conc, rate)
+
     def set_samples(self, sample_idx, bid=None):
         """dist_blockwrapper.py:556-581."""
         s = _np(sample_idx, np.int64).reshape(-1)

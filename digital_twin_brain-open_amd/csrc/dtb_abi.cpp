@@ -429,6 +429,35 @@ int dtb_gen_uniform(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uin
     });
 }
 
+int dtb_gen_gamma(uint64_t seed, uint32_t call, uint32_t first, uint32_t count, float alpha, float beta, float *out)
+{
+    return guarded([&] {
+        need(out != nullptr || count == 0, "out is NULL");
+        dtb::gen_gamma(seed, call, first, count, alpha, beta, out);
+    });
+}
+
+int dtb_gamma_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, const uint32_t *brain_ids,
+                        const float *concentration, const float *rate, int32_t n)
+{
+    return guarded([&] {
+        need(n >= 0 && (n == 0 || (prop_ids && brain_ids && concentration && rate)), "gamma arrays are required");
+        dtb::Engine &e = engine(h);
+        e.get(inst);
+        e.settle();
+        e.get(inst)->gamma_prop_cols(prop_ids, brain_ids, concentration, rate, (size_t)n, e.stream());
+        e.mark_dirty();   // the parameter tables may have been reallocated
+    });
+}
+
+int dtb_read_props(dtb_engine_t h, int inst, float *props)
+{
+    return guarded([&] {
+        need(props != nullptr, "props is NULL");
+        engine(h).get(inst)->read_props(props);
+    });
+}
+
 int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
                   const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                   uint32_t *connkinds, uint32_t *vox_sizes)

@@ -507,6 +507,38 @@ void Instance::update_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_
     derive_params(s);
 }
 
+void Instance::gamma_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_ids, const float *alpha,
+                               const float *beta, size_t n, hipStream_t s)
+{
+    // block.cpp:1472-1536, property.cpp:470-640: every neuron of the
+    // population gets its own Gamma(alpha, rate beta) draw as the property.
+    for (size_t k = 0; k < n; ++k) {
+        if (prop_type_column((int)prop_ids[k]) < 0)
+            throw Error(kInvalidParameter, "unknown property id");
+        if (!(alpha[k] >= 0.0f) || !(beta[k] > 0.0f) || !std::isfinite(alpha[k]) || !std::isfinite(beta[k]))
+            throw Error(kInvalidParameter, "gamma needs concentration >= 0 and rate > 0");
+    }
+    std::vector<float> draws;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t call = gamma_calls_++;
+        const uint32_t q = find_pop(brain_ids[k]);
+        if (q == UINT32_MAX)
+            continue;
+        const int c = prop_type_column((int)prop_ids[k]);
+        const uint32_t b = pop_begin_[q], e = pop_begin_[q + 1];
+        draws.resize(e - b);
+        gen_gamma(seed_, call, b, e - b, alpha[k], beta[k], draws.data());
+        for (uint32_t i = b; i < e; ++i)
+            props_[(size_t)i * kNumCols + c] = draws[i - b];
+    }
+    derive_params(s);
+}
+
+void Instance::read_props(float *out) const
+{
+    std::memcpy(out, props_.data(), sizeof(float) * props_.size());
+}
+
 void Instance::set_samples(const uint32_t *ids, size_t n)
 {
     for (size_t k = 0; k < n; ++k)

@@ -245,6 +245,12 @@ public:
                       hipStream_t s);
     void update_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_ids, const float *vals, size_t n,
                           bool assign, hipStream_t s);
+    // Per-population Gamma(concentration, rate) resampling of properties:
+    // draw stream = (seed, call count of this instance, neuron).
+    void gamma_prop_cols(const uint32_t *prop_ids, const uint32_t *brain_ids, const float *alpha, const float *beta,
+                         size_t n, hipStream_t s);
+    // The [n][22] property matrix as updated so far.
+    void read_props(float *out) const;
     void set_samples(const uint32_t *ids, size_t n);
     const std::vector<uint32_t> &samples() const { return samples_; }
 
@@ -336,6 +342,7 @@ private:
     bool rec_freq_ = false, rec_samples_ = false;
     std::vector<uint32_t> tile_syn_;      // synapses per target tile
     double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
+    uint32_t gamma_calls_ = 0;            // gamma draw streams used
     std::vector<uint32_t> tile_cls_;      // per tile: 1 + shared parameter class, or 0
 };
 

@@ -350,4 +350,41 @@ void gen_voxel(int64_t n, uint32_t nvox, uint32_t K, float intra_frac, float den
     build_csr(n, K, fn, rowptr, col, w, kind_bits);
 }
 
+float gamma_draw(uint64_t seed, uint32_t call, uint32_t neuron, float alpha, float beta)
+{
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    if (alpha == 0.0f)
+        return 0.0f;
+    float scale = 1.0f;
+    if (alpha < 1.0f) {
+        // boost: Gamma(alpha) = Gamma(alpha + 1) * U^(1/alpha)
+        const Philox4 b = philox4x32_10(neuron, call, 0xffffffffu, kTagGamma, k0, k1);
+        const float u = 1.0f - uniform_from_bits(b.x);   // (0, 1]
+        scale = std::pow(u, 1.0f / alpha);
+        alpha += 1.0f;
+    }
+    const float d = alpha - 1.0f / 3.0f;
+    const float c = 1.0f / std::sqrt(9.0f * d);
+    for (uint32_t attempt = 0;; ++attempt) {
+        const Philox4 x4 = philox4x32_10(neuron, call, attempt, kTagGamma, k0, k1);
+        const float x = normal_from_bits(x4.x, x4.y);
+        const float y = 1.0f + c * x;
+        if (y <= 0.0f)
+            continue;
+        const float v = y * y * y;
+        const float u = 1.0f - uniform_from_bits(x4.z);   // (0, 1]
+        const float xx = x * x;
+        if (u < 1.0f - 0.0331f * xx * xx || std::log(u) < 0.5f * xx + d * (1.0f - v + std::log(v)))
+            return scale * d * v / beta;
+    }
+}
+
+void gen_gamma(uint64_t seed, uint32_t call, uint32_t first, uint32_t count, float alpha, float beta, float *out)
+{
+    if (!(alpha >= 0.0f) || !(beta > 0.0f) || !std::isfinite(alpha) || !std::isfinite(beta))
+        throw Error(kInvalidParameter, "gamma needs alpha >= 0 and beta > 0");
+    for (uint32_t i = 0; i < count; ++i)
+        out[i] = gamma_draw(seed, call, first + i, alpha, beta);
+}
+
 }  // namespace dtb

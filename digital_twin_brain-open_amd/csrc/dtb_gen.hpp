@@ -33,4 +33,13 @@ void gen_voxel(int64_t n, uint32_t nvox, uint32_t K, float intra_frac, float den
                const float g_ui[4], float *props, int64_t *rowptr, uint32_t *col, float *w, uint32_t *kind_bits,
                uint32_t *vox_sizes);
 
+// Gamma(alpha, rate beta) draws for neurons first .. first + count - 1 of
+// stream `call` (property.cpp:470-560 semantics: alpha < 1 is boosted by
+// U^(1/alpha); alpha = 0 gives 0).  Marsaglia and Tsang (2000), "A simple
+// method for generating gamma variables", ACM TOMS 26(3), with the normal
+// and uniform deviates of Philox4x32-10, counter (neuron, call, attempt,
+// gamma tag), key = seed.
+float gamma_draw(uint64_t seed, uint32_t call, uint32_t neuron, float alpha, float beta);
+void gen_gamma(uint64_t seed, uint32_t call, uint32_t first, uint32_t count, float alpha, float beta, float *out);
+
 }  // namespace dtb

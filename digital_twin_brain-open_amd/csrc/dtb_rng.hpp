@@ -19,7 +19,7 @@
 
 namespace dtb {
 
-enum : uint32_t { kTagOU = 1u, kTagNoise = 2u, kTagGen = 3u };
+enum : uint32_t { kTagOU = 1u, kTagNoise = 2u, kTagGen = 3u, kTagGamma = 4u };
 
 struct Philox4 {
     uint32_t x, y, z, w;

@@ -200,6 +200,14 @@ def device_count():
     return c.value
 
 
+def gen_gamma(seed, call, first, count, alpha, beta):
+    """Gamma(alpha, rate beta) draws of the property resampler for neurons
+    first .. first + count - 1 of stream `call` (host)."""
+    out = np.empty(count, dtype=np.float32)
+    _lib.check(_lib.load().dtb_gen_gamma(seed, call, first, count, alpha, beta, _ptr(out)))
+    return out
+
+
 def device_memory_cached(device=0):
     """Bytes of freed device memory the library keeps for reuse on `device`."""
     b = c_uint64(0)
@@ -350,6 +358,19 @@ class Engine:
         _lib.check(self._lib.dtb_update_prop_cols(self._h, inst, _ptr(pid), _ptr(bid), _ptr(v), bid.size,
                                                   int(bool(assign))))
 
+    def gamma_prop_cols(self, inst, prop_ids, brain_ids, concentration, rate):
+        bid = np.ascontiguousarray(brain_ids, dtype=np.uint32)
+        pid = np.broadcast_to(np.asarray(prop_ids, dtype=np.uint32), bid.shape).copy()
+        a = np.broadcast_to(np.asarray(concentration, dtype=np.float32), bid.shape).copy()
+        b = np.broadcast_to(np.asarray(rate, dtype=np.float32), bid.shape).copy()
+        _lib.check(self._lib.dtb_gamma_prop_cols(self._h, inst, _ptr(pid), _ptr(bid), _ptr(a), _ptr(b), bid.size))
+
+    def read_props(self, inst):
+        """The instance's property matrix [n][22] with every update applied."""
+        out = np.empty((self.info(inst)["n"], NCOL), dtype=np.float32)
+        _lib.check(self._lib.dtb_read_props(self._h, inst, _ptr(out)))
+        return out
+
     def run_block(self, inst, iters, t_steps=1, use_ou=True, freqs=True, sample_spike=False, sample_vmemb=False,
                   sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0, vmean=False,
                   imean=False, receptor_imean=False):
@@ -456,6 +477,15 @@ class Block:
     def assign_property_by_subblk(self, population_id, property_idx, hyper_parameter):
         """dist_blockwrapper.py:517-527, assigns per population."""
         self.engine.update_prop_cols(self.inst, property_idx, population_id, hyper_parameter, assign=True)
+
+    def gamma_property_by_subblk(self, population_id, property_idx, concentration, rate):
+        """dist_blockwrapper.py:529-554: each neuron of the population draws
+        the property from Gamma(concentration, rate)."""
+        self.engine.gamma_prop_cols(self.inst, property_idx, population_id, concentration, rate)
+
+    def props(self):
+        """Property matrix [n][22] with every update applied."""
+        return self.engine.read_props(self.inst)
 
     def run(self, iterations, freqs=True, output_sample_spike=False, output_sample_vmemb=False,
             output_sample_iou=False, output_sample_isynaptic=False, iou=True, t_steps=1, graph=False,

@@ -11,10 +11,11 @@ with grpc's generic handlers.  The handlers map one to one onto the C ABI:
   Run                     dtb_run, one RunResponse per step
   Updateprop              dtb_update_props
   Updatehyperpara         dtb_update_prop_cols (assigned -> assign)
+  Updategamma             dtb_gamma_prop_cols
   Updatesample            dtb_set_samples
   Updateoubackgroundparam dtb_set_ou
   GetVersion, Shutdown
-  Updategamma, Updatettypecacurrentparam, Updatedopaminecurrentparam,
+  Updatettypecacurrentparam, Updatedopaminecurrentparam,
   Updateadaptationcurrentparam: success = false (not supported here)
 """
 import threading
@@ -267,6 +268,20 @@ class SnnServicer:
             except _lib.DtbError:
                 ok = False
         return self.m.UpdateOUBackgroundParamResponse(success=ok)
+
+    def Updategamma(self, it, context):
+        ok = self.block is not None
+        for req in it:
+            if not ok or not req.brain_id:
+                continue
+            try:
+                self.block.engine.gamma_prop_cols(self.block.inst, np.asarray(req.prop_id, np.uint32),
+                                                  np.asarray(req.brain_id, np.uint32),
+                                                  np.asarray(req.gamma_concentration, np.float32),
+                                                  np.asarray(req.gamma_rate, np.float32))
+            except _lib.DtbError:
+                ok = False
+        return self.m.UpdateGammaResponse(success=ok)
 
     def _unsupported(self, resp):
         def handler(it, context):

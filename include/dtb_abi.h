@@ -160,6 +160,17 @@ int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const
 int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, const uint32_t *brain_ids,
                          const float *vals, int32_t n, int32_t assign);
 
+/* BrainBlock::update_gamma_Prop_Cols_gpu, src/block.cpp:1472-1536 ->
+ * src/stage/property.cpp:470-640 (Snn.Updategamma): every neuron of
+ * population brain_ids[k] gets its own Gamma(concentration[k], rate[k])
+ * draw as property prop_ids[k] (dtb_gen_gamma's stream, keyed by the
+ * instance seed, one stream per (call, k)). */
+int dtb_gamma_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, const uint32_t *brain_ids,
+                        const float *concentration, const float *rate, int32_t n);
+/* No counterpart in the reference (its properties live on the device): the
+ * instance's property matrix [n][22] with every update applied. */
+int dtb_read_props(dtb_engine_t h, int inst, float *props);
+
 /* The Run loop of tools/dist_simulator.cpp:1820-2332 (snn_run) for one
  * instance: iters steps of t_steps substeps, per-step outputs into `out`.
  * Every other instance of the engine advances too. */
@@ -229,6 +240,10 @@ int dtb_gen_fanout(int64_t n, uint32_t k, uint64_t seed, const float *g_ui, uint
 int dtb_gen_voxel(int64_t n, uint32_t nvox, uint32_t k, float intra_frac, float density, uint64_t seed,
                   const float *g_ui, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                   uint32_t *connkinds, uint32_t *vox_sizes);
+/* Gamma(alpha, rate beta) draws of neurons first .. first + count - 1 in
+ * stream `call` (Marsaglia-Tsang with Philox4x32-10 deviates; the sampler of
+ * dtb_gamma_prop_cols, on the host). */
+int dtb_gen_gamma(uint64_t seed, uint32_t call, uint32_t first, uint32_t count, float alpha, float beta, float *out);
 
 /* ---- reference block files (host) --------------------------------------- */
 

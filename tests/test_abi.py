@@ -138,3 +138,27 @@ def test_gen_fanout_structure(dtb):
     assert abs(indeg.mean() - k) < 1e-9 and indeg.std() < 3 * np.sqrt(k)
     pops, begin = a.populations()
     assert list(pops) == [0, 1] and list(begin) == [0, n * 8 // 10, n]
+
+
+@pytest.mark.parametrize("alpha", [0.3, 1.0, 2.5, 20.0])
+def test_gamma_sampler_distribution(dtb, alpha):
+    """The property resampler's Gamma(alpha, rate) draws: Kolmogorov-Smirnov
+    against the gamma CDF, mean alpha / rate, and distinct streams per call
+    and per neuron; alpha = 0 gives 0."""
+    scipy_stats = pytest.importorskip("scipy.stats")
+    rate = 2.0
+    x = dtb.gen_gamma(7, 3, 0, 100000, alpha, rate).astype(np.float64)
+    ks = scipy_stats.kstest(x, scipy_stats.gamma(alpha, scale=1.0 / rate).cdf)
+    assert ks.pvalue > 1e-4, ks
+    assert abs(x.mean() - alpha / rate) < 5 * np.sqrt(alpha) / rate / np.sqrt(x.size)
+    y = dtb.gen_gamma(7, 4, 0, 1000, alpha, rate)
+    assert not np.array_equal(y, x[:1000].astype(np.float32))
+    z = dtb.gen_gamma(7, 3, 500, 500, alpha, rate)
+    assert np.array_equal(z, x[500:1000].astype(np.float32))   # per-neuron streams
+    assert (dtb.gen_gamma(7, 3, 0, 10, 0.0, rate) == 0).all()
+
+
+def test_gamma_sampler_rejects_bad_parameters(dtb):
+    for a, b in [(-1.0, 1.0), (1.0, 0.0), (float("nan"), 1.0)]:
+        with pytest.raises(dtb.DtbError):
+            dtb.gen_gamma(1, 0, 0, 4, a, b)

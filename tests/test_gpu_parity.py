@@ -516,3 +516,37 @@ def test_queue_capacity_spill_bit_exact(dtb, orc, gpu, floor, shift):
     # the sampled I_syn of the last step (pending delta materialised with the
     # spill) is the state's I_syn
     assert np.array_equal(bits(b["isynaptic"][-1]), bits(st["I_syn"][ids]))
+
+
+def test_gamma_resampling_matches_oracle(dtb, orc, gpu):
+    """Per-population gamma resampling of properties (Snn.Updategamma): the
+    population's neurons hold the sampler's draws (one stream per call), and
+    the run with those per-neuron properties is bit-exact against the oracle
+    given the same property matrix."""
+    n, steps, seed = 6000, 300, 13
+    net = dtb.gen_uniform(n, 100, seed=21)
+    blk = dtb.Block(net, delta_t=DT, seed=seed)
+    try:
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 1.5, 0.1)
+        pops = blk.engine.info(blk.inst)
+        b0, b1 = int(pops["pop_begin"][0]), int(pops["pop_begin"][1])
+        # g_AMPA (prop id 7) of population 0 ~ Gamma(4, 400): mean 0.01;
+        # i_ext (prop id 0) of population 1 ~ Gamma(0.5, 1)
+        blk.gamma_property_by_subblk([0, 1], [7, 0], [4.0, 0.5], [400.0, 1.0])
+        props = blk.props()
+        out = blk.run(steps)
+        st = blk.state()
+    finally:
+        blk.engine.close()
+    assert np.array_equal(props[b0:b1, 10], dtb.gen_gamma(seed, 0, b0, b1 - b0, 4.0, 400.0))
+    assert np.array_equal(props[b1:, 2], dtb.gen_gamma(seed, 1, b1, n - b1, 0.5, 1.0))
+    untouched = np.ones(dtb.NCOL, bool)
+    untouched[[2, 10]] = False
+    assert np.array_equal(props[:, untouched], net.props[:, untouched])
+    on = orc.OracleNet(net.props.copy(), net.rowptr, net.col, net.w, net.kind_per_synapse(), dt=DT, seed=seed)
+    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
+    sim = orc.Sim(on, 32)
+    on.props[:] = props
+    ref = sim.run(steps, pop_counts=True)
+    assert np.array_equal(out["freq"], ref["freq"])
+    assert_state_equal(st, sim)
