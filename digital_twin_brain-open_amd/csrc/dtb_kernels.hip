@@ -331,22 +331,16 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
 }
 
 
-// The step of launch parity `par`: drains queue set par, fills set par ^ 1,
-// reads the step index from step[par] and advances it into step[par ^ 1].
-__global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ tiles,
-                                                 const InstDev *__restrict__ insts,
-                                                 QueueSet qs,
-                                                 uint32_t par,
-                                                 StepCtl *__restrict__ ctl)
+// One tile's step (the body of k_step).  kUniform: every neuron of the tile
+// has parameter class d.pcls - 1, so the parameter records are read once per
+// tile through the scalar unit instead of per lane.
+template <bool kUniform>
+__device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__restrict__ tiles,
+                                          const InstDev *__restrict__ insts, const QueueSet &qs, uint32_t par,
+                                          StepCtl *__restrict__ ctl, unsigned long long *acc,
+                                          unsigned long long *seg_dst, uint32_t *seg_room, long long *row_rb,
+                                          uint32_t *row_len, uint32_t &nrows)
 {
-    __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ unsigned long long seg_dst[4 * kPass];    // per wave: one pass
-    __shared__ uint32_t seg_room[4 * kPass];
-    __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
-    __shared__ uint32_t row_len[kBlock];                  // and length
-    __shared__ uint32_t nrows;
-
-    const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -364,9 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
     const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
     // A tile whose neurons share one parameter class (the usual case: a
     // population spans many tiles) reads its records without the class load.
-    uint32_t c = d.pcls - 1u;
-    if (d.pcls == 0u)
-        c = (lc & I.pmask) + (uint32_t)gp(I.cls)[lc];
+    const uint32_t c = kUniform ? d.pcls - 1u : (lc & I.pmask) + (uint32_t)gp(I.cls)[lc];
     float4 J = ld4(I.J, lc);
     float V = gp(I.V)[lc];
     int32_t t = gp(I.tcnt)[lc];
@@ -402,25 +394,35 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
 
     bool emit = false;
     uint32_t p = d.pop0;
-    long long stv[kStatFields];   // statistics of this neuron (when recorded)
+    float isyn = 0.0f;
     if (active) {
         if (!(fl & kFlagFolded))
             J = fold_delta(J, dc, acc_to_delta(acc, tid));
-        uint32_t f = fl & kFlagSpike;
-        float isyn = syn_current(g, vu, J, V);
-        if (I.rec_stat != nullptr && current_fields(I.stat_mask) != 0u) {
-            // I_syn and receptor currents at the end of the previous step
-            // (J folded, V before this step's update).
+        isyn = syn_current(g, vu, J, V);
+        while (p + 1 < I.npop && li >= I.pops[p].end)
+            ++p;
+    }
+    // stat_result.cpp:266-391: population sums, blocked neurons excluded.
+    // The I_syn-based statistics belong to the previous step (its J is
+    // complete only now: J folded, V before this step's update); the V
+    // statistic (after the update, below) to this one.
+    const int64_t row_v = step - I.rec_step0;
+    const bool stat_valid = I.rec_stat != nullptr && active && !gp(I.blocked)[lc];
+    if (I.rec_stat != nullptr) {
+        const uint32_t cf = current_fields(I.stat_mask);
+        if (cf != 0u && row_v - 1 >= 0 && row_v - 1 < (int64_t)I.rec_len) {
+            long long stv[kStatFields] = {};
             const float4 rc = receptor_currents(g, vu, J, V);
             stv[1] = stat_q(isyn);
             stv[2] = stat_q(rc.x);
             stv[3] = stat_q(rc.y);
             stv[4] = stat_q(rc.z);
             stv[5] = stat_q(rc.w);
+            wave_pop_stats(I.rec_stat + (size_t)(row_v - 1) * I.npop * kStatFields, stat_valid, p, stv, cf, lane);
         }
-
-        while (p + 1 < I.npop && li >= I.pops[p].end)
-            ++p;
+    }
+    if (active) {
+        uint32_t f = fl & kFlagSpike;
         const PopDev pop = I.pops[p];
         const bool ou = I.use_ou && pop.ou_on;
         const int32_t T = I.t_steps;
@@ -471,20 +473,11 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
         gp(I.flag)[li] = (uint8_t)(f | (emit ? kFlagEmit : 0u));
         gp(I.iou)[li] = iou;
         st4(I.J, li, J);
-        stv[0] = stat_q(V);
     }
-
-    if (I.rec_stat != nullptr) {
-        // stat_result.cpp:266-391: V of this step, I_syn-based statistics of
-        // the previous step (its J is complete only now), blocked neurons
-        // excluded.
-        const bool valid = active && !gp(I.blocked)[li];
-        const int64_t row_v = step - I.rec_step0;
-        if ((I.stat_mask & kStatV) && row_v >= 0 && row_v < (int64_t)I.rec_len)
-            wave_pop_stats(I.rec_stat + (size_t)row_v * I.npop * kStatFields, valid, p, stv, 1u, lane);
-        const uint32_t cf = current_fields(I.stat_mask);
-        if (cf != 0u && row_v - 1 >= 0 && row_v - 1 < (int64_t)I.rec_len)
-            wave_pop_stats(I.rec_stat + (size_t)(row_v - 1) * I.npop * kStatFields, valid, p, stv, cf, lane);
+    if (I.rec_stat != nullptr && (I.stat_mask & kStatV) && row_v >= 0 && row_v < (int64_t)I.rec_len) {
+        long long stv[kStatFields] = {};
+        stv[0] = stat_q(V);
+        wave_pop_stats(I.rec_stat + (size_t)row_v * I.npop * kStatFields, stat_valid, p, stv, 1u, lane);
     }
 
     // Spike propagation (presynaptic_voltage.cpp:122-225): the tile's fired
@@ -545,6 +538,28 @@ __global__ __launch_bounds__(kBlock) void k_step(const TileDesc *__restrict__ ti
                          (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, I.spill + (size_t)(par ^ 1u) * I.n * 4u,
                          ctl, seg_dst + w * kPass, seg_room + w * kPass, lane);
     }
+}
+
+// The step of launch parity `par`: drains queue set par, fills set par ^ 1,
+// reads the step index from step[par] and advances it into step[par ^ 1].
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_step(const TileDesc *__restrict__ tiles,
+                                                 const InstDev *__restrict__ insts,
+                                                 QueueSet qs,
+                                                 uint32_t par,
+                                                 StepCtl *__restrict__ ctl)
+{
+    __shared__ unsigned long long acc[kBlock * 4];
+    __shared__ unsigned long long seg_dst[4 * kPass];    // per wave: one pass
+    __shared__ uint32_t seg_room[4 * kPass];
+    __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
+    __shared__ uint32_t row_len[kBlock];                  // and length
+    __shared__ uint32_t nrows;
+
+    const TileDesc d = tiles[blockIdx.x];
+    if (d.pcls != 0u)
+        step_tile<true>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
+    else
+        step_tile<false>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
 }
 
 // Writes the pending delta (the runs queued in queue set `set` for the next
