@@ -40,6 +40,12 @@ OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
 NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
 GRAPH_CHUNK = 100
+# The warm-up ends with the autotuner, in the firing regime that is timed:
+# the schedules (2 count layouts x 3 pass sizes) take turns in 4-step windows
+# until each has run about TRIAL timed steps, the fastest total is kept; then
+# GRAPH_CHUNK steps capture the graph.
+AUTOTUNE_TRIAL = 32
+AUTOTUNE_STEPS = 6 * (2 + (AUTOTUNE_TRIAL + 3) // 4 * 6)
 # rocprofv3 FETCH_SIZE + WRITE_SIZE of one k_step launch per workload (at
 # its default size), scaled by a device-copy calibration (each file gives
 # its commands and factors).
@@ -100,10 +106,24 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
         eng.configure(b.inst, 1, True)
         return b
 
+    trial = 16
+    tuned_steps = dtb.Engine.autotune_steps(trial)
+
+    def tune(eng):
+        # warm-up into the firing regime, autotune there, capture the graph
+        if warmup >= tuned_steps + 2 * chunk:
+            eng.run(warmup - tuned_steps - chunk, graph_chunk=0)
+            eng.autotune(trial)
+            eng.run(chunk, graph_chunk=chunk)
+        else:
+            eng.run(warmup, graph_chunk=chunk)
+        eng.sync()
+        k, _, layout = eng.schedule()
+        return {"pass_synapses": 64 * k, "count_layout": layout, "autotuned": warmup >= tuned_steps + 2 * chunk}
+
     eng = dtb.Engine(0)
     blks = [setup(eng, i, net) for i, net in enumerate(nets)]
-    eng.run(warmup, graph_chunk=chunk)
-    eng.sync()
+    schedule = tune(eng)
     c0 = [eng.counters(b.inst) for b in blks]
     t0 = time.perf_counter()
     eng.run(steps, graph_chunk=chunk)
@@ -119,8 +139,7 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
     for i, net in enumerate(nets):
         e = dtb.Engine(0)
         b = setup(e, i, net)
-        e.run(warmup, graph_chunk=chunk)
-        e.sync()
+        tune(e)
         s0 = e.counters(b.inst)
         t0 = time.perf_counter()
         e.run(steps, graph_chunk=chunk)
@@ -135,6 +154,8 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
                         "OU mu %.1f, dt %.1f ms" % (ENS_INSTANCES, ENS_MIN, ENS_MAX, int(sizes.sum()), ENS_DEGREE,
                                                    ENS_MU, DT_MS),
             "steps": steps, "warmup": warmup, "graph_chunk": chunk,
+            "batched_schedule": schedule, "autotune": "every engine (batched, and each plain instance) "
+                                                      "autotuned over %d warm-up steps" % tuned_steps,
             "batched_events_per_s": ev_b / tb, "plain_events_per_s": ev_p / tp, "speedup": tp / tb,
             "batched_ms_per_step": tb * 1e3 / steps, "plain_ms_per_step": tp * 1e3 / steps,
             "mean_rate_hz": spikes / (int(sizes.sum()) * steps * DT_MS * 1e-3),
@@ -252,7 +273,7 @@ def main():
                     help="only run device copies of this size (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10000; stress 2000)")
-    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 500; stress 200)")
+    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 1000: the block fires after a few hundred; the autotuner runs at the end)")
     ap.add_argument("--cpu-steps", type=int, default=1000, help="CPU oracle sample length (0 = skip)")
     ap.add_argument("--workload", choices=("block", "stress", "serving"), default="block",
                     help="block: the 2M-neuron block (headline); stress: 500k neurons at in-degree 5000; "
@@ -291,7 +312,7 @@ def main():
     if args.steps <= 0:
         args.steps = 10000 if args.workload == "block" else 2000
     if args.warmup < 0:
-        args.warmup = 500 if args.workload == "block" else 200
+        args.warmup = 1000
     t0 = time.perf_counter()
     net, mu, sigma, desc = make_workload(dtb, args.workload, args.neurons)
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
@@ -302,9 +323,19 @@ def main():
     eng.configure(blk.inst, 1, True)
     log("uploaded in %.1f s" % (time.perf_counter() - t0))
 
-    # the warm-up also captures the step graph the timed region replays
-    eng.run(args.warmup, graph_chunk=GRAPH_CHUNK)
+    # warm-up: steps into the firing regime, the autotuner's trial steps,
+    # then steps that also capture the graph the timed region replays
+    k_best, table = None, {}
+    if args.warmup >= AUTOTUNE_STEPS + 2 * GRAPH_CHUNK:
+        eng.run(args.warmup - AUTOTUNE_STEPS - GRAPH_CHUNK, graph_chunk=0)
+        k_best, _, table = eng.autotune(AUTOTUNE_TRIAL)
+        eng.run(GRAPH_CHUNK, graph_chunk=GRAPH_CHUNK)
+    else:
+        eng.run(args.warmup, graph_chunk=GRAPH_CHUNK)
     eng.sync()
+    pk, _, layout_used = eng.schedule()
+    schedule = {"pass_synapses": 64 * pk, "count_layout": layout_used, "autotuned": bool(k_best),
+                "trial_us_per_step": {"%s/%d" % (lay, 64 * kk): round(us, 2) for (lay, kk), us in table.items()}}
     step0, ev0, sp0 = eng.counters(blk.inst)
 
     # Timed region: K steps replayed from HIP graphs of GRAPH_CHUNK steps,
@@ -354,6 +385,7 @@ def main():
         "events_per_step": events / args.steps,
         "kernel_us_per_step": {"k_step": us},
         "kernel_us_isolated": {"k_step": us_iso},
+        "schedule": schedule,
         "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
         "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -104,6 +104,9 @@ _SIGS = {
     "dtb_gamma_prop_cols": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int32]),
     "dtb_read_props": (c_int, [c_void_p, c_int, c_void_p]),
     "dtb_engine_queue_capacity": (c_int, [c_void_p, c_uint32, c_uint32]),
+    "dtb_engine_schedule": (c_int, [c_void_p, c_int32, c_int32]),
+    "dtb_engine_get_schedule": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+    "dtb_engine_autotune": (c_int, [c_void_p, c_int64, POINTER(c_int32), POINTER(c_int32), c_void_p]),
     "dtb_engine_spilled": (c_int, [c_void_p, POINTER(c_uint64)]),
     "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),
     "dtb_engine_time_kernels": (c_int, [c_void_p, c_int64, POINTER(c_float)]),

@@ -348,6 +348,40 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift)
     return guarded([&] { engine(h).set_queue_capacity(floor, shift); });
 }
 
+int dtb_engine_schedule(dtb_engine_t h, int32_t pass_k, int32_t layout)
+{
+    return guarded([&] {
+        need(pass_k > 0, "pass size must be 1, 2 or 4");
+        engine(h).set_schedule((uint32_t)pass_k, layout);
+    });
+}
+
+int dtb_engine_get_schedule(dtb_engine_t h, int32_t *pass_k, int32_t *layout, int32_t *layout_in_use)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        if (pass_k)
+            *pass_k = (int32_t)e.pass_k();
+        if (layout)
+            *layout = e.count_layout();
+        if (layout_in_use)
+            *layout_in_use = e.count_shift() == dtb::kCountShiftSpread ? DTB_LAYOUT_SPREAD : DTB_LAYOUT_DENSE;
+    });
+}
+
+int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, int32_t *layout, float *us_per_step)
+{
+    return guarded([&] {
+        uint32_t k = 0;
+        int l = 0;
+        engine(h).autotune(trial_steps, &k, &l, us_per_step);
+        if (pass_k)
+            *pass_k = (int32_t)k;
+        if (layout)
+            *layout = l;
+    });
+}
+
 int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses)
 {
     return guarded([&] {

@@ -369,6 +369,25 @@ public:
     // Synapses delivered through the spill accumulators since the engine was
     // made (queues past their capacity; not an error).
     uint64_t spilled();
+    // Step-kernel schedule: synapses per lane in a binning pass (1, 2, 4) and
+    // the queue-count layout (kLayoutAuto picks by row coverage).  Every
+    // schedule gives the same results.
+    void set_schedule(uint32_t pass_k, int layout);
+    uint32_t pass_k() const { return pass_k_; }
+    int count_layout() const { return count_layout_; }
+    uint32_t count_shift() const { return cshift_; }
+    // Times every schedule on about `trial_steps` real steps of all
+    // instances, in turns of kAutotuneWindow steps (each preceded by 2 steps
+    // that apply it, and one untimed turn first), keeps the fastest; us[6] =
+    // mean microseconds per step, [dense, spread] x [K = 1, 2, 4].  Steps run:
+    // autotune_steps(trial_steps).
+    static constexpr int64_t kAutotuneWindow = 4;
+    static int64_t autotune_steps(int64_t trial_steps)
+    {
+        const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
+        return 6 * (2 + rounds * (2 + kAutotuneWindow));
+    }
+    void autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, float *us);
     // Queue capacity per tile: min(synapses targeting it, floor +
     // (synapses >> shift)); shift 0 keeps the exact worst case.
     void set_queue_capacity(uint32_t floor, uint32_t shift);
@@ -413,6 +432,8 @@ private:
     uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;
     uint64_t qtotal_ = 0;                  // queue slots per parity
     uint32_t cshift_ = kCountShiftDense;   // queue count layout (count_slot)
+    int count_layout_ = kLayoutAuto;       // requested layout
+    uint32_t pass_k_ = kPassKDefault;      // synapses per lane in a binning pass
     uint32_t queue_floor_ = 4096, queue_shift_ = 3;   // queue_capacity
     DataAllocator<InstDev> d_insts_;
     DataAllocator<StepCtl> d_ctl_;

@@ -67,6 +67,59 @@ size_t Engine::active() const
     return k;
 }
 
+void Engine::set_schedule(uint32_t pass_k, int layout)
+{
+    if (pass_k != 1 && pass_k != 2 && pass_k != 4)
+        throw Error(kInvalidParameter, "pass size must be 1, 2 or 4 synapses per lane");
+    if (layout != kLayoutAuto && layout != kLayoutDense && layout != kLayoutSpread)
+        throw Error(kInvalidParameter, "unknown queue-count layout");
+    if (pass_k != pass_k_ || layout != count_layout_) {
+        pass_k_ = pass_k;
+        count_layout_ = layout;
+        dirty_ = true;   // graphs hold the old kernel; counts move to the new layout
+    }
+}
+
+void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, float *us)
+{
+    if (trial_steps < 1)
+        throw Error(kInvalidParameter, "autotune needs at least one step per trial");
+    static const uint32_t ks[3] = {1, 2, 4};
+    static const int layouts[2] = {kLayoutDense, kLayoutSpread};
+    // The cost of a step follows the network's activity, which drifts and
+    // bursts; so the schedules take turns in short windows (kAutotuneWindow
+    // timed steps, after 2 steps that apply the schedule) over the whole
+    // tuning period, and each is judged by its total.  A first untimed turn
+    // loads every kernel variant.
+    const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
+    double t_sum[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t r = -1; r < rounds; ++r) {
+        for (int s = 0; s < 6; ++s) {
+            set_schedule(ks[s % 3], layouts[s / 3]);
+            run(2, 0);   // applies the schedule (rebuild) outside the timed span
+            if (r < 0)
+                continue;
+            float ms = 0.0f;
+            run_span(kAutotuneWindow, 0, &ms);
+            t_sum[s] += ms;
+        }
+    }
+    int bi = 0;
+    for (int i = 1; i < 6; ++i)
+        if (t_sum[i] < t_sum[bi])
+            bi = i;
+    if (us)
+        for (int i = 0; i < 6; ++i)
+            us[i] = (float)(t_sum[i] * 1e3 / (double)(rounds * kAutotuneWindow));
+    const uint32_t bk = ks[bi % 3];
+    const int bl = layouts[bi / 3];
+    set_schedule(bk, bl);
+    if (best_k)
+        *best_k = bk;
+    if (best_layout)
+        *best_layout = bl;
+}
+
 uint64_t Engine::spilled()
 {
     StepCtl c{};
@@ -169,7 +222,12 @@ void Engine::rebuild()
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
     // Rows that reach more than a tenth of their instance's tiles make many
     // waves reserve in the same counts at once: give each count its own line.
-    cshift_ = (total > 0 && coverage / (double)total > 0.1) ? kCountShiftSpread : kCountShiftDense;
+    if (count_layout_ == kLayoutDense)
+        cshift_ = kCountShiftDense;
+    else if (count_layout_ == kLayoutSpread)
+        cshift_ = kCountShiftSpread;
+    else
+        cshift_ = (total > 0 && coverage / (double)total > 0.1) ? kCountShiftSpread : kCountShiftDense;
     d_qcnt_.resize(count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
     // The queues only grow: when instances come and go (serving) the buffer
     // is reused instead of being freed, reallocated and zero-filled.
@@ -193,7 +251,7 @@ void Engine::rebuild()
 void Engine::launch_step_k()
 {
     DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
-                               d_ctl_.mutable_gpu_data(), stream_));
+                               d_ctl_.mutable_gpu_data(), pass_k_, stream_));
 }
 
 void Engine::launch_aux()
@@ -324,7 +382,7 @@ void Engine::run_timed(int64_t steps, float *ms_step)
         for (int64_t k = 0; k < m; ++k) {
             hipEvent_t *e = &ev[(size_t)(2 * k)];
             DTB_HIP(launch_step_kernel_timed(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
-                                             d_ctl_.mutable_gpu_data(), e[0], e[1], stream_));
+                                             d_ctl_.mutable_gpu_data(), pass_k_, e[0], e[1], stream_));
             par_ ^= 1u;
             launch_aux();
         }

@@ -208,26 +208,23 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
-// Synapses per lane and per pass of bin_pass (one pass = one wave).
-constexpr uint32_t kPassK = 2;
-constexpr uint32_t kPass = kWave * kPassK;
-
-// Queues synapses c0 .. c0 + kPass - 1 of one fired row (one pass) for the next
+// Queues synapses c0 .. c0 + 64 K - 1 of one fired row (one pass) for the next
 // launch: each synapse's target index within its tile, channel pair and
 // weight pair go into the target tile's queue.  The row's synapses are
 // ordered by target, so the synapses of one tile form segments; each segment
 // reserves its slots with one atomic add on the tile's count and its
 // synapses are stored contiguously.  Called by a whole wave with
-// wave-uniform arguments.  Lane l holds synapses c0 + 2l and c0 + 2l + 1 of
-// the row (segments end at pass boundaries).  A synapse's segment start is
+// wave-uniform arguments.  Lane l holds synapses c0 + K l .. c0 + K l + K - 1
+// of the row (segments end at pass boundaries).  A synapse's segment start is
 // the last start at or before it (within the lane, else from a wave max-scan
 // of the lanes' last starts); the lane holding the segment's last synapse
 // reserves and publishes the destination and the number of the segment's
-// synapses that fit the queue in seg_dst / seg_room (this wave's kPass LDS
+// synapses that fit the queue in seg_dst / seg_room (this wave's 64 K LDS
 // entries, indexed by segment start within the pass).  Synapses past the
 // queue's capacity add their weights into the target's spill accumulator
 // instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
 // does not depend on which path a synapse took).
+template <uint32_t K>
 __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
@@ -235,7 +232,6 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
                                         StepCtl *__restrict__ ctl, unsigned long long *seg_dst, uint32_t *seg_room,
                                         uint32_t lane)
 {
-    constexpr uint32_t K = kPassK;
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const auto col = gp(col_all) + row_begin;
     const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
@@ -333,8 +329,9 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
 
 // One tile's step (the body of k_step).  kUniform: every neuron of the tile
 // has parameter class d.pcls - 1, so the parameter records are read once per
-// tile through the scalar unit instead of per lane.
-template <bool kUniform>
+// tile through the scalar unit instead of per lane.  K: synapses per lane in
+// a binning pass (a pass is 64 K synapses).
+template <bool kUniform, uint32_t K>
 __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__restrict__ tiles,
                                           const InstDev *__restrict__ insts, const QueueSet &qs, uint32_t par,
                                           StepCtl *__restrict__ ctl, unsigned long long *acc,
@@ -481,7 +478,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     }
 
     // Spike propagation (presynaptic_voltage.cpp:122-225): the tile's fired
-    // rows are listed in LDS, and their passes of kPass synapses are dealt
+    // rows are listed in LDS, and their passes of 64 K synapses are dealt
     // round-robin to the four waves, so one long row does not keep a single
     // wave busy while the others idle.
     const unsigned long long m = __ballot(emit);
@@ -532,34 +529,38 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     for (uint32_t r = 0; r < nr; ++r) {
         const long long rb_s = row_rb[r];
         const uint32_t len_s = row_len[r];
-        for (uint32_t c0 = 0; c0 < len_s; c0 += kPass, ++pass)
+        for (uint32_t c0 = 0; c0 < len_s; c0 += kWave * K, ++pass)
             if ((pass & 3u) == w)
-                bin_pass(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
-                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, I.spill + (size_t)(par ^ 1u) * I.n * 4u,
-                         ctl, seg_dst + w * kPass, seg_room + w * kPass, lane);
+                bin_pass<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                            (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, I.spill + (size_t)(par ^ 1u) * I.n * 4u,
+                            ctl, seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
     }
 }
 
 // The step of launch parity `par`: drains queue set par, fills set par ^ 1,
 // reads the step index from step[par] and advances it into step[par ^ 1].
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_step(const TileDesc *__restrict__ tiles,
+// K: synapses per lane in a binning pass (the schedule; every K gives the
+// same results).  Eight waves per SIMD for K = 1, 2; K = 4's larger LDS
+// segment tables allow six.
+template <uint32_t K>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ? 6 : 8))) void k_step(const TileDesc *__restrict__ tiles,
                                                  const InstDev *__restrict__ insts,
                                                  QueueSet qs,
                                                  uint32_t par,
                                                  StepCtl *__restrict__ ctl)
 {
     __shared__ unsigned long long acc[kBlock * 4];
-    __shared__ unsigned long long seg_dst[4 * kPass];    // per wave: one pass
-    __shared__ uint32_t seg_room[4 * kPass];
+    __shared__ unsigned long long seg_dst[4 * kWave * K];   // per wave: one pass
+    __shared__ uint32_t seg_room[4 * kWave * K];
     __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
     __shared__ uint32_t row_len[kBlock];                  // and length
     __shared__ uint32_t nrows;
 
     const TileDesc d = tiles[blockIdx.x];
     if (d.pcls != 0u)
-        step_tile<true>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
+        step_tile<true, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
     else
-        step_tile<false>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
+        step_tile<false, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
 }
 
 // Writes the pending delta (the runs queued in queue set `set` for the next
@@ -718,20 +719,40 @@ __global__ __launch_bounds__(kBlock) void k_normals(uint64_t seed, uint32_t neur
 // ---- launchers ----------------------------------------------------------
 
 hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                              uint32_t par, StepCtl *ctl, hipStream_t stream)
+                              uint32_t par, StepCtl *ctl, uint32_t pass_k, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
+    switch (pass_k) {
+    case 1: hipLaunchKernelGGL(k_step<1>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
+    case 2: hipLaunchKernelGGL(k_step<2>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
+    case 4: hipLaunchKernelGGL(k_step<4>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                                    uint32_t par, StepCtl *ctl, hipEvent_t start, hipEvent_t stop, hipStream_t stream)
+                                    uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
+                                    hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    hipExtLaunchKernelGGL(k_step, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par, ctl);
+    switch (pass_k) {
+    case 1:
+        hipExtLaunchKernelGGL(k_step<1>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
+                              ctl);
+        break;
+    case 2:
+        hipExtLaunchKernelGGL(k_step<2>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
+                              ctl);
+        break;
+    case 4:
+        hipExtLaunchKernelGGL(k_step<4>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
+                              ctl);
+        break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -8,12 +8,13 @@ namespace dtb {
 // One step of every tile, launch parity `par` (queue accumulation + fold +
 // OU + LIF + binning of the fired rows into queue set par ^ 1).
 hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                              uint32_t par, StepCtl *ctl, hipStream_t stream);
+                              uint32_t par, StepCtl *ctl, uint32_t pass_k, hipStream_t stream);
 
 // launch_step_kernel with `start` / `stop` recorded at the kernel's own
 // dispatch boundaries (kernel time without launch gaps).
 hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                                    uint32_t par, StepCtl *ctl, hipEvent_t start, hipEvent_t stop, hipStream_t stream);
+                                    uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
+                                    hipStream_t stream);
 
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,

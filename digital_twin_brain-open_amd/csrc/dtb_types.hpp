@@ -143,6 +143,12 @@ constexpr uint32_t kQueuePrefetch = 128;
 // when every fired row reaches most tiles, many waves reserve in the same
 // counts at once, and counts sharing a line would serialise on it.
 constexpr uint32_t kCountLineWords = 32;
+// Synapses per lane in a binning pass of the step kernel: 1, 2 or 4 (passes
+// of 64, 128 or 256 synapses).  Every value gives the same results.
+constexpr uint32_t kPassKDefault = 2;
+
+// Requested layout of an engine's queue counts.
+enum : int { kLayoutAuto = 0, kLayoutDense = 1, kLayoutSpread = 2 };
 constexpr uint32_t kCountShiftDense = 5;
 constexpr uint32_t kCountShiftSpread = 0;
 

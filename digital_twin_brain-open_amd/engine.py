@@ -298,6 +298,42 @@ class Engine:
         past it go through the exact spill accumulators."""
         _lib.check(self._lib.dtb_engine_queue_capacity(self._h, int(floor), int(shift)))
 
+    LAYOUTS = {"auto": 0, "dense": 1, "spread": 2}
+
+    def schedule(self, pass_k=None, layout=None):
+        """Sets (when given) and returns the step kernel's schedule: synapses
+        per lane in a binning pass (1, 2, 4) and the queue-count layout
+        ("auto", "dense", "spread"); every schedule gives the same results.
+        Returns (pass_k, requested layout, layout in use)."""
+        names = {v: k for k, v in self.LAYOUTS.items()}
+        k, l, u = c_int32(0), c_int32(0), c_int32(0)
+        _lib.check(self._lib.dtb_engine_get_schedule(self._h, byref(k), byref(l), byref(u)))
+        if pass_k is not None or layout is not None:
+            k = c_int32(int(pass_k) if pass_k is not None else k.value)
+            l = c_int32(self.LAYOUTS[layout] if layout is not None else l.value)
+            _lib.check(self._lib.dtb_engine_schedule(self._h, k, l))
+            _lib.check(self._lib.dtb_engine_get_schedule(self._h, byref(k), byref(l), byref(u)))
+        return k.value, names[l.value], names.get(u.value, "dense")
+
+    def autotune(self, trial_steps=50):
+        """Times every schedule on about trial_steps real steps, the schedules
+        taking turns in 4-step windows (autotune_steps(trial_steps) steps in
+        all; they advance the simulation), and keeps the fastest; returns
+        (pass_k, layout, mean us per step as {(layout, pass_k): us})."""
+        k, l = c_int32(0), c_int32(0)
+        us = np.zeros(6, dtype=np.float32)
+        _lib.check(self._lib.dtb_engine_autotune(self._h, int(trial_steps), byref(k), byref(l), _ptr(us)))
+        names = {v: n for n, v in self.LAYOUTS.items()}
+        table = {(lay, kk): float(us[i * 3 + j]) for i, lay in enumerate(("dense", "spread"))
+                 for j, kk in enumerate((1, 2, 4))}
+        return k.value, names[l.value], table
+
+    @staticmethod
+    def autotune_steps(trial_steps):
+        """Steps that autotune(trial_steps) runs."""
+        rounds = (int(trial_steps) + 3) // 4
+        return 6 * (2 + rounds * 6)
+
     def spilled(self):
         """Synapses delivered through the spill accumulators so far."""
         v = c_uint64(0)

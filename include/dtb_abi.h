@@ -201,6 +201,24 @@ int dtb_engine_check(dtb_engine_t h);
  * with the same exact integer arithmetic, so results do not depend on the
  * capacity; dtb_engine_spilled counts them.  Takes effect at the next run. */
 int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
+
+/* No counterpart in the reference (it has one schedule).  The step kernel's
+ * schedule: pass_k synapses per lane in a binning pass (1, 2 or 4: passes of
+ * 64, 128 or 256 synapses dealt to a tile's waves) and the layout of the
+ * queue counts (DTB_LAYOUT_AUTO: by row coverage; DENSE: 32 per cache line;
+ * SPREAD: one per line).  Every schedule gives bit-identical results. */
+enum { DTB_LAYOUT_AUTO = 0, DTB_LAYOUT_DENSE = 1, DTB_LAYOUT_SPREAD = 2 };
+int dtb_engine_schedule(dtb_engine_t h, int32_t pass_k, int32_t layout);
+int dtb_engine_get_schedule(dtb_engine_t h, int32_t *pass_k, int32_t *layout, int32_t *layout_in_use);
+/* Autotuner: every schedule (2 dense/spread x 3 pass sizes) runs about
+ * trial_steps timed real steps of all instances, the schedules taking turns
+ * in windows of 4 steps (each turn preceded by 2 steps that apply it, and one
+ * untimed turn of each first), timed with HIP events; the fastest total is
+ * kept.  Steps run in all: 6 (2 + r (2 + 4)), r = ceil(trial_steps / 4).
+ * us_per_step[6] (optional) = mean per schedule, [dense K=1, 2, 4, spread
+ * K=1, 2, 4].  The steps advance the simulation exactly as any other steps
+ * would; tune in the regime that will be run (after the initial transient). */
+int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, int32_t *layout, float *us_per_step);
 int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses);
 
 /* Checkpoint: reference-visible state after the last step (v_membranes_,

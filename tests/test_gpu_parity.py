@@ -550,3 +550,85 @@ def test_gamma_resampling_matches_oracle(dtb, orc, gpu):
     ref = sim.run(steps, pop_counts=True)
     assert np.array_equal(out["freq"], ref["freq"])
     assert_state_equal(st, sim)
+
+
+def test_population_rates_over_10s_within_1pct_of_fp64(dtb, orc, gpu):
+    """10 s of model time (100,000 steps, about 17 Hz): per-population firing
+    rates of the GPU path within 1 % (rtol = 0.01) of the free-running fp64
+    oracle with the same noise stream."""
+    n, steps, seed, mu = 2000, 100_000, 5, 2.0
+    net = dtb.gen_uniform(n, 100, seed=1)
+    blk = dtb.Block(net, delta_t=DT, seed=seed)
+    try:
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, mu, 0.1)
+        out = blk.run(steps, freqs=True, graph=True)
+        begin = blk.engine.info(blk.inst)["pop_begin"].astype(np.int64)
+    finally:
+        blk.engine.close()
+    ref, _ = run_oracle(orc, net, steps, seed, mu, precision=64)
+    sizes = np.diff(begin)
+    gpu_rate = out["freq"].astype(np.int64).sum(axis=0) / (sizes * steps * DT * 1e-3)
+    r = ref["raster"]
+    ref_rate = np.array([r[:, a:b].sum() for a, b in zip(begin[:-1], begin[1:])]) / (sizes * steps * DT * 1e-3)
+    print("rates Hz gpu", gpu_rate, "fp64", ref_rate)
+    assert (ref_rate > 5).all()
+    np.testing.assert_allclose(gpu_rate, ref_rate, rtol=0.01)
+
+
+@pytest.mark.parametrize("pass_k", [1, 2, 4])
+@pytest.mark.parametrize("layout", ["dense", "spread"])
+def test_every_schedule_bit_exact(dtb, orc, gpu, pass_k, layout):
+    """Every step-kernel schedule (binning pass size x queue-count layout)
+    gives the oracle's raster and state, also when the schedule changes
+    between runs with spikes pending."""
+    n, steps, seed = 10000, 200, 20240
+    net = dtb.gen_uniform(n, 100, seed=1)
+    ids = np.arange(0, n, 5, dtype=np.uint32)
+    eng = dtb.Engine(0)
+    try:
+        eng.schedule(pass_k, layout)
+        blk = dtb.Block(net, delta_t=DT, seed=seed, engine=eng)
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 1.5, 0.1)
+        blk.set_samples(ids)
+        a = blk.run(steps, output_sample_spike=True)
+        assert eng.schedule()[0] == pass_k and eng.schedule()[2] == layout
+        eng.schedule(2 if pass_k != 2 else 1, "spread" if layout == "dense" else "dense")
+        b = blk.run(steps, output_sample_spike=True, graph=True)
+        st = blk.state()
+    finally:
+        eng.close()
+    ref, sim = run_oracle(orc, net, 2 * steps, seed, 1.5)
+    assert np.array_equal(np.concatenate([a["spike"], b["spike"]]), ref["raster"][:, ids])
+    assert np.array_equal(np.concatenate([a["freq"], b["freq"]]), ref["freq"])
+    assert_state_equal(st, sim)
+
+
+def test_autotune_changes_nothing_but_the_schedule(dtb, gpu):
+    """The autotuner's trial steps are ordinary steps: two instances stepped
+    through autotuning end bit-identical to the same number of steps run with
+    the default schedule."""
+    nets = [dtb.gen_uniform(6000, 150, seed=4), dtb.gen_voxel(20000, 10, 200, seed=6)]
+    states = []
+    for tune in (False, True):
+        eng = dtb.Engine(0)
+        try:
+            blks = [dtb.Block(net, delta_t=DT, seed=60 + i, engine=eng) for i, net in enumerate(nets)]
+            for b in blks:
+                b.update_ou_background_stimuli(b.subblk_id, 4.0, 2.0, 0.1)
+                eng.configure(b.inst, 1, True)
+            if tune:
+                k, layout, table = eng.autotune(8)
+                print("autotune", k, layout, table)
+                assert k in (1, 2, 4) and layout in ("dense", "spread") and len(table) == 6
+                assert eng.schedule()[:2] == (k, layout)
+                eng.run(100, graph_chunk=0)
+            else:
+                eng.run(dtb.Engine.autotune_steps(8) + 100, graph_chunk=0)
+            eng.sync()
+            states.append([(b.state(), b.counters()) for b in blks])
+        finally:
+            eng.close()
+    for (s0, c0), (s1, c1) in zip(*states):
+        assert c0 == c1
+        for key in ("V", "J", "i_ou", "t", "f", "I_syn"):
+            assert np.array_equal(np.asarray(s0[key]).view(np.uint8), np.asarray(s1[key]).view(np.uint8)), key
