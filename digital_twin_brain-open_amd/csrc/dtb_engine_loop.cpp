@@ -222,12 +222,17 @@ void Engine::rebuild()
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
     // Rows that reach more than a tenth of their instance's tiles make many
     // waves reserve in the same counts at once: give each count its own line.
+    // Auto: dense, unless rows reach most of their instance's tiles (the
+    // reservations of many waves then meet on the same lines).  Measured with
+    // the tile's passes dealt to all its waves, dense was as fast or faster
+    // up to a coverage of 0.92 (in-degree 5000 on 500k neurons), so the
+    // threshold sits above that; the autotuner measures instead of guessing.
     if (count_layout_ == kLayoutDense)
         cshift_ = kCountShiftDense;
     else if (count_layout_ == kLayoutSpread)
         cshift_ = kCountShiftSpread;
     else
-        cshift_ = (total > 0 && coverage / (double)total > 0.1) ? kCountShiftSpread : kCountShiftDense;
+        cshift_ = (total > 0 && coverage / (double)total > kSpreadCoverage) ? kCountShiftSpread : kCountShiftDense;
     d_qcnt_.resize(count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
     // The queues only grow: when instances come and go (serving) the buffer
     // is reused instead of being freed, reallocated and zero-filled.

@@ -147,6 +147,10 @@ constexpr uint32_t kCountLineWords = 32;
 // of 64, 128 or 256 synapses).  Every value gives the same results.
 constexpr uint32_t kPassKDefault = 2;
 
+// Row coverage (tile segments per row / tiles) above which the automatic
+// layout spreads the counts.
+constexpr double kSpreadCoverage = 0.95;
+
 // Requested layout of an engine's queue counts.
 enum : int { kLayoutAuto = 0, kLayoutDense = 1, kLayoutSpread = 2 };
 constexpr uint32_t kCountShiftDense = 5;
