@@ -174,7 +174,7 @@ def serving_bench(dtb, nreq, rate, plain):
     reqs = serving.poisson_requests(nreq, rate, SERVE_SEED)
     srv = serving.Server(device=0)
     try:
-        stats = srv.serve(reqs, realtime=True)
+        stats = srv.serve(reqs, realtime=True, progress=10.0)
     finally:
         srv.close()
     t_plain, ev_plain, ns_plain, same = 0.0, 0, 0, True

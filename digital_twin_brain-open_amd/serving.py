@@ -15,6 +15,7 @@ result is the same alone and in the mix.
 """
 import collections
 import ctypes
+import sys
 import time
 
 import numpy as np
@@ -112,11 +113,12 @@ class Server:
         r.v_mean = float(V.astype(np.float64).mean())
         eng.remove(b.inst)
 
-    def serve(self, requests, realtime=True, max_resident=None):
+    def serve(self, requests, realtime=True, max_resident=None, progress=None):
         """Runs the stream to completion; arrival times are honoured on the
         wall clock when realtime (otherwise every request has arrived at
         start).  Returns summary statistics; per-request results and times
-        are filled into the Request objects."""
+        are filled into the Request objects.  progress: seconds between
+        one-line reports on stderr (None: silent)."""
         for r in requests:
             if r.device_bytes() > self.budget:
                 raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.device_bytes()))
@@ -128,7 +130,13 @@ class Server:
         steps_run = 0
         resident_steps = 0
         t_admit = t_run = t_retire = 0.0
+        done_count = 0
+        t_report = time.monotonic()
         while pending or active:
+            if progress is not None and time.monotonic() - t_report >= progress:
+                t_report = time.monotonic()
+                print("serving: %.0f s, %d done, %d resident, %d waiting" % (t_report - t0, done_count, len(active),
+                                                                            len(pending)), file=sys.stderr, flush=True)
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
             while (pending and pending[0].arrival - offset <= now and used + pending[0].device_bytes() <= self.budget
                    and (max_resident is None or len(active) < max_resident)):
@@ -163,6 +171,7 @@ class Server:
                 t_retire += time.monotonic() - ta
                 r.finished = time.monotonic() - t0 + offset
                 used -= r.device_bytes()
+                done_count += 1
         wall = time.monotonic() - t0
         lat = np.array([r.latency() for r in requests])
         events = sum(r.events for r in requests)
