@@ -41,6 +41,9 @@ INVALID_PARAMETER = 1
 UNSET_SAMPLE_FAILURE = 2
 UNSUPPORTED = 3
 INVALID_NETWORK = 5
+HIP_ERROR = 100
+OUT_OF_MEMORY = 101
+NO_DEVICE = 102
 
 
 class DtbError(RuntimeError):

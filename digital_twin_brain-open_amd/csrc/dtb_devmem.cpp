@@ -2,17 +2,17 @@
 //
 // hipMalloc and hipFree cost time that grows with the block size (page
 // mapping: seconds for tens of GB), which dominates when instances of that
-// size come and go every few hundred milliseconds (serving).  So freed memory
-// stays with the library:
-//   * blocks of 1 MB and more are ranges of segments (device allocations),
-//     taken best-fit and merged with free neighbours when released; a
-//     segment is either reserved up front (reserve) or made for a request
-//     that no free range holds;
-//   * smaller blocks are kept by size and handed out again.
+// size come and go every few hundred milliseconds (serving).  So device
+// memory stays with the library: every block is a range of a segment (a
+// device allocation), taken best-fit in 4 KB units and merged with free
+// neighbours when released.  A segment is either reserved up front
+// (reserve: a server's whole budget in one allocation) or made when no free
+// range holds a request (at least kMinSegment, so small buffers share one;
+// with a reservation only small blocks may go outside it).
 // A released block may still be read by work in flight, so it is parked
 // first; parked blocks become reusable after one device synchronisation,
-// taken only when a request finds no free memory.  When the device runs out
-// of memory, segments with nothing allocated and cached small blocks are
+// taken only when a request finds no free range.  When the device runs out
+// of memory, segments with nothing allocated that were not reserved are
 // returned to the runtime and the allocation is retried.
 #include <map>
 #include <memory>
@@ -26,9 +26,9 @@ namespace dtb {
 namespace devmem {
 namespace {
 
-constexpr size_t kLarge = (size_t)1 << 20;
-constexpr size_t kGrain = (size_t)2 << 20;    // large blocks: multiples of 2 MB
-constexpr size_t kSmallGrain = 512;
+constexpr size_t kGrain = (size_t)4 << 10;         // allocation unit
+constexpr size_t kSegGrain = (size_t)2 << 20;      // segment sizes: multiples of 2 MB
+constexpr size_t kMinSegment = (size_t)256 << 20;  // segments made on demand
 
 struct Segment {
     char *base = nullptr;
@@ -39,13 +39,12 @@ struct Segment {
 struct Block {
     int device;
     size_t bytes;
-    int seg;           // segment index, -1 for a small block
+    int seg;
     size_t off;
 };
 
 struct DeviceCache {
     std::vector<std::unique_ptr<Segment>> segs;   // null once returned
-    std::multimap<size_t, void *> small;           // reusable small blocks by size
     std::vector<std::pair<void *, Block>> parked;  // released, maybe still in use
     size_t cached = 0;                              // bytes free or parked
 };
@@ -63,14 +62,12 @@ State &state()
     return *s;
 }
 
-size_t round_size(size_t bytes)
+size_t round_up(size_t bytes, size_t grain)
 {
-    if (bytes < kLarge)
-        return (bytes + kSmallGrain - 1) / kSmallGrain * kSmallGrain;
-    return (bytes + kGrain - 1) / kGrain * kGrain;
+    return (bytes + grain - 1) / grain * grain;
 }
 
-void *take_large(State &st, DeviceCache &c, int device, size_t want)
+void *take(State &st, DeviceCache &c, int device, size_t want)
 {
     int pick = -1;
     size_t best = 0;
@@ -93,43 +90,21 @@ void *take_large(State &st, DeviceCache &c, int device, size_t want)
     return p;
 }
 
-void *take_small(State &st, DeviceCache &c, int device, size_t want)
-{
-    auto it = c.small.find(want);
-    if (it == c.small.end())
-        return nullptr;
-    void *p = it->second;
-    c.small.erase(it);
-    st.live[p] = Block{device, want, -1, 0};
-    c.cached -= want;
-    return p;
-}
-
 void unpark(DeviceCache &c)
 {
-    for (auto &pb : c.parked) {
-        const Block &b = pb.second;
-        if (b.seg < 0)
-            c.small.emplace(b.bytes, pb.first);
-        else
-            c.segs[(size_t)b.seg]->ranges.give(b.off, b.bytes);
-    }
+    for (auto &pb : c.parked)
+        c.segs[(size_t)pb.second.seg]->ranges.give(pb.second.off, pb.second.bytes);
     c.parked.clear();
 }
 
-// Returns unused memory to the runtime (all of it with `reserved`, else the
-// segments that were not reserved).  The device is current.
+// Returns segments with nothing allocated to the runtime (all of them with
+// `reserved`, else those made on demand).  The device is current.
 void drop_unused(DeviceCache &c, bool reserved)
 {
     if (!c.parked.empty()) {
         (void)hipDeviceSynchronize();
         unpark(c);
     }
-    for (auto &kv : c.small) {
-        (void)hipFree(kv.second);
-        c.cached -= kv.first;
-    }
-    c.small.clear();
     for (auto &s : c.segs) {
         if (s && s->ranges.all_free() && (reserved || !s->reserved)) {
             (void)hipFree(s->base);
@@ -139,7 +114,7 @@ void drop_unused(DeviceCache &c, bool reserved)
     }
 }
 
-void *new_segment(DeviceCache &c, size_t bytes, bool reserved)
+void new_segment(DeviceCache &c, size_t bytes, bool reserved)
 {
     void *p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
@@ -155,7 +130,6 @@ void *new_segment(DeviceCache &c, size_t bytes, bool reserved)
     s->reserved = reserved;
     c.segs.push_back(std::move(s));
     c.cached += bytes;
-    return p;
 }
 
 }  // namespace
@@ -164,33 +138,28 @@ void *alloc(size_t bytes)
 {
     int device = 0;
     DTB_HIP(hipGetDevice(&device));
-    const size_t want = round_size(bytes);
-    const bool large = want >= kLarge;
+    const size_t want = round_up(std::max<size_t>(bytes, 1), kGrain);
     State &st = state();
     std::lock_guard<std::mutex> lock(st.mu);
     DeviceCache &c = st.dev[device];
-    auto take = [&]() { return large ? take_large(st, c, device, want) : take_small(st, c, device, want); };
-    void *p = take();
+    void *p = take(st, c, device, want);
     if (p == nullptr && !c.parked.empty()) {
         DTB_HIP(hipDeviceSynchronize());
         unpark(c);
-        p = take();
+        p = take(st, c, device, want);
     }
     if (p != nullptr)
         return p;
-    if (large) {
-        new_segment(c, want, false);
-        return take_large(st, c, device, want);
-    }
-    hipError_t e = hipMalloc(&p, want);
-    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
-        (void)hipGetLastError();
-        drop_unused(c, false);
-        e = hipMalloc(&p, want);
-    }
-    DTB_HIP(e);
-    st.live[p] = Block{device, want, -1, 0};
-    return p;
+    // With a reservation, large blocks come from it or not at all: the memory
+    // outside it stays with the runtime (its own buffers, launches) and
+    // other users; the caller (a server) waits for ranges to free up.
+    bool reserved = false;
+    for (auto &s : c.segs)
+        reserved = reserved || (s && s->reserved);
+    if (reserved && want > kMinSegment / 4)
+        throw Error(kOutOfMemory, "no free range of the reserved device memory holds the block");
+    new_segment(c, round_up(std::max(want, kMinSegment), kSegGrain), false);
+    return take(st, c, device, want);
 }
 
 void release(void *p)
@@ -219,9 +188,8 @@ void reserve(size_t bytes)
     for (auto &s : c.segs)
         if (s && s->reserved)
             have += s->ranges.size();
-    const size_t want = round_size(std::max(bytes, kLarge));
-    if (have < want)
-        new_segment(c, want - have < kLarge ? kLarge : round_size(want - have), true);
+    if (have < bytes)
+        new_segment(c, round_up(bytes - have, kSegGrain), true);
 }
 
 void trim()

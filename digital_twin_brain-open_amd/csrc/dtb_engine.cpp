@@ -767,6 +767,16 @@ void Instance::write_state(const float *V, const int32_t *t, const uint8_t *f, c
     DTB_HIP(hipStreamSynchronize(s));
 }
 
+void Instance::ensure_queue(uint64_t slots)
+{
+    slots = std::max<uint64_t>(slots, 1);
+    if (slots > qslots_ || queue_.count() == 0) {
+        queue_.resize((size_t)(2 * slots + kQueuePrefetch));
+        queue_.mutable_gpu_data();
+        qslots_ = slots;
+    }
+}
+
 void Instance::clear_spill(hipStream_t s)
 {
     DTB_HIP(hipMemsetAsync(spill_.mutable_gpu_data(), 0, spill_.size(), s));
@@ -790,6 +800,8 @@ InstDev Instance::dev(uint32_t tile_base)
     d.J = J_.mutable_gpu_data();
     d.dlt = dlt_.mutable_gpu_data();
     d.spill = spill_.mutable_gpu_data();
+    d.queue = queue_.mutable_gpu_data();
+    d.qslots = qslots_;
     d.rowptr = rowptr_.gpu_data();
     d.col = col_.gpu_data();
     d.w = w_.gpu_data();

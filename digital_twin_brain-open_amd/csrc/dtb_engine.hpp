@@ -291,6 +291,10 @@ public:
     // Zeroes both spill accumulators (the queues of this instance were
     // cleared).
     void clear_spill(hipStream_t s);
+    // Propagation queues of `slots` entries per parity (grow-only; the
+    // contents need no initialisation: only the first `count` entries of a
+    // tile's queue are ever read).
+    void ensure_queue(uint64_t slots);
     uint32_t tile_class(uint32_t tile) const { return tile_cls_.at(tile); }
     // Mean over rows of the fraction of this instance's tiles that the row's
     // synapses reach (tile segments of the row / tiles).
@@ -326,6 +330,8 @@ private:
     DataAllocator<int32_t> tcnt_;
     DataAllocator<float4> dlt_;
     DataAllocator<unsigned long long> spill_;      // [2][n][4] Q23.40
+    DataAllocator<QSyn> queue_;                    // [2][qslots_] + kQueuePrefetch
+    uint64_t qslots_ = 0;
     DataAllocator<int64_t> rowptr_;
     DataAllocator<uint32_t> col_;
     DataAllocator<float2> w_;
@@ -392,6 +398,8 @@ public:
     // (synapses >> shift)); shift 0 keeps the exact worst case.
     void set_queue_capacity(uint32_t floor, uint32_t shift);
     uint32_t queue_capacity(uint32_t synapses) const;
+    // Queue slots per parity of instance p at the current capacity.
+    uint64_t queue_slots(const Instance &p) const;
     void sync();
     void mark_dirty() { dirty_ = true; }
     // Writes the pending delta of every instance into its dlt buffer (the
@@ -430,7 +438,6 @@ private:
     bool pending_ = false;                 // queues may hold an unfolded delta
     uint32_t par_ = 0;                     // parity of the next step launch (= pending queue set)
     uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;
-    uint64_t qtotal_ = 0;                  // queue slots per parity
     uint32_t cshift_ = kCountShiftDense;   // queue count layout (count_slot)
     int count_layout_ = kLayoutAuto;       // requested layout
     uint32_t pass_k_ = kPassKDefault;      // synapses per lane in a binning pass
@@ -441,7 +448,6 @@ private:
     DataAllocator<SampleRecDev> d_recs_;
     DataAllocator<TileDesc> d_tiles_;
     DataAllocator<uint32_t> d_qcnt_;       // count_words(2 * ntiles)
-    DataAllocator<QSyn> d_queue_;          // [2][qtotal]
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
     int graph_steps_[2] = {0, 0};
 };

@@ -31,6 +31,10 @@ Engine::~Engine()
 
 int Engine::add(std::unique_ptr<Instance> inst)
 {
+    // The instance's queues are allocated now, so that running out of device
+    // memory shows up when it is added, not in the middle of a later run.
+    DTB_HIP(hipSetDevice(device_));
+    inst->ensure_queue(queue_slots(*inst));
     for (size_t i = 0; i < insts_.size(); ++i)
         if (!insts_[i]) {
             insts_[i] = std::move(inst);
@@ -128,6 +132,14 @@ uint64_t Engine::spilled()
     return c.spilled;
 }
 
+uint64_t Engine::queue_slots(const Instance &p) const
+{
+    uint64_t slots = 0;
+    for (uint32_t b = 0, t = 0; b < p.n(); b += kBlock, ++t)
+        slots += queue_capacity(p.tile_synapses(t));
+    return slots;
+}
+
 void Engine::set_queue_capacity(uint32_t floor, uint32_t shift)
 {
     if (shift > 31)
@@ -159,9 +171,7 @@ void Engine::drop_graphs()
 QueueSet Engine::queues()
 {
     QueueSet q{};
-    q.items = d_queue_.mutable_gpu_data();
     q.count = d_qcnt_.mutable_gpu_data();
-    q.qtotal = qtotal_;
     q.ntiles = ntiles_;
     q.cshift = cshift_;
     return q;
@@ -176,7 +186,7 @@ void Engine::rebuild()
     std::vector<SampleDev> samples;
     std::vector<SampleRecDev> recs;
     tile_base_.assign(insts_.size(), 0u);
-    uint64_t total = 0, qtotal = 0;
+    uint64_t total = 0;
     double coverage = 0.0;
     for (size_t slot = 0; slot < insts_.size(); ++slot) {
         Instance *p = insts_[slot].get();
@@ -184,8 +194,7 @@ void Engine::rebuild()
             continue;
         const uint32_t idx = (uint32_t)table.size();
         tile_base_[slot] = (uint32_t)tiles.size();
-        table.push_back(p->dev((uint32_t)tiles.size()));
-        recs.push_back(p->sample_rec());
+        uint64_t qtotal = 0;   // this instance's queue slots per parity
         const auto &pb = p->pop_begin();
         uint32_t q = 0;
         for (uint32_t b = 0, t = 0; b < p->n(); b += kBlock, ++t) {
@@ -202,6 +211,12 @@ void Engine::rebuild()
             qtotal += d.qcap;
             tiles.push_back(d);
         }
+        // Each instance owns its queues (both parities and the read-ahead
+        // slack), so instances of any size come and go without one large
+        // engine-wide block.
+        p->ensure_queue(qtotal);
+        table.push_back(p->dev(tile_base_[slot]));
+        recs.push_back(p->sample_rec());
         if (recs.back().nsample > 0) {
             const auto &ids = p->samples();
             for (uint32_t k = 0; k < ids.size(); ++k)
@@ -215,13 +230,10 @@ void Engine::rebuild()
     ninst_ = (uint32_t)table.size();
     ntiles_ = (uint32_t)tiles.size();
     nsamples_ = (uint32_t)samples.size();
-    qtotal_ = std::max<uint64_t>(qtotal, 1);
     d_insts_.resize(std::max<size_t>(table.size(), 1));
     d_tiles_.resize(std::max<size_t>(tiles.size(), 1));
     d_samples_.resize(std::max<size_t>(samples.size(), 1));
     d_recs_.resize(std::max<size_t>(recs.size(), 1));
-    // Rows that reach more than a tenth of their instance's tiles make many
-    // waves reserve in the same counts at once: give each count its own line.
     // Auto: dense, unless rows reach most of their instance's tiles (the
     // reservations of many waves then meet on the same lines).  Measured with
     // the tile's passes dealt to all its waves, dense was as fast or faster
@@ -234,14 +246,9 @@ void Engine::rebuild()
     else
         cshift_ = (total > 0 && coverage / (double)total > kSpreadCoverage) ? kCountShiftSpread : kCountShiftDense;
     d_qcnt_.resize(count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
-    // The queues only grow: when instances come and go (serving) the buffer
-    // is reused instead of being freed, reallocated and zero-filled.
-    if (d_queue_.count() < (size_t)(2 * qtotal_ + kQueuePrefetch))
-        d_queue_.resize((size_t)(2 * qtotal_ + kQueuePrefetch));
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
     d_qcnt_.mutable_gpu_data();
-    d_queue_.mutable_gpu_data();
     d_samples_.mutable_gpu_data();
     d_recs_.mutable_gpu_data();
     DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data(), 0, d_qcnt_.size(), stream_));

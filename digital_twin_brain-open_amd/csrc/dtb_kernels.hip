@@ -342,9 +342,9 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
-    const QSyn *__restrict__ q_in = qs.items + (size_t)par * qs.qtotal;
+    const QSyn *__restrict__ q_in = I.queue + (size_t)par * I.qslots;
     const size_t cslot_in = count_slot((size_t)par * qs.ntiles + blockIdx.x, qs.cshift);
-    QSyn *__restrict__ q_out = qs.items + (size_t)(par ^ 1u) * qs.qtotal;
+    QSyn *__restrict__ q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
 
     // Loads are issued in order of their dependence chains: the queue count
     // (kernel arguments only), the parameter class, the neuron state, the
@@ -582,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     const uint32_t cnt_raw = qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)];
     const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
-    accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
+    accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
     if (tid < d.count) {
         if (cnt_raw > d.qcap)
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     const uint32_t cnt_raw = *cnt_p;
     const uint32_t cnt = min(cnt_raw, d.qcap);
     __syncthreads();
-    accumulate_tile(acc, qs.items + (size_t)set * qs.qtotal + d.qoff, cnt, QSyn{}, 0u, tid);
+    accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;

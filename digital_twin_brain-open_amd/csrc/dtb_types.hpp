@@ -88,6 +88,8 @@ struct alignas(16) InstDev {
     float4 *J;                  // J_u after the previous step's decay (delta below pending)
     float4 *dlt;                // materialised pending delta (sampling / read-back only)
     unsigned long long *spill;  // [2][n][4] Q23.40 sums of the synapses past a queue's capacity
+    QSyn *queue;                // [2][qslots] propagation queues (+ read-ahead slack)
+    uint64_t qslots;            // queue slots per parity (tile t's start: TileDesc::qoff)
     // connectivity: presynaptic-major CSR, 64-bit row offsets
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
@@ -125,7 +127,7 @@ struct alignas(16) TileDesc {
     uint32_t begin;     // first instance-local neuron
     uint32_t count;     // neurons in the tile (<= 256)
     uint32_t pop0;      // population of the first neuron
-    uint64_t qoff;      // first queue slot (within one parity's queue set)
+    uint64_t qoff;      // first slot in the instance's queue of one parity
     uint32_t qcap;      // queue capacity: synapses targeting this tile (all rows)
     uint32_t pcls;      // 1 + the parameter class shared by every neuron of the tile, or 0
 };
@@ -167,9 +169,7 @@ __host__ __device__ inline size_t count_words(size_t n, uint32_t shift)
 }
 
 struct QueueSet {
-    QSyn *items;        // [2][qtotal]
     uint32_t *count;    // [2][ntiles], tile t of set p at word count_slot(p * ntiles + t, cshift)
-    uint64_t qtotal;
     uint32_t ntiles;
     uint32_t cshift;    // kCountShiftDense or kCountShiftSpread
 };

@@ -20,6 +20,7 @@ import time
 
 import numpy as np
 
+from ._lib import OUT_OF_MEMORY, DtbError
 from .engine import DEFAULT_G_UI, Block, Engine, device_memory_reserve
 
 # Device bytes per neuron of an instance: parameter class, V, i_ou,
@@ -132,17 +133,30 @@ class Server:
         t_admit = t_run = t_retire = 0.0
         done_count = 0
         t_report = time.monotonic()
+        wait_retire = False   # the last admission found no device range: wait for memory to free up
         while pending or active:
             if progress is not None and time.monotonic() - t_report >= progress:
                 t_report = time.monotonic()
                 print("serving: %.0f s, %d done, %d resident, %d waiting" % (t_report - t0, done_count, len(active),
                                                                             len(pending)), file=sys.stderr, flush=True)
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
-            while (pending and pending[0].arrival - offset <= now and used + pending[0].device_bytes() <= self.budget
+            while (pending and not wait_retire and pending[0].arrival - offset <= now
+                   and used + pending[0].device_bytes() <= self.budget
                    and (max_resident is None or len(active) < max_resident)):
                 r = pending.popleft()
                 ta = time.monotonic()
-                b = self._admit(r)
+                try:
+                    b = self._admit(r)
+                except DtbError as e:
+                    # the budget fits but no contiguous device range does
+                    # (fragmentation): wait for retirements, unless nothing
+                    # is resident that could free memory
+                    t_admit += time.monotonic() - ta
+                    if e.status != OUT_OF_MEMORY or not active:
+                        raise
+                    pending.appendleft(r)
+                    wait_retire = True
+                    break
                 t_admit += time.monotonic() - ta
                 r.admitted = time.monotonic() - t0 + offset
                 active[b.inst] = [r, b, r.steps]
@@ -172,6 +186,7 @@ class Server:
                 r.finished = time.monotonic() - t0 + offset
                 used -= r.device_bytes()
                 done_count += 1
+                wait_retire = False
         wall = time.monotonic() - t0
         lat = np.array([r.latency() for r in requests])
         events = sum(r.events for r in requests)
