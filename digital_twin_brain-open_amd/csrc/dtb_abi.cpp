@@ -139,8 +139,7 @@ int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t 
         nh.col = net->colinds;
         nh.w = net->weights;
         nh.kind_bits = net->connkinds;
-        e.sync();
-        auto inst = std::make_unique<dtb::Instance>(nh, dt, seed, e.stream());
+        auto inst = std::make_unique<dtb::Instance>(nh, dt, seed, e.admit_stream());
         *inst_id = e.add(std::move(inst));
     });
 }
@@ -159,8 +158,7 @@ int dtb_instance_add_fanout(dtb_engine_t h, int64_t n, uint32_t k, uint64_t net_
         for (int c = 0; c < 4; ++c)
             f.g_ui[c] = g_ui[c];
         f.sub_base = sub_base;
-        e.sync();
-        auto inst = std::make_unique<dtb::Instance>(f, dt, seed, e.stream());
+        auto inst = std::make_unique<dtb::Instance>(f, dt, seed, e.admit_stream());
         *inst_id = e.add(std::move(inst));
     });
 }
@@ -196,8 +194,8 @@ int dtb_set_ou(dtb_engine_t h, int inst, const uint32_t *brain_ids, const float 
         need(n == 0 || (brain_ids && mean && deviation && correlation_time), "OU arrays are required");
         dtb::Engine &e = engine(h);
         e.get(inst);
-        e.settle();
-        e.get(inst)->set_ou(brain_ids, mean, deviation, correlation_time, (size_t)n, e.stream());
+        e.settle_for(inst);
+        e.get(inst)->set_ou(brain_ids, mean, deviation, correlation_time, (size_t)n, e.stream_for(inst));
         e.mark_dirty();
     });
 }
@@ -219,8 +217,8 @@ int dtb_update_props(dtb_engine_t h, int inst, const uint32_t *neuron_ids, const
         need(n >= 0 && (n == 0 || (neuron_ids && prop_ids && vals)), "property arrays are required");
         dtb::Engine &e = engine(h);
         e.get(inst);
-        e.settle();
-        e.get(inst)->update_props(neuron_ids, prop_ids, vals, (size_t)n, e.stream());
+        e.settle_for(inst);
+        e.get(inst)->update_props(neuron_ids, prop_ids, vals, (size_t)n, e.stream_for(inst));
         e.mark_dirty();   // the parameter tables may have been reallocated
     });
 }
@@ -232,8 +230,8 @@ int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, con
         need(n >= 0 && (n == 0 || (prop_ids && brain_ids && vals)), "property arrays are required");
         dtb::Engine &e = engine(h);
         e.get(inst);
-        e.settle();
-        e.get(inst)->update_prop_cols(prop_ids, brain_ids, vals, (size_t)n, assign != 0, e.stream());
+        e.settle_for(inst);
+        e.get(inst)->update_prop_cols(prop_ids, brain_ids, vals, (size_t)n, assign != 0, e.stream_for(inst));
         e.mark_dirty();   // the parameter tables may have been reallocated
     });
 }
@@ -308,8 +306,8 @@ int dtb_configure(dtb_engine_t h, int inst, int32_t t_steps, int32_t use_ou)
     return guarded([&] {
         dtb::Engine &e = engine(h);
         e.get(inst);
-        e.settle();
-        if (e.get(inst)->configure_run(t_steps, use_ou != 0, e.stream()))
+        e.settle_for(inst);
+        if (e.get(inst)->configure_run(t_steps, use_ou != 0, e.stream_for(inst)))
             e.mark_dirty();
     });
 }
@@ -379,6 +377,16 @@ int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, in
             *pass_k = (int32_t)k;
         if (layout)
             *layout = l;
+    });
+}
+
+int dtb_instance_tile_synapses(dtb_engine_t h, int inst, uint32_t *out)
+{
+    return guarded([&] {
+        need(out != nullptr, "out is NULL");
+        const dtb::Instance *p = engine(h).get(inst);
+        for (uint32_t b = 0, t = 0; b < p->n(); b += dtb::kBlock, ++t)
+            out[t] = p->tile_synapses(t);
     });
 }
 
@@ -478,8 +486,8 @@ int dtb_gamma_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, cons
         need(n >= 0 && (n == 0 || (prop_ids && brain_ids && concentration && rate)), "gamma arrays are required");
         dtb::Engine &e = engine(h);
         e.get(inst);
-        e.settle();
-        e.get(inst)->gamma_prop_cols(prop_ids, brain_ids, concentration, rate, (size_t)n, e.stream());
+        e.settle_for(inst);
+        e.get(inst)->gamma_prop_cols(prop_ids, brain_ids, concentration, rate, (size_t)n, e.stream_for(inst));
         e.mark_dirty();   // the parameter tables may have been reallocated
     });
 }

@@ -7,7 +7,9 @@
 // weight pair (x1 >> 8, x2 >> 8) 2^-24 from Philox4x32-10 with counter
 // (r, k, 1, generator tag) and key = seed.  Row r's synapses are ordered by
 // (t, k); the channel pair is fixed by the row (inhibitory rows r >= 0.8 n).
-// Per-target in-degrees are accumulated for the queue capacities.
+// The synapses per target tile (queue capacities) are counted in LDS per
+// workgroup and added to the global counts once per workgroup when the
+// counts fit beside the sort buffer, else one global atomic per synapse.
 // gen_fanout in dtb_gen.cpp is the same generator on the host.
 #include "dtb_devgen.hpp"
 #include "dtb_rng.hpp"
@@ -38,13 +40,20 @@ __global__ __launch_bounds__(kBlock) void k_fanout_rowptr(uint32_t n, uint32_t K
 
 // One workgroup per row (grid-stride): the row's P keys (P = power of two
 // >= K, padding sorts last) are sorted in LDS by a bitonic network, then the
-// first K are written out.
+// first K are written out.  lds_tiles: count the synapses per target tile in
+// LDS (ntiles counters after the keys) and flush them at the end.
 __global__ __launch_bounds__(kBlock) void k_fanout_rows(uint32_t n, uint32_t K, uint32_t P, uint64_t seed,
                                                         uint32_t n_exc, uint32_t *__restrict__ col,
-                                                        float2 *__restrict__ w, uint32_t *__restrict__ indeg)
+                                                        float2 *__restrict__ w, uint32_t *__restrict__ tile_syn,
+                                                        uint32_t ntiles, uint32_t lds_tiles)
 {
     extern __shared__ unsigned long long keys[];
+    uint32_t *tcount = reinterpret_cast<uint32_t *>(keys + P);
     const uint32_t tid = threadIdx.x;
+    if (lds_tiles) {
+        for (uint32_t i = tid; i < ntiles; i += kBlock)
+            tcount[i] = 0u;
+    }
     for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {
         for (uint32_t i = tid; i < P; i += kBlock)
             keys[i] = (i < K) ? fanout_key(seed, n, r, i) : ~0ull;
@@ -73,9 +82,17 @@ __global__ __launch_bounds__(kBlock) void k_fanout_rows(uint32_t n, uint32_t K, 
             const Philox4 x = philox4x32_10(r, k, 1u, kTagGen, (uint32_t)seed, (uint32_t)(seed >> 32));
             col[base + i] = t | kind;
             w[base + i] = make_float2((float)(x.y >> 8) * 0x1p-24f, (float)(x.z >> 8) * 0x1p-24f);
-            atomicAdd(&indeg[t], 1u);
+            if (lds_tiles)
+                atomicAdd(&tcount[t >> kTileBits], 1u);
+            else
+                atomicAdd(&tile_syn[t >> kTileBits], 1u);
         }
         __syncthreads();   // keys are reused by the next row
+    }
+    if (lds_tiles) {
+        for (uint32_t i = tid; i < ntiles; i += kBlock)
+            if (tcount[i] != 0u)
+                atomicAdd(&tile_syn[i], tcount[i]);
     }
 }
 
@@ -90,7 +107,7 @@ uint32_t fanout_sort_width(uint32_t K)
 }
 
 hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *rowptr, uint32_t *col, float2 *w,
-                             uint32_t *indeg, hipStream_t stream)
+                             uint32_t *tile_syn, hipStream_t stream)
 {
     const uint32_t P = fanout_sort_width(K);
     const uint32_t n_exc = (uint32_t)((uint64_t)n * 8u / 10u);
@@ -98,9 +115,16 @@ hipError_t launch_gen_fanout(uint32_t n, uint32_t K, uint64_t seed, int64_t *row
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;
-    const uint32_t grid = n < 65536u ? n : 65536u;
-    hipLaunchKernelGGL(k_fanout_rows, dim3(grid), dim3(kBlock), sizeof(unsigned long long) * P, stream, n, K, P,
-                       seed, n_exc, col, w, indeg);
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    const size_t lds_keys = sizeof(unsigned long long) * P;
+    const size_t lds_all = lds_keys + sizeof(uint32_t) * ntiles;
+    const uint32_t lds_tiles = lds_all <= ((size_t)64 << 10) ? 1u : 0u;
+    // with LDS counts, fewer workgroups each take many rows, so the flush of
+    // ntiles counters per workgroup stays small against the synapses
+    const uint32_t cap = lds_tiles ? 2048u : 65536u;
+    const uint32_t grid = n < cap ? n : cap;
+    hipLaunchKernelGGL(k_fanout_rows, dim3(grid), dim3(kBlock), lds_tiles ? lds_all : lds_keys, stream, n, K, P,
+                       seed, n_exc, col, w, tile_syn, ntiles, lds_tiles);
     return hipGetLastError();
 }
 

@@ -205,15 +205,12 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
     init_common(props.data(), dt, seed, s);
 
     const uint32_t ntile = (n_ + (1u << kTileBits) - 1) >> kTileBits;
-    DataAllocator<uint32_t> indeg(n_);
+    DataAllocator<uint32_t> tsyn(ntile);
     DTB_HIP(launch_gen_fanout(n_, f.K, f.net_seed, rowptr_.mutable_gpu_data(), col_.mutable_gpu_data(),
-                              w_.mutable_gpu_data(), indeg.mutable_gpu_data(), s));
-    std::vector<uint32_t> in(n_);
-    indeg.download(in.data(), n_, s);
-    DTB_HIP(hipStreamSynchronize(s));
+                              w_.mutable_gpu_data(), tsyn.mutable_gpu_data(), s));
     tile_syn_.assign(ntile, 0u);
-    for (uint32_t i = 0; i < n_; ++i)
-        tile_syn_[i >> kTileBits] += in[i];
+    tsyn.download(tile_syn_.data(), ntile, s);
+    DTB_HIP(hipStreamSynchronize(s));
     // expected distinct tiles among K uniform targets, as a fraction
     row_coverage_ = 1.0 - std::pow(1.0 - 1.0 / ntile, (double)f.K);
 }

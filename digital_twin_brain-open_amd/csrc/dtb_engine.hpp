@@ -17,6 +17,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "dtb_types.hpp"
@@ -402,6 +403,18 @@ public:
     uint64_t queue_slots(const Instance &p) const;
     void sync();
     void mark_dirty() { dirty_ = true; }
+    // Stream for building instances: their set-up runs beside the steps of
+    // the instances already in the tables.
+    hipStream_t admit_stream() const { return admit_stream_; }
+    // Instances added since the last table build do not step yet: changing
+    // them needs no settle, and their uploads go on the admission stream.
+    bool staged(int id) const { return staged_.count(id) != 0; }
+    void settle_for(int id)
+    {
+        if (!staged(id))
+            settle();
+    }
+    hipStream_t stream_for(int id) const { return staged(id) ? admit_stream_ : stream_; }
     // Writes the pending delta of every instance into its dlt buffer (the
     // queues are left for the next step), then waits.
     void materialize();
@@ -435,6 +448,8 @@ private:
     std::vector<std::unique_ptr<Instance>> insts_;
     std::vector<uint32_t> tile_base_;      // per instance slot, first tile in the engine table
     bool dirty_ = true;
+    hipStream_t admit_stream_ = nullptr;
+    std::unordered_set<int> staged_;
     bool pending_ = false;                 // queues may hold an unfolded delta
     uint32_t par_ = 0;                     // parity of the next step launch (= pending queue set)
     uint32_t ninst_ = 0, nsamples_ = 0, ntiles_ = 0;

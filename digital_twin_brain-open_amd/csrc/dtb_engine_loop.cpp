@@ -17,6 +17,7 @@ Engine::Engine(int device) : device_(device)
         throw Error(kNoDevice, "device index out of range");
     DTB_HIP(hipSetDevice(device));
     DTB_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    DTB_HIP(hipStreamCreateWithFlags(&admit_stream_, hipStreamNonBlocking));
     d_ctl_.resize(1);
     d_ctl_.mutable_gpu_data();
 }
@@ -27,6 +28,8 @@ Engine::~Engine()
     insts_.clear();
     if (stream_)
         (void)hipStreamDestroy(stream_);
+    if (admit_stream_)
+        (void)hipStreamDestroy(admit_stream_);
 }
 
 int Engine::add(std::unique_ptr<Instance> inst)
@@ -39,10 +42,12 @@ int Engine::add(std::unique_ptr<Instance> inst)
         if (!insts_[i]) {
             insts_[i] = std::move(inst);
             dirty_ = true;
+            staged_.insert((int)i);
             return (int)i;
         }
     insts_.push_back(std::move(inst));
     dirty_ = true;
+    staged_.insert((int)insts_.size() - 1);
     return (int)insts_.size() - 1;
 }
 
@@ -53,6 +58,7 @@ std::unique_ptr<Instance> Engine::remove(int id)
     settle();
     sync();
     dirty_ = true;
+    staged_.erase(id);
     return std::move(insts_[id]);
 }
 
@@ -181,6 +187,7 @@ void Engine::rebuild()
 {
     settle();   // the queues are reallocated below; nothing pending may be lost
     drop_graphs();
+    staged_.clear();
     std::vector<InstDev> table;
     std::vector<TileDesc> tiles;
     std::vector<SampleDev> samples;

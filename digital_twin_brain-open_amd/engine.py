@@ -334,6 +334,12 @@ class Engine:
         rounds = (int(trial_steps) + 3) // 4
         return 6 * (2 + rounds * 6)
 
+    def tile_synapses(self, inst):
+        """Synapses targeting each tile of 256 neurons of an instance."""
+        out = np.empty((self.info(inst)["n"] + 255) // 256, dtype=np.uint32)
+        _lib.check(self._lib.dtb_instance_tile_synapses(self._h, inst, _ptr(out)))
+        return out
+
     def spilled(self):
         """Synapses delivered through the spill accumulators so far."""
         v = c_uint64(0)

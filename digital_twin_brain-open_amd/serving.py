@@ -134,15 +134,13 @@ class Server:
         done_count = 0
         t_report = time.monotonic()
         wait_retire = False   # the last admission found no device range: wait for memory to free up
-        while pending or active:
-            if progress is not None and time.monotonic() - t_report >= progress:
-                t_report = time.monotonic()
-                print("serving: %.0f s, %d done, %d resident, %d waiting" % (t_report - t0, done_count, len(active),
-                                                                            len(pending)), file=sys.stderr, flush=True)
+
+        def admit_arrived():
+            nonlocal used, t_admit, wait_retire
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
             while (pending and not wait_retire and pending[0].arrival - offset <= now
                    and used + pending[0].device_bytes() <= self.budget
-                   and (max_resident is None or len(active) < max_resident)):
+                   and (max_resident is None or len(active) + len(joining) < max_resident)):
                 r = pending.popleft()
                 ta = time.monotonic()
                 try:
@@ -152,23 +150,41 @@ class Server:
                     # (fragmentation): wait for retirements, unless nothing
                     # is resident that could free memory
                     t_admit += time.monotonic() - ta
-                    if e.status != OUT_OF_MEMORY or not active:
+                    if e.status != OUT_OF_MEMORY or not (active or joining):
                         raise
                     pending.appendleft(r)
                     wait_retire = True
                     break
                 t_admit += time.monotonic() - ta
                 r.admitted = time.monotonic() - t0 + offset
-                active[b.inst] = [r, b, r.steps]
+                joining[b.inst] = [r, b, r.steps]
                 used += r.device_bytes()
+
+        joining = {}   # admitted while a chunk runs: they step from the next chunk on
+        while pending or active or joining:
+            if progress is not None and time.monotonic() - t_report >= progress:
+                t_report = time.monotonic()
+                print("serving: %.0f s, %d done, %d resident, %d waiting" % (t_report - t0, done_count, len(active),
+                                                                            len(pending)), file=sys.stderr, flush=True)
+            active.update(joining)
+            joining = {}
             if not active:
-                wait = pending[0].arrival - offset - (time.monotonic() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
-                continue
+                admit_arrived()
+                active.update(joining)
+                joining = {}
+                if not active:
+                    wait = pending[0].arrival - offset - (time.monotonic() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+                    continue
             s = min(self.chunk, min(v[2] for v in active.values()))
             ta = time.monotonic()
             self.engine.run(s, graph_chunk=0)
+            # Admissions are prepared while the chunk runs (host work, and
+            # device work queued behind it on the engine stream).
+            t_run += time.monotonic() - ta
+            admit_arrived()
+            ta = time.monotonic()
             self.engine.sync()
             t_run += time.monotonic() - ta
             steps_run += s

@@ -220,6 +220,9 @@ int dtb_engine_get_schedule(dtb_engine_t h, int32_t *pass_k, int32_t *layout, in
  * would; tune in the regime that will be run (after the initial transient). */
 int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, int32_t *layout, float *us_per_step);
 int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses);
+/* Synapses targeting each tile of 256 neurons of an instance (the queues'
+ * worst case), out[ceil(n / 256)]. */
+int dtb_instance_tile_synapses(dtb_engine_t h, int inst, uint32_t *out);
 
 /* Checkpoint: reference-visible state after the last step (v_membranes_,
  * t_actives_, f_inner_actives_, emitted flags, i_ou_background_stimuli_,

@@ -400,6 +400,10 @@ def test_device_generated_network_equals_host(dtb, orc, gpu):
     for b in (bd, bh):
         b.update_ou_background_stimuli(b.subblk_id, 4.0, 2.0, 0.1)
         eng.configure(b.inst, 1, True)
+    # queue capacities: the device counts the synapses per target tile
+    expect = np.bincount(host.col.astype(np.int64) >> 8, minlength=(n + 255) // 256)
+    assert np.array_equal(eng.tile_synapses(bd.inst), expect)
+    assert np.array_equal(eng.tile_synapses(bh.inst), expect)
     eng.run(steps)
     eng.check()
     sd, sh = bd.state(), bh.state()
@@ -632,3 +636,19 @@ def test_autotune_changes_nothing_but_the_schedule(dtb, gpu):
         assert c0 == c1
         for key in ("V", "J", "i_ou", "t", "f", "I_syn"):
             assert np.array_equal(np.asarray(s0[key]).view(np.uint8), np.asarray(s1[key]).view(np.uint8)), key
+
+
+@pytest.mark.parametrize("n,k", [(3000, 8000), (2_200_000, 40)])
+def test_device_generator_tile_counts(dtb, gpu, n, k):
+    """The generator's per-tile synapse counts on both of its paths: one
+    global atomic per synapse when the tile counts do not fit beside a long
+    row's sort buffer (out-degree 8000: 64 KB of keys), and counts kept in
+    LDS per workgroup (2.2M neurons: 8594 tile counts beside 64 keys)."""
+    host = dtb.gen_fanout(n, k, 5)
+    expect = np.bincount(host.col.astype(np.int64) >> 8, minlength=(n + 255) // 256)
+    eng = dtb.Engine(0)
+    try:
+        inst = eng.add_fanout(n, k, 5, dt=DT, seed=1)
+        assert np.array_equal(eng.tile_synapses(inst), expect)
+    finally:
+        eng.close()
