@@ -109,8 +109,8 @@ const char *dtb_last_error(void);
 int dtb_device_count(int *count);
 /* No counterpart in the reference (its blocks live for the whole run).
  * Device memory of removed instances and resized buffers stays with the
- * library and is reused (blocks of 1 MB and more are carved best-fit from
- * larger allocations); a server reserves its budget once with
+ * library and is reused (every block is carved best-fit from larger
+ * allocations); a server reserves its budget once with
  * dtb_device_memory_reserve so that admitting and retiring instances makes
  * no HIP allocation.  dtb_device_memory_cached reports the unused bytes held
  * on `device`; dtb_device_memory_trim returns them to the HIP runtime. */
