@@ -94,6 +94,8 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
 {
     if (trial_steps < 1)
         throw Error(kInvalidParameter, "autotune needs at least one step per trial");
+    if (active() == 0)
+        throw Error(kInvalidParameter, "autotune needs an instance to step");
     static const uint32_t ks[3] = {1, 2, 4};
     static const int layouts[2] = {kLayoutDense, kLayoutSpread};
     // The cost of a step follows the network's activity, which drifts and
