@@ -208,24 +208,10 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
-// Queues synapses c0 .. c0 + 64 K - 1 of one fired row (one pass) for the next
-// launch: each synapse's target index within its tile, channel pair and
-// weight pair go into the target tile's queue.  The row's synapses are
-// ordered by target, so the synapses of one tile form segments; each segment
-// reserves its slots with one atomic add on the tile's count and its
-// synapses are stored contiguously.  Called by a whole wave with
-// wave-uniform arguments.  Lane l holds synapses c0 + K l .. c0 + K l + K - 1
-// of the row (segments end at pass boundaries).  A synapse's segment start is
-// the last start at or before it (within the lane, else from a wave max-scan
-// of the lanes' last starts); the lane holding the segment's last synapse
-// reserves and publishes the destination and the number of the segment's
-// synapses that fit the queue in seg_dst / seg_room (this wave's 64 K LDS
-// entries, indexed by segment start within the pass).  Synapses past the
-// queue's capacity add their weights into the target's spill accumulator
-// instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
-// does not depend on which path a synapse took).
+// bin_pass with the loads and reservations of a lane's synapses one after the
+// other (measured faster for one synapse per lane).
 template <uint32_t K>
-__device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+__device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, unsigned long long *__restrict__ spill,
@@ -324,6 +310,142 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+}
+
+
+// Queues synapses c0 .. c0 + 64 K - 1 of one fired row (one pass) for the next
+// launch: each synapse's target index within its tile, channel pair and
+// weight pair go into the target tile's queue.  The row's synapses are
+// ordered by target, so the synapses of one tile form segments; each segment
+// reserves its slots with one atomic add on the tile's count and its
+// synapses are stored contiguously.  Called by a whole wave with
+// wave-uniform arguments.  Lane l holds synapses c0 + K l .. c0 + K l + K - 1
+// of the row (segments end at pass boundaries).  A synapse's segment start is
+// the last start at or before it (within the lane, else from a wave max-scan
+// of the lanes' last starts); the lane holding the segment's last synapse
+// reserves and publishes the destination and the number of the segment's
+// synapses that fit the queue in seg_dst / seg_room (this wave's 64 K LDS
+// entries, indexed by segment start within the pass).  Synapses past the
+// queue's capacity add their weights into the target's spill accumulator
+// instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
+// does not depend on which path a synapse took).
+template <uint32_t K>
+__device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+                                        int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
+                                        const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
+                                        QSyn *__restrict__ queue, unsigned long long *__restrict__ spill,
+                                        StepCtl *__restrict__ ctl, unsigned long long *seg_dst, uint32_t *seg_room,
+                                        uint32_t lane)
+{
+    constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
+    const auto col = gp(col_all) + row_begin;
+    const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
+    const uint32_t j0 = c0 + lane * K;
+    // Every lane loads K synapses without a branch (past the row's end it
+    // re-reads the last one and marks it invalid), so all loads are in
+    // flight together.
+    uint32_t cv[K], tl[K];
+    float2 wv[K];
+    bool ok[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        ok[k] = j0 + k < len;
+        const uint32_t jj = ok[k] ? j0 + k : len - 1u;
+        cv[k] = col[jj];
+        const unsigned long long wb = wt[jj];
+        wv[k] = make_float2(__uint_as_float((uint32_t)wb), __uint_as_float((uint32_t)(wb >> 32)));
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k)
+        tl[k] = ok[k] ? (cv[k] & kColMask) >> kTileBits : 0xfffffff0u + k;
+    // neighbours across lanes; the pass boundaries end segments
+    uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
+    uint32_t nextK = __shfl_down(tl[0], 1, kWave);
+    if (lane == 0)
+        prev0 = 0xffffffe0u;
+    if (lane == kWave - 1)
+        nextK = 0xffffffe1u;
+    uint32_t last_start = kNone;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+        if (ok[k] && tl[k] != pv)
+            last_start = j0 + k;
+    }
+    // exclusive max-scan over lanes of last_start (kNone counts as -1)
+    uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t u = __shfl_up(sc, off, kWave);
+        if (lane >= (uint32_t)off)
+            sc = max(sc, u);
+    }
+    uint32_t before = __shfl_up(sc, 1, kWave);
+    if (lane == 0)
+        before = 0u;
+    uint32_t open = (before != 0u) ? before - 1u : c0;   // segment open at the lane's first synapse
+    uint32_t st[K];
+    bool end[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        const uint32_t pv = (k == 0) ? prev0 : tl[k - 1];
+        const uint32_t nx = (k == K - 1) ? nextK : tl[k + 1];
+        if (ok[k] && tl[k] != pv)
+            open = j0 + k;
+        st[k] = open;
+        end[k] = ok[k] && tl[k] != nx;
+    }
+    // Reservations: every segment-end synapse issues its atomic and its
+    // tile's queue range before any result is used, so they overlap.
+    uint32_t base[K];
+    unsigned long long qoff[K];
+    uint32_t qcap[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        base[k] = 0u;
+        qoff[k] = 0ull;
+        qcap[k] = 0u;
+        if (end[k]) {
+            const uint32_t tg = tile_base + tl[k];
+            base[k] = atomicAdd(&qcnt[count_slot(qset + tg, cshift)], j0 + k + 1u - st[k]);
+            qoff[k] = tiles[tg].qoff;
+            qcap[k] = tiles[tg].qcap;
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        if (end[k]) {
+            const uint32_t n = j0 + k + 1u - st[k];
+            const uint32_t room = qcap[k] > base[k] ? min(n, qcap[k] - base[k]) : 0u;
+            if (room < n)
+                atomicAdd(&ctl->spilled, (unsigned long long)(n - room));
+            seg_dst[st[k] - c0] = qoff[k] + base[k];
+            seg_room[st[k] - c0] = room;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+        if (ok[k]) {
+            const uint32_t at = j0 + k - st[k];
+            if (at < seg_room[st[k] - c0]) {
+                QSyn e;
+                e.key = (cv[k] & 0xffu) | ((cv[k] & kKindBit) ? kQSynKind : 0u);
+                e.w0 = wv[k].x;
+                e.w1 = wv[k].y;
+                queue[seg_dst[st[k] - c0] + at] = e;
+            } else {
+                unsigned long long *a = spill + (size_t)(cv[k] & kColMask) * 4u + ((cv[k] & kKindBit) ? 2u : 0u);
+                atomicAdd(a, f32_to_q40(wv[k].x));
+                atomicAdd(a + 1, f32_to_q40(wv[k].y));
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 
@@ -531,9 +653,18 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         const uint32_t len_s = row_len[r];
         for (uint32_t c0 = 0; c0 < len_s; c0 += kWave * K, ++pass)
             if ((pass & 3u) == w)
-                bin_pass<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
-                            (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, I.spill + (size_t)(par ^ 1u) * I.n * 4u,
-                            ctl, seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
+            {
+                if constexpr (K == 1)
+                    bin_pass_serial<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                                       (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
+                                       I.spill + (size_t)(par ^ 1u) * I.n * 4u, ctl, seg_dst + w * kWave * K,
+                                       seg_room + w * kWave * K, lane);
+                else
+                    bin_pass<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                                (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
+                                I.spill + (size_t)(par ^ 1u) * I.n * 4u, ctl, seg_dst + w * kWave * K,
+                                seg_room + w * kWave * K, lane);
+            }
     }
 }
 
