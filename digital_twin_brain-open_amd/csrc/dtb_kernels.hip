@@ -109,17 +109,6 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v, uint32_t lane)
-{
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t u = __shfl_up(v, off, kWave);
-        if (lane >= (uint32_t)off)
-            v += u;
-    }
-    return v;
-}
-
 // Adds the queued synapses of one tile into acc[256][4] (Q23.40, LDS): every
 // thread takes every 256th entry of the tile's queue (contiguous, so the
 // reads coalesce) and adds its weight pair with 64-bit integer LDS atomics,
@@ -208,6 +197,34 @@ __device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, ui
     return r;
 }
 
+// Lane l gets lane l - 1's value (lane 0: fill), within the wave (DPP
+// wave_shr:1, no LDS round trip).
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t fill)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+// Lane l gets lane l + 1's value (lane 63: fill) (DPP wave_shl:1).
+__device__ __forceinline__ uint32_t wave_shl1(uint32_t v, uint32_t fill)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x130, 0xf, 0xf, false);
+}
+
+// Inclusive max-scan over the 64 lanes with DPP: shifts by 1, 2, 4, 8 within
+// each row of 16 lanes (lanes shifted in from outside the row read 0, the
+// identity), then the last lane of row 0 / 2 into rows 1 / 3 and the last
+// lane of row 1 into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return v;
+}
+
 // bin_pass with the loads and reservations of a lane's synapses one after the
 // other (measured faster for one synapse per lane).
 template <uint32_t K>
@@ -239,12 +256,8 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
             }
         }
         // neighbours across lanes; the pass boundaries end segments
-        uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
-        uint32_t nextK = __shfl_down(tl[0], 1, kWave);
-        if (lane == 0)
-            prev0 = 0xffffffe0u;
-        if (lane == kWave - 1)
-            nextK = 0xffffffe1u;
+        const uint32_t prev0 = wave_shr1(tl[K - 1], 0xffffffe0u);
+        const uint32_t nextK = wave_shl1(tl[0], 0xffffffe1u);
         uint32_t last_start = kNone;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
@@ -253,16 +266,8 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
                 last_start = j0 + k;
         }
         // exclusive max-scan over lanes of last_start (kNone counts as -1)
-        uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-            const uint32_t u = __shfl_up(sc, off, kWave);
-            if (lane >= (uint32_t)off)
-                sc = max(sc, u);
-        }
-        uint32_t before = __shfl_up(sc, 1, kWave);
-        if (lane == 0)
-            before = 0u;
+        const uint32_t sc = wave_max_scan((last_start == kNone) ? 0u : last_start + 1u);
+        const uint32_t before = wave_shr1(sc, 0u);
         uint32_t open = (before != 0u) ? before - 1u : c0;   // segment open at the lane's first synapse
         uint32_t st[K];
 #pragma unroll
@@ -359,12 +364,8 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
     for (uint32_t k = 0; k < K; ++k)
         tl[k] = ok[k] ? (cv[k] & kColMask) >> kTileBits : 0xfffffff0u + k;
     // neighbours across lanes; the pass boundaries end segments
-    uint32_t prev0 = __shfl_up(tl[K - 1], 1, kWave);
-    uint32_t nextK = __shfl_down(tl[0], 1, kWave);
-    if (lane == 0)
-        prev0 = 0xffffffe0u;
-    if (lane == kWave - 1)
-        nextK = 0xffffffe1u;
+    const uint32_t prev0 = wave_shr1(tl[K - 1], 0xffffffe0u);
+    const uint32_t nextK = wave_shl1(tl[0], 0xffffffe1u);
     uint32_t last_start = kNone;
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
@@ -373,16 +374,8 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
             last_start = j0 + k;
     }
     // exclusive max-scan over lanes of last_start (kNone counts as -1)
-    uint32_t sc = (last_start == kNone) ? 0u : last_start + 1u;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t u = __shfl_up(sc, off, kWave);
-        if (lane >= (uint32_t)off)
-            sc = max(sc, u);
-    }
-    uint32_t before = __shfl_up(sc, 1, kWave);
-    if (lane == 0)
-        before = 0u;
+    const uint32_t sc = wave_max_scan((last_start == kNone) ? 0u : last_start + 1u);
+    const uint32_t before = wave_shr1(sc, 0u);
     uint32_t open = (before != 0u) ? before - 1u : c0;   // segment open at the lane's first synapse
     uint32_t st[K];
     bool end[K];
