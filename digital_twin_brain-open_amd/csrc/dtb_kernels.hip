@@ -606,7 +606,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             rb = gp(I.rowptr)[li];
             len = (uint32_t)(gp(I.rowptr)[li + 1] - rb);
         }
-        const unsigned long long ev = wave_sum_u64((unsigned long long)len);
+        // events = sum of the fired rows' lengths: few per wave, read lane by
+        // lane through the scalar unit
+        unsigned long long ev = 0;
+        for (unsigned long long r = m; r != 0ull; r &= r - 1ull)
+            ev += (uint32_t)__builtin_amdgcn_readlane((int)len, __ffsll((long long)r) - 1);
         uint32_t base = 0;
         if ((int)lane == leader) {
             auto c = gp(I.wave_ctr) + 2u * ((d.begin >> 6) + w);
@@ -614,7 +618,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             c[1] += (unsigned long long)cnt_w;
             base = atomicAdd(&nrows, cnt_w);
         }
-        base = __shfl(base, leader, kWave);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
         if (emit) {
             const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             row_rb[k] = rb;
