@@ -101,7 +101,7 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
     // The cost of a step follows the network's activity, which drifts and
     // bursts; so the schedules take turns in short windows (kAutotuneWindow
     // timed steps, after 2 steps that apply the schedule) over the whole
-    // tuning period, and each is judged by its total.  A first untimed turn
+    // tuning period, and their totals are compared.  A first untimed turn
     // loads every kernel variant.
     const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
     double t_sum[6] = {0, 0, 0, 0, 0, 0};
