@@ -107,10 +107,11 @@ class BlockWrapper:
         if freqs:
             f = out["freq"].astype(np.int32)
             if freq_char:
-                # stat_result.cpp:128 sends round(255 c / n) per population;
-                # the client scales it back (dist_blockwrapper.py:285-289).
+                # stat_result.cpp:103-130 sends lroundf(255 c / (n - blocked))
+                # per population; the client scales it back by the
+                # population size (dist_blockwrapper.py:285-289).
                 n = self._neurons_per_subblk.astype(np.float32)
-                ch = np.rint(np.float32(255.0) * f.astype(np.float32) / n).astype(np.uint8)
+                ch = self._block.freq_char(out["freq"])
                 f = (ch.astype(np.float32) / np.float32(255.0) * n).astype(np.int32)
             seq.append(f)
         if vmean:

@@ -498,6 +498,22 @@ class Block:
     def neurons_per_subblk(self):
         return np.diff(self.pop_begin.astype(np.int64))
 
+    def freq_char(self, freq):
+        """Per-population spike counts [steps][npop] in the reference's char
+        form (stat_result.cpp:103-130): lroundf(255 c / (n - blocked)) as
+        uint8, with n the population size and blocked its neurons excluded
+        from statistics (property column 1; no property update changes it)."""
+        if getattr(self, "_stat_neurons", None) is None:
+            blocked = self.props()[:, 1] != 0
+            begin = self.pop_begin.astype(np.int64)
+            per = np.array([int(blocked[a:b].sum()) for a, b in zip(begin[:-1], begin[1:])], dtype=np.int64)
+            self._stat_neurons = (self.neurons_per_subblk - per).astype(np.float32)
+        x = np.float32(255.0) * np.asarray(freq).astype(np.float32)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = x / self._stat_neurons   # float32 division, as the reference's
+        q = np.where(np.isfinite(q), q, np.float32(0.0))
+        return np.floor(q.astype(np.float64) + 0.5).astype(np.uint8)   # lroundf of a value >= 0
+
     def update_ou_background_stimuli(self, population_id, correlation_time, mean, deviation):
         """dist_blockwrapper.py:583-612 (argument order kept)."""
         self.engine.set_ou(self.inst, population_id, mean, deviation, correlation_time)

@@ -195,9 +195,7 @@ class SnnServicer:
             return
         freq = out.get("freq")
         if freq is not None and request.freq_char:
-            # stat_result.cpp:128: round(255 c / n) per population
-            n = b.neurons_per_subblk.astype(np.float32)
-            freq = np.rint(np.float32(255.0) * freq.astype(np.float32) / n).astype(np.uint8)
+            freq = b.freq_char(freq)   # stat_result.cpp:103-130
         elif freq is not None:
             freq = freq.astype(np.int32)
         rec = out.get("receptor_imean")

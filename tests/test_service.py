@@ -94,3 +94,25 @@ def test_service_run_equals_block(dtb, gpu, service, tmp_path):
     assert np.array_equal(vm.view(np.uint32), ref["vmemb"].view(np.uint32))
     assert np.array_equal(vmean.view(np.uint32), ref["vmean"].view(np.uint32))
     assert freq.sum() > 50
+
+
+def test_freq_char_rounds_half_away_and_excludes_blocked(dtb):
+    """The reference's char-form spike counts, lroundf(255 c / (n - blocked))
+    (stat_result.cpp:103-130): half-way values round away from zero
+    (255 / 6 = 42.5 -> 43, where rounding half to even would give 42), and
+    blocked neurons leave the denominator (255 * 2 / 3 = 170)."""
+    class Stub:
+        pop_begin = np.array([0, 6, 10, 12], dtype=np.uint32)
+        neurons_per_subblk = np.array([6, 4, 2], dtype=np.int64)
+        _stat_neurons = None
+
+        def props(self):
+            p = np.zeros((12, dtb.NCOL), dtype=np.float32)
+            p[9, 1] = 1.0            # one blocked neuron in population 1
+            p[10:12, 1] = 1.0        # population 2 entirely blocked
+            return p
+
+    freq = np.array([[1, 2, 0], [6, 3, 0], [0, 0, 0]], dtype=np.uint32)
+    out = dtb.Block.freq_char(Stub(), freq)
+    assert out.dtype == np.uint8
+    assert out.tolist() == [[43, 170, 0], [255, 255, 0], [0, 0, 0]]
