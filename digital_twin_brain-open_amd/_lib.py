@@ -110,6 +110,7 @@ _SIGS = {
     "dtb_engine_schedule": (c_int, [c_void_p, c_int32, c_int32]),
     "dtb_engine_get_schedule": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "dtb_engine_autotune": (c_int, [c_void_p, c_int64, POINTER(c_int32), POINTER(c_int32), c_void_p]),
+    "dtb_instance_propagation": (c_int, [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_float)]),
     "dtb_instance_tile_synapses": (c_int, [c_void_p, c_int, c_void_p]),
     "dtb_engine_spilled": (c_int, [c_void_p, POINTER(c_uint64)]),
     "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),

@@ -362,8 +362,25 @@ int dtb_engine_get_schedule(dtb_engine_t h, int32_t *pass_k, int32_t *layout, in
             *pass_k = (int32_t)e.pass_k();
         if (layout)
             *layout = e.count_layout();
-        if (layout_in_use)
-            *layout_in_use = e.count_shift() == dtb::kCountShiftSpread ? DTB_LAYOUT_SPREAD : DTB_LAYOUT_DENSE;
+        if (layout_in_use) {
+            *layout_in_use = e.any_gather() ? DTB_LAYOUT_GATHER
+                                 : (e.count_shift() == dtb::kCountShiftSpread ? DTB_LAYOUT_SPREAD : DTB_LAYOUT_DENSE);
+        }
+    });
+}
+
+int dtb_instance_propagation(dtb_engine_t h, int inst, int32_t *gather_allowed, int32_t *gather_in_use,
+                             float *group_coverage)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        dtb::Instance *p = e.get(inst);
+        if (gather_allowed)
+            *gather_allowed = p->gather_allowed() ? 1 : 0;
+        if (gather_in_use)
+            *gather_in_use = e.use_gather(*p) ? 1 : 0;
+        if (group_coverage)
+            *group_coverage = (float)p->group_coverage();
     });
 }
 

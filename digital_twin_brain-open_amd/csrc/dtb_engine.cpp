@@ -211,8 +211,11 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
     tile_syn_.assign(ntile, 0u);
     tsyn.download(tile_syn_.data(), ntile, s);
     DTB_HIP(hipStreamSynchronize(s));
-    // expected distinct tiles among K uniform targets, as a fraction
+    // expected distinct tiles (groups) among K uniform targets, as a fraction
     row_coverage_ = 1.0 - std::pow(1.0 - 1.0 / ntile, (double)f.K);
+    group_coverage_ = 1.0 - std::pow(1.0 - 1.0 / ngroups(), (double)f.K);
+    // rows are ordered by target, K <= 8192
+    gather_ok_ = (double)n_ * (ngroups() + 1.0) * 2.0 <= 12.0 * (double)nsyn_;
 }
 
 Instance::~Instance() = default;
@@ -227,34 +230,74 @@ void Instance::count_tile_synapses(const NetworkHost &net)
     unsigned hw = std::thread::hardware_concurrency();
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw ? hw : 4u, 16u), nsyn_ / (1 << 20)));
     std::vector<std::vector<uint32_t>> part(nt, std::vector<uint32_t>(nfine, 0u));
-    std::vector<uint64_t> segs(nt, 0);
+    std::vector<uint64_t> segs(nt, 0), gsegs(nt, 0);
+    std::vector<char> sorted(nt, 1);
     std::vector<std::thread> th;
     for (int t = 0; t < nt; ++t) {
         th.emplace_back([&, t]() {
             const int64_t a = (int64_t)n_ * t / nt, b = (int64_t)n_ * (t + 1) / nt;
             uint32_t *c = part[t].data();
-            uint64_t sg = 0;
+            uint64_t sg = 0, gs = 0;
+            bool ordered = true;
             for (int64_t r = a; r < b; ++r) {
-                uint32_t prev = 0xffffffffu;
+                uint32_t prev = 0xffffffffu, gprev = 0xffffffffu;
                 for (int64_t j = net.rowptr[r]; j < net.rowptr[r + 1]; ++j) {
                     const uint32_t tl = net.col[j] >> kTileBits;
+                    const uint32_t gr = net.col[j] >> kGroupBits;
                     ++c[tl];
                     sg += tl != prev;
+                    gs += gr != gprev;
+                    ordered = ordered && (gprev == 0xffffffffu || gr >= gprev);
                     prev = tl;
+                    gprev = gr;
                 }
+                ordered = ordered && net.rowptr[r + 1] - net.rowptr[r] <= kGatherMaxRow;
             }
             segs[t] = sg;
+            gsegs[t] = gs;
+            sorted[t] = ordered ? 1 : 0;
         });
     }
     for (auto &x : th)
         x.join();
-    uint64_t sg = 0;
+    uint64_t sg = 0, gs = 0;
+    bool ordered = true;
     for (int t = 0; t < nt; ++t) {
         sg += segs[t];
+        gs += gsegs[t];
+        ordered = ordered && sorted[t];
         for (uint32_t k = 0; k < nfine; ++k)
             tile_syn_[k] += part[t][k];
     }
     row_coverage_ = (double)sg / ((double)n_ * (double)nfine);
+    group_coverage_ = (double)gs / ((double)n_ * (double)ngroups());
+    gather_ok_ = ordered && nsyn_ > 0 && (double)n_ * (ngroups() + 1.0) * 2.0 <= 12.0 * (double)nsyn_;
+}
+
+size_t Instance::gather_bytes() const
+{
+    return (size_t)n_ * (ngroups() + 1u) * sizeof(uint16_t) + 2u * (size_t)n_ * sizeof(uint32_t) + 64u;
+}
+
+void Instance::ensure_gather(hipStream_t s)
+{
+    if (!gather_ok_)
+        throw Error(kInvalidParameter, "this network does not allow gather propagation");
+    if (segidx_.count() != 0)
+        return;
+    fcount_.resize(2);
+    fcount_.mutable_gpu_data();
+    fired_.resize(2 * (size_t)n_);
+    fired_.mutable_gpu_data();
+    segidx_.resize((size_t)n_ * (ngroups() + 1u));
+    DTB_HIP(launch_build_segidx(rowptr_.gpu_data(), col_.gpu_data(), n_, ngroups(), segidx_.mutable_gpu_data(), s));
+    DTB_HIP(hipStreamSynchronize(s));
+}
+
+void Instance::clear_fired(hipStream_t s)
+{
+    if (fcount_.count() != 0)
+        DTB_HIP(hipMemsetAsync(fcount_.mutable_gpu_data(), 0, fcount_.size(), s));
 }
 
 uint32_t Instance::tile_synapses(uint32_t tile) const
@@ -779,9 +822,17 @@ void Instance::clear_spill(hipStream_t s)
     DTB_HIP(hipMemsetAsync(spill_.mutable_gpu_data(), 0, spill_.size(), s));
 }
 
-InstDev Instance::dev(uint32_t tile_base)
+InstDev Instance::dev(uint32_t tile_base, bool gather)
 {
     InstDev d{};
+    if (gather) {
+        if (segidx_.count() == 0)
+            throw Error(kInvalidParameter, "gather buffers not built");
+        d.segidx = segidx_.gpu_data();
+        d.fired = fired_.mutable_gpu_data();
+        d.fcount = fcount_.mutable_gpu_data();
+        d.ngroups = ngroups();
+    }
     d.pA = pA_.gpu_data();
     d.pB = pB_.gpu_data();
     d.gU = gU_.gpu_data();

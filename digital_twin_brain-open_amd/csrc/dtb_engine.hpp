@@ -280,7 +280,8 @@ public:
     void set_step(int64_t step, hipStream_t s);
     void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
 
-    InstDev dev(uint32_t tile_base);
+    // Kernel descriptor; `gather`: propagate by gather (ensure_gather first).
+    InstDev dev(uint32_t tile_base, bool gather);
     // Distinct parameter records (classes); n() when stored per neuron.
     uint32_t param_classes() const { return (uint32_t)pA_.count(); }
     SampleRecDev sample_rec();
@@ -300,6 +301,20 @@ public:
     // Mean over rows of the fraction of this instance's tiles that the row's
     // synapses reach (tile segments of the row / tiles).
     double row_coverage() const { return row_coverage_; }
+
+    // Gather propagation (k_gather): possible when every row is ordered by
+    // target group, no row is longer than kGatherMaxRow and the group offsets
+    // ([n][ngroups + 1] x 2 bytes) are no larger than the synapse arrays.
+    bool gather_allowed() const { return gather_ok_; }
+    // Mean over rows of the fraction of the groups that the row reaches.
+    double group_coverage() const { return group_coverage_; }
+    uint32_t ngroups() const { return (n_ + (1u << kGroupBits) - 1) >> kGroupBits; }
+    // Device bytes of the gather buffers (group offsets, fired lists).
+    size_t gather_bytes() const;
+    // Builds the group offsets and fired lists if not built yet.
+    void ensure_gather(hipStream_t s);
+    // Empties both fired lists (checkpoint restore).
+    void clear_fired(hipStream_t s);
 
 private:
     void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
@@ -349,6 +364,11 @@ private:
     bool rec_freq_ = false, rec_samples_ = false;
     std::vector<uint32_t> tile_syn_;      // synapses per target tile
     double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
+    double group_coverage_ = 0.0;         // mean fraction of the gather groups one row reaches
+    bool gather_ok_ = false;              // gather_allowed()
+    DataAllocator<uint16_t> segidx_;      // [n][ngroups + 1] group offsets (built on demand)
+    DataAllocator<uint32_t> fired_;       // [2][n] fired rows, by parity
+    DataAllocator<uint32_t> fcount_;      // [2]
     uint32_t gamma_calls_ = 0;            // gamma draw streams used
     std::vector<uint32_t> tile_cls_;      // per tile: 1 + shared parameter class, or 0
 };
@@ -377,23 +397,35 @@ public:
     // made (queues past their capacity; not an error).
     uint64_t spilled();
     // Step-kernel schedule: synapses per lane in a binning pass (1, 2, 4) and
-    // the queue-count layout (kLayoutAuto picks by row coverage).  Every
-    // schedule gives the same results.
+    // the propagation layout: queues with dense or spread counts
+    // (kLayoutAuto: by row coverage), or gather (kLayoutGather: every
+    // instance that allows it; the others use dense queues).  Every schedule
+    // gives the same results.
     void set_schedule(uint32_t pass_k, int layout);
+    // Whether instance p propagates by gather under the current layout.
+    bool use_gather(const Instance &p) const;
+    // Instances of the current tables that propagate by gather.
+    uint32_t gather_instances() const { return ngather_; }
     uint32_t pass_k() const { return pass_k_; }
     int count_layout() const { return count_layout_; }
     uint32_t count_shift() const { return cshift_; }
     // Times every schedule on about `trial_steps` real steps of all
     // instances, in turns of kAutotuneWindow steps (each preceded by 2 steps
-    // that apply it, and one untimed turn first), keeps the fastest; us[6] =
-    // mean microseconds per step, [dense, spread] x [K = 1, 2, 4].  Steps run:
+    // that apply it, and one untimed turn first), keeps the fastest; us[7] =
+    // mean microseconds per step, [dense, spread] x [K = 1, 2, 4], then
+    // gather (K = 2; a candidate only when some instance allows it, else its
+    // turns run the default schedule and us[6] = 0).  Steps run:
     // autotune_steps(trial_steps).
     static constexpr int64_t kAutotuneWindow = 4;
     static int64_t autotune_steps(int64_t trial_steps)
     {
         const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
-        return 6 * (2 + rounds * (2 + kAutotuneWindow));
+        return 7 * (2 + rounds * (2 + kAutotuneWindow));
     }
+    // Some instance allows gather propagation.
+    bool gather_possible() const;
+    // Some instance propagates by gather under the current layout.
+    bool any_gather() const;
     void autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, float *us);
     // Queue capacity per tile: min(synapses targeting it, floor +
     // (synapses >> shift)); shift 0 keeps the exact worst case.
@@ -463,6 +495,9 @@ private:
     DataAllocator<SampleRecDev> d_recs_;
     DataAllocator<TileDesc> d_tiles_;
     DataAllocator<uint32_t> d_qcnt_;       // count_words(2 * ntiles)
+    DataAllocator<GroupDesc> d_groups_;    // gather groups of the gather instances
+    uint32_t ngroups_ = 0, ngather_ = 0;
+    void launch_gather_k(uint32_t par);    // k_gather of parity par, if any instance gathers
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
     int graph_steps_[2] = {0, 0};
 };

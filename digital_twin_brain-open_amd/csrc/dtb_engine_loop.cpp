@@ -38,6 +38,8 @@ int Engine::add(std::unique_ptr<Instance> inst)
     // memory shows up when it is added, not in the middle of a later run.
     DTB_HIP(hipSetDevice(device_));
     inst->ensure_queue(queue_slots(*inst));
+    if (use_gather(*inst))
+        inst->ensure_gather(admit_stream_);
     for (size_t i = 0; i < insts_.size(); ++i)
         if (!insts_[i]) {
             insts_[i] = std::move(inst);
@@ -81,13 +83,34 @@ void Engine::set_schedule(uint32_t pass_k, int layout)
 {
     if (pass_k != 1 && pass_k != 2 && pass_k != 4)
         throw Error(kInvalidParameter, "pass size must be 1, 2 or 4 synapses per lane");
-    if (layout != kLayoutAuto && layout != kLayoutDense && layout != kLayoutSpread)
-        throw Error(kInvalidParameter, "unknown queue-count layout");
+    if (layout != kLayoutAuto && layout != kLayoutDense && layout != kLayoutSpread && layout != kLayoutGather)
+        throw Error(kInvalidParameter, "unknown propagation layout");
     if (pass_k != pass_k_ || layout != count_layout_) {
         pass_k_ = pass_k;
         count_layout_ = layout;
         dirty_ = true;   // graphs hold the old kernel; counts move to the new layout
     }
+}
+
+bool Engine::use_gather(const Instance &p) const
+{
+    return p.gather_allowed() && count_layout_ == kLayoutGather;
+}
+
+bool Engine::any_gather() const
+{
+    for (auto &p : insts_)
+        if (p && use_gather(*p))
+            return true;
+    return false;
+}
+
+bool Engine::gather_possible() const
+{
+    for (auto &p : insts_)
+        if (p && p->gather_allowed())
+            return true;
+    return false;
 }
 
 void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, float *us)
@@ -96,18 +119,26 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
         throw Error(kInvalidParameter, "autotune needs at least one step per trial");
     if (active() == 0)
         throw Error(kInvalidParameter, "autotune needs an instance to step");
-    static const uint32_t ks[3] = {1, 2, 4};
-    static const int layouts[2] = {kLayoutDense, kLayoutSpread};
+    static const uint32_t ks[7] = {1, 2, 4, 1, 2, 4, kPassKDefault};
+    static const int layouts[7] = {kLayoutDense, kLayoutDense, kLayoutDense, kLayoutSpread,
+                                   kLayoutSpread, kLayoutSpread, kLayoutGather};
+    // Without an instance that allows gather the seventh turn runs the
+    // default schedule and is not a candidate, so the number of steps does
+    // not depend on the networks.
+    const int nc = gather_possible() ? 7 : 6;
     // The cost of a step follows the network's activity, which drifts and
     // bursts; so the schedules take turns in short windows (kAutotuneWindow
     // timed steps, after 2 steps that apply the schedule) over the whole
     // tuning period, and their totals are compared.  A first untimed turn
-    // loads every kernel variant.
+    // loads every kernel variant (and builds the gather offsets).
     const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
-    double t_sum[6] = {0, 0, 0, 0, 0, 0};
+    double t_sum[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int64_t r = -1; r < rounds; ++r) {
-        for (int s = 0; s < 6; ++s) {
-            set_schedule(ks[s % 3], layouts[s / 3]);
+        for (int s = 0; s < 7; ++s) {
+            if (s < nc)
+                set_schedule(ks[s], layouts[s]);
+            else
+                set_schedule(kPassKDefault, kLayoutDense);
             run(2, 0);   // applies the schedule (rebuild) outside the timed span
             if (r < 0)
                 continue;
@@ -117,14 +148,14 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
         }
     }
     int bi = 0;
-    for (int i = 1; i < 6; ++i)
+    for (int i = 1; i < nc; ++i)
         if (t_sum[i] < t_sum[bi])
             bi = i;
     if (us)
-        for (int i = 0; i < 6; ++i)
-            us[i] = (float)(t_sum[i] * 1e3 / (double)(rounds * kAutotuneWindow));
-    const uint32_t bk = ks[bi % 3];
-    const int bl = layouts[bi / 3];
+        for (int i = 0; i < 7; ++i)
+            us[i] = i < nc ? (float)(t_sum[i] * 1e3 / (double)(rounds * kAutotuneWindow)) : 0.0f;
+    const uint32_t bk = ks[bi];
+    const int bl = layouts[bi];
     set_schedule(bk, bl);
     if (best_k)
         *best_k = bk;
@@ -192,6 +223,8 @@ void Engine::rebuild()
     staged_.clear();
     std::vector<InstDev> table;
     std::vector<TileDesc> tiles;
+    std::vector<GroupDesc> groups;
+    uint32_t ngather = 0;
     std::vector<SampleDev> samples;
     std::vector<SampleRecDev> recs;
     tile_base_.assign(insts_.size(), 0u);
@@ -224,20 +257,32 @@ void Engine::rebuild()
         // slack), so instances of any size come and go without one large
         // engine-wide block.
         p->ensure_queue(qtotal);
-        table.push_back(p->dev(tile_base_[slot]));
+        const bool gat = use_gather(*p);
+        if (gat) {
+            p->ensure_gather(stream_);
+            for (uint32_t g = 0; g < p->ngroups(); ++g)
+                groups.push_back(GroupDesc{idx, g, std::min<uint32_t>(1u << kGroupBits, p->n() - (g << kGroupBits)), 0u});
+            ++ngather;
+        }
+        table.push_back(p->dev(tile_base_[slot], gat));
         recs.push_back(p->sample_rec());
         if (recs.back().nsample > 0) {
             const auto &ids = p->samples();
             for (uint32_t k = 0; k < ids.size(); ++k)
                 samples.push_back(SampleDev{idx, ids[k], k, 0});
         }
-        total += p->n();
-        coverage += p->row_coverage() * p->n();
+        if (!gat) {
+            total += p->n();
+            coverage += p->row_coverage() * p->n();
+        }
     }
     if (total > 0xffffffffull || tiles.size() > 0xffffffffull)
         throw Error(kInvalidParameter, "too many neurons in one engine");
     ninst_ = (uint32_t)table.size();
     ntiles_ = (uint32_t)tiles.size();
+    ngroups_ = (uint32_t)groups.size();
+    ngather_ = ngather;
+    d_groups_.resize(std::max<size_t>(groups.size(), 1));
     nsamples_ = (uint32_t)samples.size();
     d_insts_.resize(std::max<size_t>(table.size(), 1));
     d_tiles_.resize(std::max<size_t>(tiles.size(), 1));
@@ -248,7 +293,7 @@ void Engine::rebuild()
     // the tile's passes dealt to all its waves, dense was as fast or faster
     // up to a coverage of 0.92 (in-degree 5000 on 500k neurons), so the
     // threshold sits above that; the autotuner measures instead of guessing.
-    if (count_layout_ == kLayoutDense)
+    if (count_layout_ == kLayoutDense || count_layout_ == kLayoutGather)
         cshift_ = kCountShiftDense;
     else if (count_layout_ == kLayoutSpread)
         cshift_ = kCountShiftSpread;
@@ -260,17 +305,25 @@ void Engine::rebuild()
     d_qcnt_.mutable_gpu_data();
     d_samples_.mutable_gpu_data();
     d_recs_.mutable_gpu_data();
+    d_groups_.mutable_gpu_data();
     DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data(), 0, d_qcnt_.size(), stream_));
     d_insts_.upload(table.data(), table.size(), stream_);
     d_tiles_.upload(tiles.data(), tiles.size(), stream_);
+    d_groups_.upload(groups.data(), groups.size(), stream_);
     d_samples_.upload(samples.data(), samples.size(), stream_);
     d_recs_.upload(recs.data(), recs.size(), stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
     dirty_ = false;
 }
 
+void Engine::launch_gather_k(uint32_t par)
+{
+    DTB_HIP(launch_gather(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par, stream_));
+}
+
 void Engine::launch_step_k()
 {
+    launch_gather_k(par_);
     DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
                                d_ctl_.mutable_gpu_data(), pass_k_, stream_));
 }
@@ -280,6 +333,7 @@ void Engine::launch_aux()
     // par_ has already advanced: the step just run had parity par_ ^ 1 and
     // its spikes are queued in set par_.
     if (nsamples_ > 0) {
+        launch_gather_k(par_);
         DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
         DTB_HIP(launch_sample(d_insts_.gpu_data(), d_samples_.gpu_data(), nsamples_, d_recs_.gpu_data(), par_ ^ 1u,
                               stream_));
@@ -339,6 +393,7 @@ void Engine::settle()
     if (!pending_)
         return;
     DTB_HIP(hipSetDevice(device_));
+    launch_gather_k(par_);
     DTB_HIP(launch_settle(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
     pending_ = false;
@@ -349,6 +404,7 @@ void Engine::materialize()
     DTB_HIP(hipSetDevice(device_));
     if (dirty_)
         rebuild();
+    launch_gather_k(par_);
     DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
 }
@@ -367,6 +423,7 @@ void Engine::clear_queues(int id)
         DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data() + first, 0, (last + 1 - first) * sizeof(uint32_t), stream_));
     }
     get(id)->clear_spill(stream_);
+    get(id)->clear_fired(stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -377,6 +434,7 @@ void Engine::stat_tail()
         rebuild();
     if (ninst_ == 0)
         return;
+    launch_gather_k(par_);
     DTB_HIP(launch_materialize(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
     DTB_HIP(launch_stat_tail(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), par_, stream_));
     DTB_HIP(hipStreamSynchronize(stream_));
@@ -393,26 +451,35 @@ void Engine::run_timed(int64_t steps, float *ms_step)
     // Every step kernel records its own start and stop events (at the
     // dispatch boundaries, on the engine stream); they are read back once
     // per chunk so the launch pipeline is not drained every step.
+    // With gather instances the step is two kernels (k_gather, k_step),
+    // each timed.
     const int64_t chunk = 256;
-    std::vector<hipEvent_t> ev((size_t)(2 * chunk));
+    std::vector<hipEvent_t> ev((size_t)(4 * chunk));
     for (auto &e : ev)
         DTB_HIP(hipEventCreate(&e));
     double ts = 0.0;
     for (int64_t base = 0; base < steps; base += chunk) {
         const int64_t m = std::min<int64_t>(chunk, steps - base);
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(2 * k)];
+            hipEvent_t *e = &ev[(size_t)(4 * k)];
+            if (ngroups_ > 0)
+                DTB_HIP(launch_gather_timed(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par_, e[2], e[3],
+                                            stream_));
             DTB_HIP(launch_step_kernel_timed(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
                                              d_ctl_.mutable_gpu_data(), pass_k_, e[0], e[1], stream_));
             par_ ^= 1u;
             launch_aux();
         }
-        DTB_HIP(hipEventSynchronize(ev[(size_t)(2 * (m - 1) + 1)]));
+        DTB_HIP(hipEventSynchronize(ev[(size_t)(4 * (m - 1) + 1)]));
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(2 * k)];
+            hipEvent_t *e = &ev[(size_t)(4 * k)];
             float a = 0.0f;
             DTB_HIP(hipEventElapsedTime(&a, e[0], e[1]));
             ts += a;
+            if (ngroups_ > 0) {
+                DTB_HIP(hipEventElapsedTime(&a, e[2], e[3]));
+                ts += a;
+            }
         }
     }
     for (auto &e : ev)

@@ -457,6 +457,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / kWave;
+    const bool gat = I.segidx != nullptr;   // gather propagation: k_gather wrote the delta into dlt
     const QSyn *__restrict__ q_in = I.queue + (size_t)par * I.qslots;
     const size_t cslot_in = count_slot((size_t)par * qs.ntiles + blockIdx.x, qs.cshift);
     QSyn *__restrict__ q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
@@ -484,7 +485,12 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     // The queue is complete when the launch starts; entries past the count
     // are ignored (the queue buffer has kQueuePrefetch slots of slack at its
     // end, so the read needs no branch).
-    const QSyn head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
+    QSyn head{};
+    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (gat)
+        dl = ld4(I.dlt, lc);
+    else
+        head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
     const int64_t step = gp(I.step)[par];
 
     acc[tid * 4 + 0] = 0ull;
@@ -493,11 +499,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     acc[tid * 4 + 3] = 0ull;
     if (tid == 0)
         nrows = 0u;
-    const uint32_t cnt = min(cnt_raw, d.qcap);
+    const uint32_t cnt = gat ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
     __syncthreads();
-    if (cnt_raw > d.qcap && active)
+    if (!gat && cnt_raw > d.qcap && active)
         take_spill(acc, I, par, li, tid, true);   // synapses past the queue's capacity
     if (tid == 0)
         qs.count[cslot_in] = 0u;   // this launch consumed the queue
@@ -509,7 +515,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     float isyn = 0.0f;
     if (active) {
         if (!(fl & kFlagFolded))
-            J = fold_delta(J, dc, acc_to_delta(acc, tid));
+            J = fold_delta(J, dc, gat ? dl : acc_to_delta(acc, tid));
         isyn = syn_current(g, vu, J, V);
         while (p + 1 < I.npop && li >= I.pops[p].end)
             ++p;
@@ -616,13 +622,30 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             auto c = gp(I.wave_ctr) + 2u * ((d.begin >> 6) + w);
             c[0] += ev;
             c[1] += (unsigned long long)cnt_w;
-            base = atomicAdd(&nrows, cnt_w);
+            if (!gat)
+                base = atomicAdd(&nrows, cnt_w);
         }
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if (emit) {
-            const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            row_rb[k] = rb;
-            row_len[k] = len;
+        if (gat) {
+            // gather propagation: the rows with synapses go to the instance's
+            // fired list for the next launch's k_gather (one atomic per wave)
+            const bool has = emit && len != 0u;
+            const unsigned long long mg = __ballot(has);
+            if (mg != 0ull) {
+                const int lg = __ffsll((long long)mg) - 1;
+                uint32_t gb = 0;
+                if ((int)lane == lg)
+                    gb = atomicAdd(I.fcount + (par ^ 1u), (uint32_t)__popcll(mg));
+                gb = (uint32_t)__builtin_amdgcn_readlane((int)gb, lg);
+                if (has)
+                    gp(I.fired)[(size_t)(par ^ 1u) * I.n + gb + (uint32_t)__popcll(mg & ((1ull << lane) - 1ull))] = li;
+            }
+        } else {
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+            if (emit) {
+                const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                row_rb[k] = rb;
+                row_len[k] = len;
+            }
         }
         if (I.rec_counts != nullptr) {
             // stat_result.cpp:161-206: one atomic per population present in
@@ -691,6 +714,131 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
         step_tile<false, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
 }
 
+// Gather propagation (instances with InstDev::segidx; their k_step reads
+// the delta from dlt): the delta of the rows fired by the last launch (list
+// `par`) for one group of kGroupTiles tiles, written into dlt.  A wave takes
+// 64 fired rows at a time: lane l reads row l's segment bounds for the group
+// from the row's group offsets, a wave prefix sum lays the segments end to
+// end, and the wave then reads them with consecutive lanes on consecutive
+// synapses (a segment is contiguous in the row), adding each weight pair
+// into the group's Q23.40 LDS sums (64-bit integer atomics, so the result
+// does not depend on the order).  Also empties fired list par ^ 1 (the one
+// the k_step launch after this one fills; its last reader was the previous
+// k_gather).  Consecutive groups go to one XCD (workgroups are dealt to the
+// 8 XCDs round-robin by index), so the segments of neighbouring groups,
+// which share cache lines, meet in one L2.
+__global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__restrict__ groups,
+                                                         const InstDev *__restrict__ insts, uint32_t par)
+{
+    constexpr uint32_t kWaves = kGatherBlock / kWave;
+    constexpr uint32_t kNeurons = 1u << kGroupBits;
+    __shared__ unsigned long long acc[kNeurons * 4];
+    __shared__ uint32_t s_off[kWaves][kWave];             // per wave: first item of each row's segment
+    __shared__ unsigned long long s_base[kWaves][kWave];  // and the segment's first synapse
+    const uint32_t nb = gridDim.x, x = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const GroupDesc d = groups[x * (nb >> 3) + min(x, nb & 7u) + slot];
+    const InstDev &I = insts[d.inst];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t wv = tid / kWave;
+    for (uint32_t i = tid; i < kNeurons * 4; i += kGatherBlock)
+        acc[i] = 0ull;
+    const uint32_t nf = gp(I.fcount)[par];
+    if (d.g == 0 && tid == 0)
+        gp(I.fcount)[par ^ 1u] = 0u;
+    __syncthreads();
+    const auto fired = gp(I.fired) + (size_t)par * I.n;
+    const auto col = gp(I.col);
+    const auto wt = gp(reinterpret_cast<const unsigned long long *>(I.w));
+    const uint32_t stride = I.ngroups + 1u;
+    const uint32_t first = d.g << kGroupBits;
+    for (uint32_t c0 = wv * kWave; c0 < nf; c0 += kGatherBlock) {
+        const uint32_t k = c0 + lane;
+        const bool valid = k < nf;
+        const uint32_t r = fired[valid ? k : nf - 1u];
+        const auto sg = gp(I.segidx) + (size_t)r * stride + d.g;
+        const uint32_t a = sg[0];
+        const uint32_t e = sg[1];
+        const int64_t rb = gp(I.rowptr)[r];
+        const uint32_t len = (valid && e > a) ? e - a : 0u;
+        uint32_t incl = len;
+#pragma unroll
+        for (uint32_t o = 1; o < (uint32_t)kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, kWave);
+            if (lane >= o)
+                incl += v;
+        }
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        s_off[wv][lane] = incl - len;
+        s_base[wv][lane] = (unsigned long long)(rb + (int64_t)a);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t it = 0; it < total; it += 2u * kWave) {
+            uint32_t cv[2];
+            unsigned long long wb[2];
+            bool ok[2];
+#pragma unroll
+            for (uint32_t u = 0; u < 2; ++u) {
+                const uint32_t item = it + u * kWave + lane;
+                ok[u] = item < total;
+                const uint32_t ic = ok[u] ? item : total - 1u;
+                // the row whose segment holds item: the last lane whose
+                // segment starts at or before it (empty segments before a
+                // non-empty one start at the same item and lose to it)
+                uint32_t lo = 0;
+#pragma unroll
+                for (uint32_t st = kWave / 2; st > 0; st >>= 1)
+                    if (s_off[wv][lo + st] <= ic)
+                        lo += st;
+                const size_t j = (size_t)s_base[wv][lo] + (ic - s_off[wv][lo]);
+                cv[u] = col[j];
+                wb[u] = wt[j];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 2; ++u) {
+                if (ok[u]) {
+                    const uint32_t sl = ((cv[u] & kColMask) - first) * 4u + ((cv[u] & kKindBit) ? 2u : 0u);
+                    atomicAdd(&acc[sl], f32_to_q40(__uint_as_float((uint32_t)wb[u])));
+                    atomicAdd(&acc[sl + 1u], f32_to_q40(__uint_as_float((uint32_t)(wb[u] >> 32))));
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < d.count; i += kGatherBlock)
+        st4(I.dlt, first + i, acc_to_delta(acc, i));
+}
+
+// Group offsets of the gather propagation, one wave per row: segidx[r][g] =
+// offset in row r of the first synapse whose target lies in group g or a
+// later one, segidx[r][ngroups] = row length.  Rows must be ordered by
+// target group (checked on the host).
+__global__ __launch_bounds__(kBlock) void k_build_segidx(const int64_t *__restrict__ rowptr,
+                                                         const uint32_t *__restrict__ col, uint32_t n,
+                                                         uint32_t ngroups, uint16_t *__restrict__ segidx)
+{
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    if (r >= n)
+        return;
+    const int64_t rb = rowptr[r];
+    const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
+    uint16_t *s = segidx + (size_t)r * (ngroups + 1u);
+    for (uint32_t j = lane; j < len; j += kWave) {
+        const uint32_t gc = (col[rb + j] & kColMask) >> kGroupBits;
+        const uint32_t g0 = j == 0 ? 0u : ((col[rb + j - 1] & kColMask) >> kGroupBits) + 1u;
+        for (uint32_t g = g0; g <= gc; ++g)
+            s[g] = (uint16_t)j;
+    }
+    const uint32_t tail = len == 0 ? 0u : ((col[rb + len - 1] & kColMask) >> kGroupBits) + 1u;
+    for (uint32_t g = tail + lane; g <= ngroups; g += kWave)
+        s[g] = (uint16_t)len;
+}
+
 // Writes the pending delta (the runs queued in queue set `set` for the next
 // k_step) of every neuron of every tile without consuming the queues: used
 // before sampling I_syn and before reading the state back.
@@ -702,6 +850,8 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     __shared__ unsigned long long acc[kBlock * 4];
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
+    if (I.segidx != nullptr)
+        return;   // gather propagation: k_gather has written dlt
     const uint32_t tid = threadIdx.x;
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
@@ -732,6 +882,21 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     const TileDesc d = tiles[blockIdx.x];
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
+    if (I.segidx != nullptr) {
+        // gather propagation: k_gather has written the pending delta into
+        // dlt; after the fold the fired list of this set is empty
+        if (tid < d.count) {
+            const uint32_t li = d.begin + tid;
+            const uint32_t fl = gp(I.flag)[li];
+            if (!(fl & kFlagFolded)) {
+                st4(I.J, li, fold_delta(ld4(I.J, li), ld4(I.dec, param_index(I, li)), ld4(I.dlt, li)));
+                gp(I.flag)[li] = (uint8_t)(fl | kFlagFolded);
+            }
+        }
+        if (d.begin == 0 && tid == 0)
+            gp(I.fcount)[set] = 0u;
+        return;
+    }
     uint32_t *cnt_p = qs.count + count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift);
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
@@ -881,6 +1046,35 @@ hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, cons
         break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
+                         hipStream_t stream)
+{
+    if (ngroups == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(ngroups), dim3(kGatherBlock), 0, stream, groups, insts, par);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
+                               hipEvent_t start, hipEvent_t stop, hipStream_t stream)
+{
+    if (ngroups == 0)
+        return hipSuccess;
+    hipExtLaunchKernelGGL(k_gather, dim3(ngroups), dim3(kGatherBlock), 0, stream, start, stop, 0, groups, insts, par);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint32_t n, uint32_t ngroups,
+                               uint16_t *segidx, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t rows_per_block = kBlock / kWave;
+    const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
+    hipLaunchKernelGGL(k_build_segidx, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, n, ngroups, segidx);
     return hipGetLastError();
 }
 

@@ -16,6 +16,19 @@ hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, cons
                                     uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
                                     hipStream_t stream);
 
+// Gather propagation: delta of fired list `par` into InstDev::dlt for every
+// group (runs before the k_step launch of parity `par`, and before
+// materialize / settle of set `par`); empties fired list par ^ 1.
+hipError_t launch_gather(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
+                         hipStream_t stream);
+hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
+                               hipEvent_t start, hipEvent_t stop, hipStream_t stream);
+
+// Group offsets of the gather propagation ([n][ngroups + 1], rows ordered
+// by target group).
+hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint32_t n, uint32_t ngroups,
+                               uint16_t *segidx, hipStream_t stream);
+
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
                               uint32_t set, hipStream_t stream);

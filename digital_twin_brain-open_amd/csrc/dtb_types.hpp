@@ -94,6 +94,13 @@ struct alignas(16) InstDev {
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
     const float2 *w;            // [S] weight pair
+    // Gather propagation (null: queue propagation).  segidx[r][g] = offset in
+    // row r of its first synapse whose target lies in group g or later
+    // (segidx[r][ngroups] = row length); fired[p][] lists the rows with
+    // synapses fired by the launch of parity p ^ 1, fcount[p] of them.
+    const uint16_t *segidx;     // [n][ngroups + 1]
+    uint32_t *fired;            // [2][n]
+    uint32_t *fcount;           // [2]
     const PopDev *pops;         // [npop]
     // recording (device buffers, row = step - rec_step0)
     uint32_t *rec_counts;       // [rec_len][npop] spikes per population
@@ -117,7 +124,7 @@ struct alignas(16) InstDev {
     uint32_t tile_base;         // engine-wide index of this instance's first tile
     uint32_t stat_mask;         // kStatV | kStatI | kStatReceptor: statistics recorded in rec_stat
     uint32_t pmask;             // parameter record of neuron i: (i & pmask) + cls[i]
-    uint32_t pad;
+    uint32_t ngroups;           // gather groups of the instance (kGroupTiles tiles each)
 };
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
@@ -130,6 +137,23 @@ struct alignas(16) TileDesc {
     uint64_t qoff;      // first slot in the instance's queue of one parity
     uint32_t qcap;      // queue capacity: synapses targeting this tile (all rows)
     uint32_t pcls;      // 1 + the parameter class shared by every neuron of the tile, or 0
+};
+
+// Gather propagation: one workgroup of kGatherBlock threads gathers the
+// synapses of every fired row that target one group of kGroupTiles
+// consecutive tiles (2^kGroupBits neurons) into LDS, then writes their delta.
+constexpr uint32_t kGroupTiles = 4;
+constexpr uint32_t kGroupBits = kTileBits + 2;
+static_assert((1u << (kGroupBits - kTileBits)) == kGroupTiles, "a group is kGroupTiles tiles");
+constexpr int kGatherBlock = 512;
+// Longest row the 16-bit group offsets can index.
+constexpr int64_t kGatherMaxRow = 65535;
+
+struct alignas(16) GroupDesc {
+    uint32_t inst;
+    uint32_t g;         // group index within the instance (first neuron g << kGroupBits)
+    uint32_t count;     // neurons in the group
+    uint32_t pad;
 };
 
 // Queues of both parities: the launch of parity p drains set p and fills set
@@ -154,7 +178,9 @@ constexpr uint32_t kPassKDefault = 2;
 constexpr double kSpreadCoverage = 0.95;
 
 // Requested layout of an engine's queue counts.
-enum : int { kLayoutAuto = 0, kLayoutDense = 1, kLayoutSpread = 2 };
+// kLayoutGather: instances that allow it propagate by gather (the others
+// keep dense queues); it is chosen explicitly or by the autotuner.
+enum : int { kLayoutAuto = 0, kLayoutDense = 1, kLayoutSpread = 2, kLayoutGather = 3 };
 constexpr uint32_t kCountShiftDense = 5;
 constexpr uint32_t kCountShiftSpread = 0;
 

@@ -298,12 +298,13 @@ class Engine:
         past it go through the exact spill accumulators."""
         _lib.check(self._lib.dtb_engine_queue_capacity(self._h, int(floor), int(shift)))
 
-    LAYOUTS = {"auto": 0, "dense": 1, "spread": 2}
+    LAYOUTS = {"auto": 0, "dense": 1, "spread": 2, "gather": 3}
 
     def schedule(self, pass_k=None, layout=None):
         """Sets (when given) and returns the step kernel's schedule: synapses
-        per lane in a binning pass (1, 2, 4) and the queue-count layout
-        ("auto", "dense", "spread"); every schedule gives the same results.
+        per lane in a binning pass (1, 2, 4) and the propagation layout
+        ("dense" or "spread" queue counts, "gather" for every instance that
+        allows it, "auto"); every schedule gives the same results.
         Returns (pass_k, requested layout, layout in use)."""
         names = {v: k for k, v in self.LAYOUTS.items()}
         k, l, u = c_int32(0), c_int32(0), c_int32(0)
@@ -319,20 +320,33 @@ class Engine:
         """Times every schedule on about trial_steps real steps, the schedules
         taking turns in 4-step windows (autotune_steps(trial_steps) steps in
         all; they advance the simulation), and keeps the fastest; returns
-        (pass_k, layout, mean us per step as {(layout, pass_k): us})."""
+        (pass_k, layout, mean us per step as {(layout, pass_k): us}); the
+        gather schedule is a candidate (and in the table) only when some
+        instance allows it."""
         k, l = c_int32(0), c_int32(0)
-        us = np.zeros(6, dtype=np.float32)
+        us = np.zeros(7, dtype=np.float32)
         _lib.check(self._lib.dtb_engine_autotune(self._h, int(trial_steps), byref(k), byref(l), _ptr(us)))
         names = {v: n for n, v in self.LAYOUTS.items()}
         table = {(lay, kk): float(us[i * 3 + j]) for i, lay in enumerate(("dense", "spread"))
                  for j, kk in enumerate((1, 2, 4))}
+        if us[6] > 0:
+            table[("gather", 2)] = float(us[6])
         return k.value, names[l.value], table
 
     @staticmethod
     def autotune_steps(trial_steps):
         """Steps that autotune(trial_steps) runs."""
         rounds = (int(trial_steps) + 3) // 4
-        return 6 * (2 + rounds * 6)
+        return 7 * (2 + rounds * 6)
+
+    def propagation(self, inst):
+        """Gather propagation of an instance: {"allowed", "in_use",
+        "group_coverage"} (rows ordered by target group and at most 65535
+        long, offsets no larger than the synapses; in use under the current
+        layout; mean fraction of the 1024-neuron groups a row reaches)."""
+        a, u, c = c_int32(0), c_int32(0), c_float(0.0)
+        _lib.check(self._lib.dtb_instance_propagation(self._h, inst, byref(a), byref(u), byref(c)))
+        return {"allowed": bool(a.value), "in_use": bool(u.value), "group_coverage": float(c.value)}
 
     def tile_synapses(self, inst):
         """Synapses targeting each tile of 256 neurons of an instance."""

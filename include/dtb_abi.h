@@ -204,21 +204,35 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
 
 /* No counterpart in the reference (it has one schedule).  The step kernel's
  * schedule: pass_k synapses per lane in a binning pass (1, 2 or 4: passes of
- * 64, 128 or 256 synapses dealt to a tile's waves) and the layout of the
- * queue counts (DTB_LAYOUT_AUTO: by row coverage; DENSE: 32 per cache line;
- * SPREAD: one per line).  Every schedule gives bit-identical results. */
-enum { DTB_LAYOUT_AUTO = 0, DTB_LAYOUT_DENSE = 1, DTB_LAYOUT_SPREAD = 2 };
+ * 64, 128 or 256 synapses dealt to a tile's waves) and the propagation
+ * layout: queues whose counts are DENSE (32 per cache line) or SPREAD (one
+ * per line; DTB_LAYOUT_AUTO picks by row coverage), or GATHER (every
+ * instance that allows it gathers, per group of 1024 target neurons, the
+ * synapses of the last step's fired rows through per-row group offsets; the
+ * others use dense queues).  Every schedule gives bit-identical results.
+ * layout_in_use: GATHER when some instance gathers, else DENSE or SPREAD. */
+enum { DTB_LAYOUT_AUTO = 0, DTB_LAYOUT_DENSE = 1, DTB_LAYOUT_SPREAD = 2, DTB_LAYOUT_GATHER = 3 };
 int dtb_engine_schedule(dtb_engine_t h, int32_t pass_k, int32_t layout);
 int dtb_engine_get_schedule(dtb_engine_t h, int32_t *pass_k, int32_t *layout, int32_t *layout_in_use);
-/* Autotuner: every schedule (2 dense/spread x 3 pass sizes) runs about
- * trial_steps timed real steps of all instances, the schedules taking turns
- * in windows of 4 steps (each turn preceded by 2 steps that apply it, and one
- * untimed turn of each first), timed with HIP events; the fastest total is
- * kept.  Steps run in all: 6 (2 + r (2 + 4)), r = ceil(trial_steps / 4).
- * us_per_step[6] (optional) = mean per schedule, [dense K=1, 2, 4, spread
- * K=1, 2, 4].  The steps advance the simulation exactly as any other steps
- * would; tune in the regime that will be run (after the initial transient). */
+/* Autotuner: every schedule (2 dense/spread x 3 pass sizes, then gather with
+ * pass size 2) runs about trial_steps timed real steps of all instances, the
+ * schedules taking turns in windows of 4 steps (each turn preceded by 2 steps
+ * that apply it, and one untimed turn of each first), timed with HIP events;
+ * the fastest total is kept.  Gather is a candidate only if some instance
+ * allows it; otherwise its turns run the default schedule, untimed for the
+ * choice.  Steps run in all: 7 (2 + r (2 + 4)), r = ceil(trial_steps / 4).
+ * us_per_step[7] (optional) = mean per schedule, [dense K=1, 2, 4, spread
+ * K=1, 2, 4, gather] (gather 0 when not a candidate).  The steps advance the
+ * simulation exactly as any other steps would; tune in the regime that will
+ * be run (after the initial transient). */
 int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, int32_t *layout, float *us_per_step);
+/* No counterpart in the reference.  Gather propagation of one instance:
+ * whether it allows it (rows ordered by target group, rows of at most 65535
+ * synapses, group offsets no larger than the synapse arrays), whether the
+ * current layout makes it gather, and the mean fraction of the 1024-neuron
+ * groups that one row reaches (any pointer may be null). */
+int dtb_instance_propagation(dtb_engine_t h, int inst, int32_t *gather_allowed, int32_t *gather_in_use,
+                             float *group_coverage);
 int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses);
 /* Synapses targeting each tile of 256 neurons of an instance (the queues'
  * worst case), out[ceil(n / 256)]. */

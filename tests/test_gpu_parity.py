@@ -459,14 +459,23 @@ def _edge_network(dtb, case):
     return dtb.NetworkArrays.from_edges(props, pre[order], post[order], w[order], kind[order])
 
 
+@pytest.mark.parametrize("layout", ["auto", "gather"])
 @pytest.mark.parametrize("case,mu", [("single", 3.0), ("ragged", 2.0), ("hot_tile", 2.0)])
-def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu):
+def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu, layout):
     """A single neuron without synapses, ragged rows over a partial tile, and
-    all synapses into one tile: raster, V trace and final state bit-exact."""
+    all synapses into one tile, by queue and by gather: raster, V trace and
+    final state bit-exact."""
     net = _edge_network(dtb, case)
     steps, seed = 400, 99
     ids = np.arange(net.n, dtype=np.uint32)
-    out, st, cnt = run_gpu(dtb, net, steps, seed, mu, sample=ids)
+    eng = dtb.Engine(0)
+    try:
+        eng.schedule(layout=layout)
+        out, st, cnt = run_gpu(dtb, net, steps, seed, mu, sample=ids, engine=eng)
+        if layout == "gather" and net.nsyn > 0:
+            assert eng.propagation(0)["in_use"]
+    finally:
+        eng.close()
     ref, sim = run_oracle(orc, net, steps, seed, mu, vtrace=True)
     print(case, "n", net.n, "synapses", net.nsyn, "spikes", int(ref["raster"].sum()))
     assert ref["raster"].sum() > 0
@@ -580,9 +589,9 @@ def test_population_rates_over_10s_within_1pct_of_fp64(dtb, orc, gpu):
 
 
 @pytest.mark.parametrize("pass_k", [1, 2, 4])
-@pytest.mark.parametrize("layout", ["dense", "spread"])
+@pytest.mark.parametrize("layout", ["dense", "spread", "gather"])
 def test_every_schedule_bit_exact(dtb, orc, gpu, pass_k, layout):
-    """Every step-kernel schedule (binning pass size x queue-count layout)
+    """Every step-kernel schedule (binning pass size x propagation layout)
     gives the oracle's raster and state, also when the schedule changes
     between runs with spikes pending."""
     n, steps, seed = 10000, 200, 20240
@@ -623,7 +632,8 @@ def test_autotune_changes_nothing_but_the_schedule(dtb, gpu):
             if tune:
                 k, layout, table = eng.autotune(8)
                 print("autotune", k, layout, table)
-                assert k in (1, 2, 4) and layout in ("dense", "spread") and len(table) == 6
+                assert k in (1, 2, 4) and layout in ("dense", "spread", "gather")
+                assert ("gather", 2) in table and len(table) == 7
                 assert eng.schedule()[:2] == (k, layout)
                 eng.run(100, graph_chunk=0)
             else:
@@ -652,3 +662,126 @@ def test_device_generator_tile_counts(dtb, gpu, n, k):
         assert np.array_equal(eng.tile_synapses(inst), expect)
     finally:
         eng.close()
+
+
+def test_gather_stress_cut_bit_exact(dtb, orc, gpu):
+    """Gather propagation on the propagation stress cut (in-degree 2000 on
+    20k neurons, strong drive): rates and state equal the oracle's."""
+    n, steps, seed = 20_000, 300, 5
+    net = dtb.gen_uniform(n, 2000, seed=8)
+    eng = dtb.Engine(0)
+    try:
+        eng.schedule(layout="gather")
+        out, st, cnt = run_gpu(dtb, net, steps, seed, mu=8.0, sigma=0.5, engine=eng, graph=True)
+        prop = eng.propagation(0)
+        assert prop["allowed"] and prop["in_use"] and prop["group_coverage"] > 0.9
+        assert eng.schedule()[2] == "gather"
+    finally:
+        eng.close()
+    ref, sim = run_oracle(orc, net, steps, seed, mu=8.0, sigma=0.5)
+    assert np.array_equal(out["freq"], ref["freq"])
+    assert_state_equal(st, sim)
+    assert ref["raster"].sum() > 1000
+    deg = np.diff(net.rowptr)
+    assert cnt[1] == int((ref["raster"].astype(np.int64) * deg[None, :]).sum())
+
+
+def test_gather_mixed_engine_switching_and_checkpoint(dtb, orc, gpu):
+    """One engine holding an instance that gathers and one that cannot (its
+    group offsets would outweigh its synapses), switched between gather and
+    queues with spikes pending, sampled, read back and checkpointed: every
+    result equals the oracle (gathering instance) and the instance run alone
+    with queues (the other), bit for bit."""
+    na, nb, seed_a, seed_b = 8000, 200_000, 31, 32
+    net_a = dtb.gen_uniform(na, 150, seed=3)
+    net_b = dtb.gen_uniform(nb, 2, seed=4)
+    ids = np.arange(0, na, 3, dtype=np.uint32)
+    eng = dtb.Engine(0)
+    try:
+        eng.schedule(layout="gather")
+        a = dtb.Block(net_a, delta_t=DT, seed=seed_a, engine=eng)
+        b = dtb.Block(net_b, delta_t=DT, seed=seed_b, engine=eng)
+        a.update_ou_background_stimuli(a.subblk_id, 4.0, 1.5, 0.1)
+        b.update_ou_background_stimuli(b.subblk_id, 4.0, 9.0, 0.1)
+        a.set_samples(ids)
+        pa, pb = eng.propagation(a.inst), eng.propagation(b.inst)
+        assert pa["allowed"] and pa["in_use"] and not pb["allowed"] and not pb["in_use"]
+        parts = [a.run(120, output_sample_spike=True, output_sample_isynaptic=True)]
+        mid = a.state()                        # read back with spikes pending (gather)
+        eng.schedule(layout="dense")
+        assert not eng.propagation(a.inst)["in_use"]
+        parts.append(a.run(120, output_sample_spike=True, graph=True))
+        eng.schedule(layout="gather")
+        parts.append(a.run(120, output_sample_spike=True, graph=True))
+        snap = a.state()
+        tail = a.run(60, output_sample_spike=True)
+        st_a, st_b = a.state(), b.state()
+        cnt_b = b.counters()
+    finally:
+        eng.close()
+    steps = 360
+    ref, sim = run_oracle(orc, net_a, steps, seed_a, 1.5)
+    raster = np.concatenate([p["spike"] for p in parts])
+    assert np.array_equal(raster, ref["raster"][:, ids])
+    assert np.array_equal(np.concatenate([p["freq"] for p in parts]), ref["freq"])
+    assert_state_equal(snap, sim)
+    ref_mid, sim_mid = run_oracle(orc, net_a, 120, seed_a, 1.5)
+    assert_state_equal(mid, sim_mid)
+    # the sampled I_syn of the first part equals that of a run with queues
+    eng3 = dtb.Engine(0)
+    try:
+        a3 = dtb.Block(net_a, delta_t=DT, seed=seed_a, engine=eng3)
+        a3.update_ou_background_stimuli(a3.subblk_id, 4.0, 1.5, 0.1)
+        a3.set_samples(ids)
+        q0 = a3.run(120, output_sample_spike=True, output_sample_isynaptic=True)
+    finally:
+        eng3.close()
+    assert np.array_equal(bits(parts[0]["isynaptic"]), bits(q0["isynaptic"]))
+    assert np.abs(q0["isynaptic"]).max() > 0
+    # checkpoint taken under gather, resumed in a fresh gathering engine
+    eng2 = dtb.Engine(0)
+    try:
+        eng2.schedule(layout="gather")
+        a2 = dtb.Block(net_a, delta_t=DT, seed=seed_a, engine=eng2)
+        a2.update_ou_background_stimuli(a2.subblk_id, 4.0, 1.5, 0.1)
+        a2.set_samples(ids)
+        a2.run(17)
+        a2.load_state(snap)
+        tail2 = a2.run(60, output_sample_spike=True)
+        st_a2 = a2.state()
+    finally:
+        eng2.close()
+    assert np.array_equal(tail["spike"], tail2["spike"])
+    for k in ("V", "J", "i_ou", "I_syn"):
+        assert np.array_equal(bits(st_a[k]), bits(st_a2[k])), k
+    # the instance that cannot gather equals itself run alone with queues
+    out_b, st_b_alone, cnt_b_alone = run_gpu(dtb, net_b, steps + 60, seed_b, 9.0)
+    assert cnt_b == cnt_b_alone
+    for k in ("V", "J", "i_ou", "I_syn"):
+        assert np.array_equal(bits(st_b[k]), bits(st_b_alone[k])), k
+    assert cnt_b[2] > 0
+
+
+def test_gather_device_generated_stress_shape(dtb, gpu):
+    """A device-generated fan-out network at the stress shape's degree (in
+    5000 out of 60k neurons): gather and queue propagation give bit-identical
+    state, counters and spike counts over 200 strongly driven steps."""
+    n, k, nseed, seed, steps = 60_000, 5000, 11, 3, 200
+    res = []
+    for layout in ("dense", "gather"):
+        eng = dtb.Engine(0)
+        try:
+            eng.schedule(layout=layout)
+            blk = dtb.Block(None, delta_t=DT, engine=eng, inst=eng.add_fanout(n, k, nseed, dt=DT, seed=seed))
+            blk.update_ou_background_stimuli(blk.subblk_id, 4.0, 6.0, 0.5)
+            eng.configure(blk.inst, 1, True)
+            out = blk.run(steps, freqs=True, graph=True)
+            assert eng.propagation(blk.inst)["in_use"] == (layout == "gather")
+            res.append((out["freq"], blk.state(), blk.counters()))
+        finally:
+            eng.close()
+    (f0, s0, c0), (f1, s1, c1) = res
+    assert c0 == c1 and c0[2] > 10_000
+    assert np.array_equal(f0, f1)
+    for key in ("V", "J", "i_ou", "I_syn"):
+        assert np.array_equal(bits(s0[key]), bits(s1[key])), key
