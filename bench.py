@@ -41,16 +41,16 @@ NET_SEED, SIM_SEED = 2024, 7
 HBM_PEAK_GBS = 8000.0
 GRAPH_CHUNK = 100
 # The warm-up ends with the autotuner, in the firing regime that is timed:
-# the schedules (2 count layouts x 3 pass sizes) take turns in 4-step windows
-# until each has run about TRIAL timed steps, the fastest total is kept; then
-# GRAPH_CHUNK steps capture the graph.
+# the schedules (2 queue-count layouts x 3 pass sizes, and gather) take turns
+# in 4-step windows until each has run about TRIAL timed steps, the fastest
+# total is kept; then GRAPH_CHUNK steps capture the graph.
 AUTOTUNE_TRIAL = 32
-AUTOTUNE_STEPS = 6 * (2 + (AUTOTUNE_TRIAL + 3) // 4 * 6)
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of one k_step launch per workload (at
-# its default size), scaled by a device-copy calibration (each file gives
-# its commands and factors).
-TRAFFIC_PROFILE = {"block": os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md"),
-                   "stress": os.path.join(ROOT, "profiles", "k_step_uniform500k_in5000.md")}
+AUTOTUNE_STEPS = 7 * (2 + (AUTOTUNE_TRIAL + 3) // 4 * 6)   # Engine.autotune_steps(AUTOTUNE_TRIAL)
+# rocprofv3 FETCH_SIZE + WRITE_SIZE of one step's launches per workload and
+# propagation layout (at the workload's default size), scaled by a
+# device-copy calibration (each file gives its commands and factors).
+TRAFFIC_PROFILE = {("block", "dense"): os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md"),
+                   ("stress", "gather"): os.path.join(ROOT, "profiles", "k_gather_k_step_uniform500k_in5000.md")}
 
 # Algorithmic HBM bytes of one step, as the step is defined on per-neuron
 # arrays: per neuron 165 B (parameters 76 B: 7 scalars and g, V_u, decay of
@@ -69,12 +69,20 @@ ALGO_NEURON_BYTES, ALGO_ROW_BYTES, ALGO_SYN_BYTES = 165, 16, 12
 LAYOUT_NEURON_BYTES = 2 + 2 * 16 + 2 * 12 + 2 * 1
 LAYOUT_ROW_BYTES = 16
 LAYOUT_SYN_BYTES = 12 + 2 * 12
+# With gather propagation (k_gather, then k_step): per neuron the delta
+# (16 B) is written by k_gather and read by k_step; per fired row 16 B (row
+# offset, fired-list entry written and read) and its two 2-byte offsets for
+# each group of 2048 neurons; per delivered synapse 12 B (column index +
+# weight pair, no queue).
+GATHER_NEURON_BYTES = LAYOUT_NEURON_BYTES + 2 * 16
+GATHER_ROW_BYTES, GATHER_ROW_GROUP_BYTES = 16, 4
+GATHER_SYN_BYTES = 12
 
 
-def profiled_traffic(workload, neurons):
-    """HBM bytes per k_step launch measured with rocprofv3 counters on the
-    workload at its default size, or None."""
-    path = TRAFFIC_PROFILE.get(workload)
+def profiled_traffic(workload, neurons, layout):
+    """HBM bytes per step (its kernel launches) measured with rocprofv3
+    counters on the workload at its default size and this layout, or None."""
+    path = TRAFFIC_PROFILE.get((workload, layout))
     default = N_NEURONS if workload == "block" else STRESS_NEURONS
     if path is None or neurons != default or not os.path.exists(path):
         return None
@@ -361,9 +369,16 @@ def main():
     ev_step, sp_step = events / args.steps, spikes / args.steps
     us = ms_step * 1e3 / args.steps
     algo_bytes = ALGO_NEURON_BYTES * net.n + ALGO_ROW_BYTES * sp_step + ALGO_SYN_BYTES * ev_step
-    layout_bytes = LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step + LAYOUT_SYN_BYTES * ev_step
+    gather = layout_used == "gather"
+    if gather:
+        ngroups = (net.n + 2047) // 2048
+        layout_bytes = (GATHER_NEURON_BYTES * net.n + (GATHER_ROW_BYTES + GATHER_ROW_GROUP_BYTES * ngroups) * sp_step
+                        + GATHER_SYN_BYTES * ev_step)
+    else:
+        layout_bytes = LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step + LAYOUT_SYN_BYTES * ev_step
+    kname = "k_gather+k_step" if gather else "k_step"
     achieved = algo_bytes / (us * 1e-6) / 1e9
-    traffic = profiled_traffic(args.workload, args.neurons)
+    traffic = profiled_traffic(args.workload, args.neurons, layout_used)
 
     res = {
         "metric": "synaptic events/s (%s)" % ("2M-neuron block" if args.workload == "block" else
@@ -383,11 +398,11 @@ def main():
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,
         "events_per_step": events / args.steps,
-        "kernel_us_per_step": {"k_step": us},
-        "kernel_us_isolated": {"k_step": us_iso},
+        "kernel_us_per_step": {kname: us},
+        "kernel_us_isolated": {kname: us_iso},
         "schedule": schedule,
-        "kernel_algo_bytes_per_step": {"k_step": algo_bytes},
-        "roofline": {"kernel": "k_step", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "kernel_algo_bytes_per_step": {kname: algo_bytes},
+        "roofline": {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algo_bytes_per_launch": algo_bytes,
                      "layout_bytes_per_launch": layout_bytes,

@@ -722,11 +722,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 // end, and the wave then reads them with consecutive lanes on consecutive
 // synapses (a segment is contiguous in the row), adding each weight pair
 // into the group's Q23.40 LDS sums (64-bit integer atomics, so the result
-// does not depend on the order).  Also empties fired list par ^ 1 (the one
-// the k_step launch after this one fills; its last reader was the previous
-// k_gather).  Consecutive groups go to one XCD (workgroups are dealt to the
-// 8 XCDs round-robin by index), so the segments of neighbouring groups,
-// which share cache lines, meet in one L2.
+// does not depend on the order); each lane has kGatherItems synapses in
+// flight.  Groups of 2048 neurons give segments of about 20 synapses at the
+// stress shape (in-degree 5000 on 500k neurons), which is what coalesces the
+// reads; one 1024-thread workgroup per group.  Also empties fired list
+// par ^ 1 (the one the k_step launch after this one fills; its last reader
+// was the previous k_gather).  Consecutive groups go to one XCD (workgroups
+// are dealt to the 8 XCDs round-robin by index), so the segments of
+// neighbouring groups, which share cache lines, meet in one L2.
 __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__restrict__ groups,
                                                          const InstDev *__restrict__ insts, uint32_t par)
 {
@@ -774,12 +777,12 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t it = 0; it < total; it += 2u * kWave) {
-            uint32_t cv[2];
-            unsigned long long wb[2];
-            bool ok[2];
+        for (uint32_t it = 0; it < total; it += kGatherItems * kWave) {
+            uint32_t cv[kGatherItems];
+            unsigned long long wb[kGatherItems];
+            bool ok[kGatherItems];
 #pragma unroll
-            for (uint32_t u = 0; u < 2; ++u) {
+            for (uint32_t u = 0; u < kGatherItems; ++u) {
                 const uint32_t item = it + u * kWave + lane;
                 ok[u] = item < total;
                 const uint32_t ic = ok[u] ? item : total - 1u;
@@ -796,7 +799,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
                 wb[u] = wt[j];
             }
 #pragma unroll
-            for (uint32_t u = 0; u < 2; ++u) {
+            for (uint32_t u = 0; u < kGatherItems; ++u) {
                 if (ok[u]) {
                     const uint32_t sl = ((cv[u] & kColMask) - first) * 4u + ((cv[u] & kKindBit) ? 2u : 0u);
                     atomicAdd(&acc[sl], f32_to_q40(__uint_as_float((uint32_t)wb[u])));

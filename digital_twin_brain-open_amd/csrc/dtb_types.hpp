@@ -142,10 +142,12 @@ struct alignas(16) TileDesc {
 // Gather propagation: one workgroup of kGatherBlock threads gathers the
 // synapses of every fired row that target one group of kGroupTiles
 // consecutive tiles (2^kGroupBits neurons) into LDS, then writes their delta.
-constexpr uint32_t kGroupTiles = 4;
-constexpr uint32_t kGroupBits = kTileBits + 2;
+constexpr uint32_t kGroupTiles = 8;
+constexpr uint32_t kGroupBits = kTileBits + 3;
 static_assert((1u << (kGroupBits - kTileBits)) == kGroupTiles, "a group is kGroupTiles tiles");
-constexpr int kGatherBlock = 512;
+constexpr int kGatherBlock = 1024;
+// Synapses each lane of k_gather loads before it adds them (in flight together).
+constexpr uint32_t kGatherItems = 8;
 // Longest row the 16-bit group offsets can index.
 constexpr int64_t kGatherMaxRow = 65535;
 

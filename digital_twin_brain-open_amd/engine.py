@@ -343,7 +343,7 @@ class Engine:
         """Gather propagation of an instance: {"allowed", "in_use",
         "group_coverage"} (rows ordered by target group and at most 65535
         long, offsets no larger than the synapses; in use under the current
-        layout; mean fraction of the 1024-neuron groups a row reaches)."""
+        layout; mean fraction of the 2048-neuron groups a row reaches)."""
         a, u, c = c_int32(0), c_int32(0), c_float(0.0)
         _lib.check(self._lib.dtb_instance_propagation(self._h, inst, byref(a), byref(u), byref(c)))
         return {"allowed": bool(a.value), "in_use": bool(u.value), "group_coverage": float(c.value)}

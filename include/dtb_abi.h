@@ -207,7 +207,7 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
  * 64, 128 or 256 synapses dealt to a tile's waves) and the propagation
  * layout: queues whose counts are DENSE (32 per cache line) or SPREAD (one
  * per line; DTB_LAYOUT_AUTO picks by row coverage), or GATHER (every
- * instance that allows it gathers, per group of 1024 target neurons, the
+ * instance that allows it gathers, per group of 2048 target neurons, the
  * synapses of the last step's fired rows through per-row group offsets; the
  * others use dense queues).  Every schedule gives bit-identical results.
  * layout_in_use: GATHER when some instance gathers, else DENSE or SPREAD. */
@@ -229,7 +229,7 @@ int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, in
 /* No counterpart in the reference.  Gather propagation of one instance:
  * whether it allows it (rows ordered by target group, rows of at most 65535
  * synapses, group offsets no larger than the synapse arrays), whether the
- * current layout makes it gather, and the mean fraction of the 1024-neuron
+ * current layout makes it gather, and the mean fraction of the 2048-neuron
  * groups that one row reaches (any pointer may be null). */
 int dtb_instance_propagation(dtb_engine_t h, int inst, int32_t *gather_allowed, int32_t *gather_in_use,
                              float *group_coverage);
