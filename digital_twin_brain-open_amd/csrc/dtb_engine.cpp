@@ -274,9 +274,16 @@ void Instance::count_tile_synapses(const NetworkHost &net)
     gather_ok_ = ordered && nsyn_ > 0 && (double)n_ * (ngroups() + 1.0) * 2.0 <= 12.0 * (double)nsyn_;
 }
 
+uint32_t Instance::fired_capacity() const
+{
+    const uint32_t ntile = (n_ + kBlock - 1) / kBlock;
+    return (ntile + kFiredLists - 1) / kFiredLists * kBlock;
+}
+
 size_t Instance::gather_bytes() const
 {
-    return (size_t)n_ * (ngroups() + 1u) * sizeof(uint16_t) + 2u * (size_t)n_ * sizeof(uint32_t) + 64u;
+    return (size_t)n_ * (ngroups() + 1u) * sizeof(uint16_t) +
+           2u * kFiredLists * ((size_t)fired_capacity() + kFiredCountStride) * sizeof(uint32_t);
 }
 
 void Instance::ensure_gather(hipStream_t s)
@@ -285,9 +292,9 @@ void Instance::ensure_gather(hipStream_t s)
         throw Error(kInvalidParameter, "this network does not allow gather propagation");
     if (segidx_.count() != 0)
         return;
-    fcount_.resize(2);
+    fcount_.resize(2 * (size_t)kFiredLists * kFiredCountStride);
     fcount_.mutable_gpu_data();
-    fired_.resize(2 * (size_t)n_);
+    fired_.resize(2 * (size_t)kFiredLists * fired_capacity());
     fired_.mutable_gpu_data();
     segidx_.resize((size_t)n_ * (ngroups() + 1u));
     DTB_HIP(launch_build_segidx(rowptr_.gpu_data(), col_.gpu_data(), n_, ngroups(), segidx_.mutable_gpu_data(), s));
@@ -832,6 +839,7 @@ InstDev Instance::dev(uint32_t tile_base, bool gather)
         d.fired = fired_.mutable_gpu_data();
         d.fcount = fcount_.mutable_gpu_data();
         d.ngroups = ngroups();
+        d.fcap = fired_capacity();
     }
     d.pA = pA_.gpu_data();
     d.pB = pB_.gpu_data();

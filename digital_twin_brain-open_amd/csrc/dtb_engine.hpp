@@ -311,6 +311,8 @@ public:
     uint32_t ngroups() const { return (n_ + (1u << kGroupBits) - 1) >> kGroupBits; }
     // Device bytes of the gather buffers (group offsets, fired lists).
     size_t gather_bytes() const;
+    // Rows per fired list (the tiles of one list hold at most this many).
+    uint32_t fired_capacity() const;
     // Builds the group offsets and fired lists if not built yet.
     void ensure_gather(hipStream_t s);
     // Empties both fired lists (checkpoint restore).
@@ -367,8 +369,8 @@ private:
     double group_coverage_ = 0.0;         // mean fraction of the gather groups one row reaches
     bool gather_ok_ = false;              // gather_allowed()
     DataAllocator<uint16_t> segidx_;      // [n][ngroups + 1] group offsets (built on demand)
-    DataAllocator<uint32_t> fired_;       // [2][n] fired rows, by parity
-    DataAllocator<uint32_t> fcount_;      // [2]
+    DataAllocator<uint32_t> fired_;       // [2][kFiredLists][fired_capacity()] fired rows
+    DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]
     uint32_t gamma_calls_ = 0;            // gamma draw streams used
     std::vector<uint32_t> tile_cls_;      // per tile: 1 + shared parameter class, or 0
 };

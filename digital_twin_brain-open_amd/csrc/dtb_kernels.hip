@@ -626,18 +626,20 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
                 base = atomicAdd(&nrows, cnt_w);
         }
         if (gat) {
-            // gather propagation: the rows with synapses go to the instance's
-            // fired list for the next launch's k_gather (one atomic per wave)
+            // gather propagation: the rows with synapses go to this tile's
+            // fired list of set par ^ 1 for the next launch's k_gather (one
+            // atomic per wave, on a count shared with 1/64 of the tiles)
             const bool has = emit && len != 0u;
             const unsigned long long mg = __ballot(has);
             if (mg != 0ull) {
+                const uint32_t list = (par ^ 1u) * kFiredLists + ((d.begin >> kTileBits) & (kFiredLists - 1u));
                 const int lg = __ffsll((long long)mg) - 1;
                 uint32_t gb = 0;
                 if ((int)lane == lg)
-                    gb = atomicAdd(I.fcount + (par ^ 1u), (uint32_t)__popcll(mg));
+                    gb = atomicAdd(I.fcount + (size_t)list * kFiredCountStride, (uint32_t)__popcll(mg));
                 gb = (uint32_t)__builtin_amdgcn_readlane((int)gb, lg);
                 if (has)
-                    gp(I.fired)[(size_t)(par ^ 1u) * I.n + gb + (uint32_t)__popcll(mg & ((1ull << lane) - 1ull))] = li;
+                    gp(I.fired)[(size_t)list * I.fcap + gb + (uint32_t)__popcll(mg & ((1ull << lane) - 1ull))] = li;
             }
         } else {
             base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
@@ -738,6 +740,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
     __shared__ unsigned long long acc[kNeurons * 4];
     __shared__ uint32_t s_off[kWaves][kWave];             // per wave: first item of each row's segment
     __shared__ unsigned long long s_base[kWaves][kWave];  // and the segment's first synapse
+    __shared__ uint32_t s_list[kWaves][kWave];            // per wave: first row of each fired list
     const uint32_t nb = gridDim.x, x = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const GroupDesc d = groups[x * (nb >> 3) + min(x, nb & 7u) + slot];
     const InstDev &I = insts[d.inst];
@@ -746,11 +749,21 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
     const uint32_t wv = tid / kWave;
     for (uint32_t i = tid; i < kNeurons * 4; i += kGatherBlock)
         acc[i] = 0ull;
-    const uint32_t nf = gp(I.fcount)[par];
-    if (d.g == 0 && tid == 0)
-        gp(I.fcount)[par ^ 1u] = 0u;
+    // the fired lists of set par laid end to end: lane l holds list l's count
+    const uint32_t lcnt = gp(I.fcount)[((size_t)par * kFiredLists + lane) * kFiredCountStride];
+    uint32_t lincl = lcnt;
+#pragma unroll
+    for (uint32_t o = 1; o < (uint32_t)kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(lincl, o, kWave);
+        if (lane >= o)
+            lincl += v;
+    }
+    const uint32_t nf = (uint32_t)__builtin_amdgcn_readlane((int)lincl, kWave - 1);
+    s_list[wv][lane] = lincl - lcnt;
+    if (d.g == 0 && tid < kFiredLists)
+        gp(I.fcount)[((size_t)(par ^ 1u) * kFiredLists + tid) * kFiredCountStride] = 0u;
     __syncthreads();
-    const auto fired = gp(I.fired) + (size_t)par * I.n;
+    const auto fired = gp(I.fired) + (size_t)par * kFiredLists * I.fcap;
     const auto col = gp(I.col);
     const auto wt = gp(reinterpret_cast<const unsigned long long *>(I.w));
     const uint32_t stride = I.ngroups + 1u;
@@ -758,7 +771,14 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
     for (uint32_t c0 = wv * kWave; c0 < nf; c0 += kGatherBlock) {
         const uint32_t k = c0 + lane;
         const bool valid = k < nf;
-        const uint32_t r = fired[valid ? k : nf - 1u];
+        const uint32_t kc = valid ? k : nf - 1u;
+        // the list holding row kc: the last list starting at or before it
+        uint32_t ls = 0;
+#pragma unroll
+        for (uint32_t st = kWave / 2; st > 0; st >>= 1)
+            if (s_list[wv][ls + st] <= kc)
+                ls += st;
+        const uint32_t r = fired[(size_t)ls * I.fcap + (kc - s_list[wv][ls])];
         const auto sg = gp(I.segidx) + (size_t)r * stride + d.g;
         const uint32_t a = sg[0];
         const uint32_t e = sg[1];
@@ -896,8 +916,8 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
                 gp(I.flag)[li] = (uint8_t)(fl | kFlagFolded);
             }
         }
-        if (d.begin == 0 && tid == 0)
-            gp(I.fcount)[set] = 0u;
+        if (d.begin == 0 && tid < kFiredLists)
+            gp(I.fcount)[((size_t)set * kFiredLists + tid) * kFiredCountStride] = 0u;
         return;
     }
     uint32_t *cnt_p = qs.count + count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift);

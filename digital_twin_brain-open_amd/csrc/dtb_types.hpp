@@ -96,11 +96,14 @@ struct alignas(16) InstDev {
     const float2 *w;            // [S] weight pair
     // Gather propagation (null: queue propagation).  segidx[r][g] = offset in
     // row r of its first synapse whose target lies in group g or later
-    // (segidx[r][ngroups] = row length); fired[p][] lists the rows with
-    // synapses fired by the launch of parity p ^ 1, fcount[p] of them.
+    // (segidx[r][ngroups] = row length).  The rows with synapses fired by the
+    // launch of parity p ^ 1 are in the kFiredLists lists of set p: list s
+    // takes the rows of tiles t with t % kFiredLists == s, so it never holds
+    // more than fcap rows; its count is word (p * kFiredLists + s) *
+    // kFiredCountStride of fcount (one cache line per count).
     const uint16_t *segidx;     // [n][ngroups + 1]
-    uint32_t *fired;            // [2][n]
-    uint32_t *fcount;           // [2]
+    uint32_t *fired;            // [2][kFiredLists][fcap]
+    uint32_t *fcount;           // [2][kFiredLists][kFiredCountStride]
     const PopDev *pops;         // [npop]
     // recording (device buffers, row = step - rec_step0)
     uint32_t *rec_counts;       // [rec_len][npop] spikes per population
@@ -125,6 +128,8 @@ struct alignas(16) InstDev {
     uint32_t stat_mask;         // kStatV | kStatI | kStatReceptor: statistics recorded in rec_stat
     uint32_t pmask;             // parameter record of neuron i: (i & pmask) + cls[i]
     uint32_t ngroups;           // gather groups of the instance (kGroupTiles tiles each)
+    uint32_t fcap;              // rows per fired list
+    uint32_t pad;
 };
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
@@ -150,6 +155,11 @@ constexpr int kGatherBlock = 1024;
 constexpr uint32_t kGatherItems = 8;
 // Longest row the 16-bit group offsets can index.
 constexpr int64_t kGatherMaxRow = 65535;
+// Fired lists per set (appends spread over this many counters, one wave of
+// k_gather reads one count per lane) and the words between their counts.
+constexpr uint32_t kFiredLists = 64;
+constexpr uint32_t kFiredCountStride = 32;
+static_assert(kFiredLists == 64, "k_gather reads one list count per lane");
 
 struct alignas(16) GroupDesc {
     uint32_t inst;
