@@ -114,11 +114,13 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
         eng.configure(b.inst, 1, True)
         return b
 
-    trial = 16
+    trial = 32
     tuned_steps = dtb.Engine.autotune_steps(trial)
 
     def tune(eng):
-        # warm-up into the firing regime, autotune there, capture the graph
+        # warm-up into the firing regime (the first ~300 steps are the
+        # transient, whose costs rank the schedules differently), autotune
+        # there, capture the graph
         if warmup >= tuned_steps + 2 * chunk:
             eng.run(warmup - tuned_steps - chunk, graph_chunk=0)
             eng.autotune(trial)
@@ -420,7 +422,7 @@ def main():
     del blk, net
     if args.ensemble_steps > 0 and args.workload == "block":
         t0 = time.perf_counter()
-        res["ensemble"] = ensemble_bench(dtb, args.ensemble_steps, min(args.warmup, 500))
+        res["ensemble"] = ensemble_bench(dtb, args.ensemble_steps, min(args.warmup, 800))
         log("ensemble %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(res), flush=True)
 
