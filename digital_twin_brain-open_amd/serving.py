@@ -32,6 +32,7 @@ from .engine import DEFAULT_G_UI, Block, Engine, device_memory_reserve
 BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 64 + 1 + 8 + 1
 BYTES_PER_SYNAPSE = 4 + 8
 QUEUE_ENTRY_BYTES, QUEUE_FLOOR, QUEUE_SHIFT = 12, 4096, 3
+GATHER_GROUP, FIRED_LISTS = 2048, 64
 
 
 class Request:
@@ -46,11 +47,18 @@ class Request:
         self.events = self.spikes = None
         self.v_hash = self.v_mean = None
 
-    def device_bytes(self):
+    def device_bytes(self, gather=False):
+        """Device footprint; gather: also the gather buffers (2-byte offsets
+        per row and group of 2048 neurons, 64 fired lists per parity)."""
         syn = self.n * self.k
         tiles = (self.n + 255) // 256
         queue = min(syn, tiles * QUEUE_FLOOR + (syn >> QUEUE_SHIFT))   # bound on the sum over tiles
-        return self.n * BYTES_PER_NEURON + syn * BYTES_PER_SYNAPSE + 2 * QUEUE_ENTRY_BYTES * queue
+        b = self.n * BYTES_PER_NEURON + syn * BYTES_PER_SYNAPSE + 2 * QUEUE_ENTRY_BYTES * queue
+        if gather:
+            groups = (self.n + GATHER_GROUP - 1) // GATHER_GROUP
+            fcap = (tiles + FIRED_LISTS - 1) // FIRED_LISTS * 256
+            b += self.n * (groups + 1) * 2 + 2 * FIRED_LISTS * (fcap + 32) * 4
+        return b
 
     def latency(self):
         return self.finished - self.arrival
@@ -83,8 +91,13 @@ def device_free_bytes(device=0):
 class Server:
     """Continuous batching of simulation requests on one engine."""
 
-    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1, reserve=True):
+    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1, reserve=True, layout="auto"):
+        """layout: the engine's propagation layout (Engine.schedule); with
+        "gather" every request propagates by gather and its footprint counts
+        the gather buffers."""
         self.engine = Engine(device)
+        self.engine.schedule(layout=layout)
+        self.gather = layout == "gather"
         self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes(device))
         self.chunk = int(chunk)
         self.dt = dt
@@ -121,8 +134,8 @@ class Server:
         are filled into the Request objects.  progress: seconds between
         one-line reports on stderr (None: silent)."""
         for r in requests:
-            if r.device_bytes() > self.budget:
-                raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.device_bytes()))
+            if r.device_bytes(self.gather) > self.budget:
+                raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.device_bytes(self.gather)))
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active = {}    # inst -> (request, block, remaining steps)
         used = 0
@@ -139,7 +152,7 @@ class Server:
             nonlocal used, t_admit, wait_retire
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
             while (pending and not wait_retire and pending[0].arrival - offset <= now
-                   and used + pending[0].device_bytes() <= self.budget
+                   and used + pending[0].device_bytes(self.gather) <= self.budget
                    and (max_resident is None or len(active) + len(joining) < max_resident)):
                 r = pending.popleft()
                 ta = time.monotonic()
@@ -158,7 +171,7 @@ class Server:
                 t_admit += time.monotonic() - ta
                 r.admitted = time.monotonic() - t0 + offset
                 joining[b.inst] = [r, b, r.steps]
-                used += r.device_bytes()
+                used += r.device_bytes(self.gather)
 
         joining = {}   # admitted while a chunk runs: they step from the next chunk on
         while pending or active or joining:
@@ -200,7 +213,7 @@ class Server:
                 self._retire(r, b)
                 t_retire += time.monotonic() - ta
                 r.finished = time.monotonic() - t0 + offset
-                used -= r.device_bytes()
+                used -= r.device_bytes(self.gather)
                 done_count += 1
                 wait_retire = False
         wall = time.monotonic() - t0

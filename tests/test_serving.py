@@ -22,15 +22,18 @@ def test_request_stream_is_seeded(dtb, serving):
 
 
 @pytest.mark.gpu
-def test_results_same_alone_and_in_the_mix(dtb, gpu, serving):
+@pytest.mark.parametrize("layout", ["auto", "gather"])
+def test_results_same_alone_and_in_the_mix(dtb, gpu, serving, layout):
     """Requests admitted while others are resident, under a budget that
     forces waiting, retire with exactly the counters and final V they have
-    when run alone."""
+    when run alone (alone: queue propagation; in the mix: queues or
+    gather)."""
     reqs = [serving.Request(i, 0.01 * i, n, k, steps, net_seed=100 + i, seed=7 + i, mu=3.0)
             for i, (n, k, steps) in enumerate([(30000, 200, 700), (12000, 150, 300), (50000, 300, 450),
                                                (20000, 120, 900), (9000, 500, 250), (40000, 250, 600)])]
-    budget = sum(sorted(r.device_bytes() for r in reqs)[-3:])   # at most about three resident at once
-    srv = serving.Server(device=0, hbm_budget=budget, chunk=64)
+    gather = layout == "gather"
+    budget = sum(sorted(r.device_bytes(gather) for r in reqs)[-3:])   # at most about three resident at once
+    srv = serving.Server(device=0, hbm_budget=budget, chunk=64, layout=layout)
     try:
         stats = srv.serve(reqs, realtime=False)
     finally:
