@@ -133,6 +133,12 @@ public:
         check(dtb_counters(eng_, inst(), &step, &events, &spikes));
     }
 
+    // No counterpart in the reference (it has one schedule): the step
+    // schedule (dtb_engine_schedule: pass size 1/2/4, DTB_LAYOUT_*) and the
+    // autotuner (dtb_engine_autotune; its trial steps advance the block).
+    void set_schedule(int32_t pass_k, int32_t layout) { check(dtb_engine_schedule(eng_, pass_k, layout)); }
+    void autotune(int64_t trial_steps) { check(dtb_engine_autotune(eng_, trial_steps, nullptr, nullptr, nullptr)); }
+
     dtb_engine_t engine() const { return eng_; }
     int instance() const { return inst_; }
 

@@ -2,7 +2,8 @@
 // generates the seeded 80/20 network of dtb_gen_uniform, sets the OU
 // background of every population, runs `steps` steps with per-population
 // counts, and prints "counts <step> <events> <spikes> <sum of freq> <xor of
-// V bits>".  Without a GPU it prints "no device <status>".
+// V bits>".  Without a GPU it prints "no device <status>".  Arguments: n, k,
+// steps, propagation layout (DTB_LAYOUT_*, default auto).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -15,6 +16,7 @@ int main(int argc, char **argv)
     const int64_t n = argc > 1 ? std::atoll(argv[1]) : 3000;
     const uint32_t k = argc > 2 ? (uint32_t)std::atoi(argv[2]) : 100;
     const int32_t steps = argc > 3 ? std::atoi(argv[3]) : 200;
+    const int32_t layout = argc > 4 ? std::atoi(argv[4]) : DTB_LAYOUT_AUTO;
     const float g[4] = {0.01f, 0.0005f, 0.1f, 0.005f};
     std::vector<float> props((size_t)n * 22), w((size_t)n * k * 2);
     std::vector<int64_t> rowptr((size_t)n + 1);
@@ -22,6 +24,7 @@ int main(int argc, char **argv)
     dtb::check(dtb_gen_uniform(n, k, 1, g, 0, props.data(), rowptr.data(), col.data(), w.data(), kinds.data()));
     try {
         dtb::Block blk(0, 0.1f, 11);
+        blk.set_schedule(2, layout);
         const dtb_network net{n, props.data(), n * (int64_t)k, rowptr.data(), col.data(), w.data(), kinds.data()};
         blk.from_network(net);
         const unsigned int npop = blk.get_total_subblocks();

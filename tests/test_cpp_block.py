@@ -31,10 +31,12 @@ def test_cpp_block_builds_and_reports_no_device(dtb, tmp_path):
 
 
 @pytest.mark.gpu
-def test_cpp_block_steps_like_python(dtb, gpu, tmp_path):
+@pytest.mark.parametrize("layout", [0, 3])   # DTB_LAYOUT_AUTO, DTB_LAYOUT_GATHER
+def test_cpp_block_steps_like_python(dtb, gpu, tmp_path, layout):
     exe = _build(dtb, tmp_path)
     n, k, steps = 3000, 100, 200
-    out = subprocess.run([str(exe), str(n), str(k), str(steps)], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([str(exe), str(n), str(k), str(steps), str(layout)], capture_output=True, text=True,
+                         timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     fields = out.stdout.split()
     assert fields[0] == "counts"
