@@ -290,20 +290,30 @@ void Instance::ensure_gather(hipStream_t s)
 {
     if (!gather_ok_)
         throw Error(kInvalidParameter, "this network does not allow gather propagation");
-    if (segidx_.count() != 0)
+    if (gather_built_)
         return;
-    fcount_.resize(2 * (size_t)kFiredLists * kFiredCountStride);
-    fcount_.mutable_gpu_data();
-    fired_.resize(2 * (size_t)kFiredLists * fired_capacity());
-    fired_.mutable_gpu_data();
-    segidx_.resize((size_t)n_ * (ngroups() + 1u));
-    DTB_HIP(launch_build_segidx(rowptr_.gpu_data(), col_.gpu_data(), n_, ngroups(), segidx_.mutable_gpu_data(), s));
-    DTB_HIP(hipStreamSynchronize(s));
+    try {
+        fcount_.resize(2 * (size_t)kFiredLists * kFiredCountStride);
+        fcount_.mutable_gpu_data();
+        fired_.resize(2 * (size_t)kFiredLists * fired_capacity());
+        fired_.mutable_gpu_data();
+        segidx_.resize((size_t)n_ * (ngroups() + 1u));
+        DTB_HIP(launch_build_segidx(rowptr_.gpu_data(), col_.gpu_data(), n_, ngroups(), segidx_.mutable_gpu_data(),
+                                    s));
+        DTB_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        // out of device memory (or a failed build): nothing half-built stays
+        segidx_.resize(0);
+        fired_.resize(0);
+        fcount_.resize(0);
+        throw;
+    }
+    gather_built_ = true;
 }
 
 void Instance::clear_fired(hipStream_t s)
 {
-    if (fcount_.count() != 0)
+    if (gather_built_)
         DTB_HIP(hipMemsetAsync(fcount_.mutable_gpu_data(), 0, fcount_.size(), s));
 }
 
@@ -833,7 +843,7 @@ InstDev Instance::dev(uint32_t tile_base, bool gather)
 {
     InstDev d{};
     if (gather) {
-        if (segidx_.count() == 0)
+        if (!gather_built_)
             throw Error(kInvalidParameter, "gather buffers not built");
         d.segidx = segidx_.gpu_data();
         d.fired = fired_.mutable_gpu_data();

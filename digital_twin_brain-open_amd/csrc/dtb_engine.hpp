@@ -368,6 +368,7 @@ private:
     double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
     double group_coverage_ = 0.0;         // mean fraction of the gather groups one row reaches
     bool gather_ok_ = false;              // gather_allowed()
+    bool gather_built_ = false;           // segidx_, fired_, fcount_ built
     DataAllocator<uint16_t> segidx_;      // [n][ngroups + 1] group offsets (built on demand)
     DataAllocator<uint32_t> fired_;       // [2][kFiredLists][fired_capacity()] fired rows
     DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]

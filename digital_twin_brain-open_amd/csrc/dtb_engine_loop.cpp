@@ -122,10 +122,10 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
     static const uint32_t ks[7] = {1, 2, 4, 1, 2, 4, kPassKDefault};
     static const int layouts[7] = {kLayoutDense, kLayoutDense, kLayoutDense, kLayoutSpread,
                                    kLayoutSpread, kLayoutSpread, kLayoutGather};
-    // Without an instance that allows gather the seventh turn runs the
-    // default schedule and is not a candidate, so the number of steps does
-    // not depend on the networks.
-    const int nc = gather_possible() ? 7 : 6;
+    // Without an instance that allows gather (or device memory for its
+    // buffers) the seventh turn runs the default schedule and is not a
+    // candidate, so the number of steps does not depend on the networks.
+    int nc = gather_possible() ? 7 : 6;
     // The cost of a step follows the network's activity, which drifts and
     // bursts; so the schedules take turns in short windows (kAutotuneWindow
     // timed steps, after 2 steps that apply the schedule) over the whole
@@ -139,7 +139,17 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
                 set_schedule(ks[s], layouts[s]);
             else
                 set_schedule(kPassKDefault, kLayoutDense);
-            run(2, 0);   // applies the schedule (rebuild) outside the timed span
+            try {
+                run(2, 0);   // applies the schedule (rebuild) outside the timed span
+            } catch (const Error &e) {
+                // the gather buffers did not fit: drop the candidate (the
+                // failed table build is redone by the next run)
+                if (e.code != kOutOfMemory || layouts[s] != kLayoutGather || s >= nc)
+                    throw;
+                nc = 6;
+                set_schedule(kPassKDefault, kLayoutDense);
+                run(2, 0);
+            }
             if (r < 0)
                 continue;
             float ms = 0.0f;
