@@ -27,6 +27,11 @@
 // The queues are double-buffered: the launch of parity p drains set p and
 // fills set p ^ 1, so step k consumes exactly the synapses queued from the
 // spikes of step k - 1.  No global atomics touch the delta.
+//   With the gather schedule (instances with group offsets) a k_gather
+//   launch precedes each k_step: per group of 2048 target neurons it gathers
+//   the last step's fired rows into the same exact LDS sums and writes the
+//   delta, which k_step folds instead of draining a queue; k_step lists its
+//   fired rows for the next k_gather instead of binning them.
 //   k_materialize writes the pending delta of every neuron (without consuming
 //   the queues) for sampling and state read-back; k_sample gathers samples.
 #include <hip/hip_ext.h>
