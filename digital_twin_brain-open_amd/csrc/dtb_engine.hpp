@@ -377,8 +377,9 @@ private:
 };
 
 // Steps a set of instances together.  One launch per step covers every
-// neuron of every active instance (one workgroup per tile of 256 neurons).
-// The step loop can be captured into HIP graphs (one per starting parity).
+// neuron of every active instance (one workgroup per tile of 256 neurons),
+// preceded by one k_gather launch when some instance gathers.  The step
+// loop can be captured into HIP graphs (one per starting parity).
 class Engine {
 public:
     explicit Engine(int device);

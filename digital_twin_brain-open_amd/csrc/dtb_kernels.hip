@@ -10,7 +10,8 @@
 //   J = J * decay + delta (presynaptic_voltage.cpp:46-120), I_syn
 //             (synaptic_current.cpp:7-105)
 //
-// One launch per step here (k_step).  Every workgroup of 256 threads owns a
+// One launch per step here (k_step; see the end of this comment for the
+// gather schedule's second launch).  Every workgroup of 256 threads owns a
 // tile of 256 consecutive neurons of one instance, and per tile:
 //   1. reads the tile's queue (the synapses delivered to it: target index,
 //      channel pair, weight pair; contiguous, so the reads coalesce) and adds
