@@ -447,6 +447,34 @@ def _edge_network(dtb, case):
         n = 2048
         pre = np.repeat(np.arange(n), 64)
         post = rng.integers(0, 256, pre.size)
+    elif case == "ragged_groups":
+        # three gather groups of 2048 neurons, the last partial: a row of
+        # 65535 synapses (the longest the 16-bit group offsets index), a row
+        # onto every neuron, rows only into the last group or only into the
+        # middle one, one-synapse rows, ragged rows and many empty rows
+        n = 5000
+        pre, post = [], []
+        pre += [7] * 65535
+        post += list(rng.integers(0, n, 65535))
+        pre += [13] * n
+        post += list(range(n))
+        pre += [11] * 40
+        post += list(rng.integers(4096, n, 40))
+        pre += [12]
+        post += [2100]
+        pre += [14] * 3
+        post += [2047, 2048, 4095]
+        for r in range(100, 3000, 7):
+            k = int(rng.integers(1, 300))
+            pre += [r] * k
+            post += list(rng.integers(0, n, k))
+        pre, post = np.asarray(pre), np.asarray(post)
+    elif case == "long_row":
+        # one row longer than the group offsets can index: gather is not
+        # allowed and the queues carry it
+        n = 3000
+        pre = np.concatenate([np.full(70000, 5), np.repeat(np.arange(100, 3000, 3), 20)])
+        post = rng.integers(0, n, pre.size)
     else:
         raise ValueError(case)
     props = dtb.gen_uniform(max(n, 2), 1, seed=3).props[:n]
@@ -460,11 +488,13 @@ def _edge_network(dtb, case):
 
 
 @pytest.mark.parametrize("layout", ["auto", "gather"])
-@pytest.mark.parametrize("case,mu", [("single", 3.0), ("ragged", 2.0), ("hot_tile", 2.0)])
+@pytest.mark.parametrize("case,mu", [("single", 3.0), ("ragged", 2.0), ("hot_tile", 2.0), ("ragged_groups", 2.0),
+                                     ("long_row", 2.0)])
 def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu, layout):
-    """A single neuron without synapses, ragged rows over a partial tile, and
-    all synapses into one tile, by queue and by gather: raster, V trace and
-    final state bit-exact."""
+    """A single neuron without synapses, ragged rows over a partial tile, all
+    synapses into one tile, ragged rows over partial gather groups (up to the
+    longest row the group offsets index), a row too long for them; by queue
+    and by gather: raster, V trace and final state bit-exact."""
     net = _edge_network(dtb, case)
     steps, seed = 400, 99
     ids = np.arange(net.n, dtype=np.uint32)
@@ -472,8 +502,9 @@ def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu, layout):
     try:
         eng.schedule(layout=layout)
         out, st, cnt = run_gpu(dtb, net, steps, seed, mu, sample=ids, engine=eng)
-        if layout == "gather" and net.nsyn > 0:
-            assert eng.propagation(0)["in_use"]
+        prop = eng.propagation(0)
+        assert prop["allowed"] == (net.nsyn > 0 and case != "long_row")
+        assert prop["in_use"] == (layout == "gather" and prop["allowed"])
     finally:
         eng.close()
     ref, sim = run_oracle(orc, net, steps, seed, mu, vtrace=True)
