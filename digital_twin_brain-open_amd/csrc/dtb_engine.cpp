@@ -200,14 +200,19 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
         throw Error(kInvalidNetwork, "too many synapses for one block");
     n_ = (uint32_t)f.n;
     nsyn_ = f.n * (int64_t)f.K;
-    std::vector<float> props((size_t)n_ * kNumCols);
-    gen_default_props(f.n, f.g_ui, f.sub_base, props.data());
-    init_common(props.data(), dt, seed, s);
-
+    // The synapses are generated on the device while the host builds the
+    // property matrix and parameter records (init_common's uploads follow
+    // the generation on the same stream).
+    rowptr_.resize((size_t)n_ + 1);
+    col_.resize((size_t)nsyn_);
+    w_.resize((size_t)nsyn_);
     const uint32_t ntile = (n_ + (1u << kTileBits) - 1) >> kTileBits;
     DataAllocator<uint32_t> tsyn(ntile);
     DTB_HIP(launch_gen_fanout(n_, f.K, f.net_seed, rowptr_.mutable_gpu_data(), col_.mutable_gpu_data(),
                               w_.mutable_gpu_data(), tsyn.mutable_gpu_data(), s));
+    std::vector<float> props((size_t)n_ * kNumCols);
+    gen_default_props(f.n, f.g_ui, f.sub_base, props.data());
+    init_common(props.data(), dt, seed, s);
     tile_syn_.assign(ntile, 0u);
     tsyn.download(tile_syn_.data(), ntile, s);
     DTB_HIP(hipStreamSynchronize(s));
@@ -441,6 +446,20 @@ void Instance::derive_params(hipStream_t s)
     uint8_t *bl = blocked_.mutable_cpu_data();
     for (uint32_t i = 0; i < n_; ++i)
         bl[i] = props_[(size_t)i * kNumCols + kColBlocked] != 0.0f;
+    pA_.to_gpu(s);
+    pB_.to_gpu(s);
+    gU_.to_gpu(s);
+    vU_.to_gpu(s);
+    dec_.to_gpu(s);
+    cls_.to_gpu(s);
+    blocked_.to_gpu(s);
+    derive_pops(s);
+}
+
+void Instance::derive_pops(hipStream_t s)
+{
+    // extern_stimuli.cpp:84-110: per-population OU factors at the substep dt.
+    const float dt_sub = dt_ / (float)t_steps_;
     PopDev *pd = pops_.mutable_cpu_data();
     for (size_t q = 0; q < pop_ids_.size(); ++q) {
         PopDev r{};
@@ -453,13 +472,6 @@ void Instance::derive_params(hipStream_t s)
         }
         pd[q] = r;
     }
-    pA_.to_gpu(s);
-    pB_.to_gpu(s);
-    gU_.to_gpu(s);
-    vU_.to_gpu(s);
-    dec_.to_gpu(s);
-    cls_.to_gpu(s);
-    blocked_.to_gpu(s);
     pops_.to_gpu(s);
     DTB_HIP(hipStreamSynchronize(s));
 }
@@ -523,7 +535,7 @@ void Instance::set_ou(const uint32_t *brain_ids, const float *mean, const float 
             DTB_HIP(hipStreamSynchronize(s));
         }
     }
-    derive_params(s);
+    derive_pops(s);   // the neurons' parameter records do not involve the OU current
 }
 
 void Instance::update_props(const uint32_t *neuron_ids, const uint32_t *prop_ids, const float *vals, size_t n,
