@@ -144,11 +144,16 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
 
 void Instance::init_common(const float *props, float dt, uint64_t seed, hipStream_t s)
 {
+    init_common(std::vector<float>(props, props + (size_t)n_ * kNumCols), dt, seed, s);
+}
+
+void Instance::init_common(std::vector<float> &&props, float dt, uint64_t seed, hipStream_t s)
+{
     if (!(dt > 0.0f))
         throw Error(kInvalidParameter, "dt must be positive");
     dt_ = dt;
     seed_ = seed;
-    props_.assign(props, props + (size_t)n_ * kNumCols);
+    props_ = std::move(props);
 
     // Populations: contiguous runs of the sub-block column
     // (configuration.cpp:1902-1930 requires each sub-block to be contiguous).
@@ -212,7 +217,7 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
                               w_.mutable_gpu_data(), tsyn.mutable_gpu_data(), s));
     std::vector<float> props((size_t)n_ * kNumCols);
     gen_default_props(f.n, f.g_ui, f.sub_base, props.data());
-    init_common(props.data(), dt, seed, s);
+    init_common(std::move(props), dt, seed, s);
     tile_syn_.assign(ntile, 0u);
     tsyn.download(tile_syn_.data(), ntile, s);
     DTB_HIP(hipStreamSynchronize(s));

@@ -320,6 +320,7 @@ public:
 
 private:
     void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
+    void init_common(std::vector<float> &&props, float dt, uint64_t seed, hipStream_t s);
     void derive_params(hipStream_t s);
     void derive_pops(hipStream_t s);     // population table (OU factors) only
     void init_state(hipStream_t s);

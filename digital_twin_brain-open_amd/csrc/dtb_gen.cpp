@@ -227,9 +227,13 @@ void gen_uniform(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint
 
 void gen_default_props(int64_t n, const float g_ui[4], uint32_t sub_base, float *props)
 {
+    // two rows (excitatory, inhibitory population) copied into place
     const uint32_t n_exc = (uint32_t)(n * 8 / 10);
+    float rows[2 * kNumCols];
+    fill_props(rows, 0, sub_base, g_ui);
+    fill_props(rows, 1, sub_base + 1u, g_ui);
     for (uint32_t i = 0; i < (uint32_t)n; ++i)
-        fill_props(props, i, sub_base + (i >= n_exc ? 1u : 0u), g_ui);
+        std::memcpy(props + (size_t)i * kNumCols, rows + (i >= n_exc ? kNumCols : 0), sizeof(float) * kNumCols);
 }
 
 void gen_fanout(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint32_t sub_base, float *props,
