@@ -723,8 +723,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 }
 
 // Gather propagation (instances with InstDev::segidx; their k_step reads
-// the delta from dlt): the delta of the rows fired by the last launch (list
-// `par`) for one group of kGroupTiles tiles, written into dlt.  A wave takes
+// the delta from dlt): the delta of the rows fired by the last launch (the
+// fired lists of set `par`) for one group of kGroupTiles tiles, written into
+// dlt.  A wave takes
 // 64 fired rows at a time: lane l reads row l's segment bounds for the group
 // from the row's group offsets, a wave prefix sum lays the segments end to
 // end, and the wave then reads them with consecutive lanes on consecutive
@@ -733,9 +734,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 // does not depend on the order); each lane has kGatherItems synapses in
 // flight.  Groups of 2048 neurons give segments of about 20 synapses at the
 // stress shape (in-degree 5000 on 500k neurons), which is what coalesces the
-// reads; one 1024-thread workgroup per group.  Also empties fired list
-// par ^ 1 (the one the k_step launch after this one fills; its last reader
-// was the previous k_gather).  Consecutive groups go to one XCD (workgroups
+// reads; one 1024-thread workgroup per group.  Also empties the fired lists
+// of set par ^ 1 (the ones the k_step launch after this one fills; their
+// last reader was the previous k_gather).  Consecutive groups go to one XCD (workgroups
 // are dealt to the 8 XCDs round-robin by index), so the segments of
 // neighbouring groups, which share cache lines, meet in one L2.
 __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__restrict__ groups,
