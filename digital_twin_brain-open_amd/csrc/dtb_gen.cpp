@@ -232,8 +232,10 @@ void gen_default_props(int64_t n, const float g_ui[4], uint32_t sub_base, float 
     float rows[2 * kNumCols];
     fill_props(rows, 0, sub_base, g_ui);
     fill_props(rows, 1, sub_base + 1u, g_ui);
-    for (uint32_t i = 0; i < (uint32_t)n; ++i)
-        std::memcpy(props + (size_t)i * kNumCols, rows + (i >= n_exc ? kNumCols : 0), sizeof(float) * kNumCols);
+    parallel_ranges(n, thread_count(n * 4), [&](int, int64_t a, int64_t b) {
+        for (int64_t i = a; i < b; ++i)
+            std::memcpy(props + (size_t)i * kNumCols, rows + (i >= n_exc ? kNumCols : 0), sizeof(float) * kNumCols);
+    });
 }
 
 void gen_fanout(int64_t n, uint32_t K, uint64_t seed, const float g_ui[4], uint32_t sub_base, float *props,
