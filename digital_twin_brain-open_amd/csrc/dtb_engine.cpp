@@ -290,12 +290,6 @@ uint32_t Instance::fired_capacity() const
     return (ntile + kFiredLists - 1) / kFiredLists * kBlock;
 }
 
-size_t Instance::gather_bytes() const
-{
-    return (size_t)n_ * (ngroups() + 1u) * sizeof(uint16_t) +
-           2u * kFiredLists * ((size_t)fired_capacity() + kFiredCountStride) * sizeof(uint32_t);
-}
-
 void Instance::ensure_gather(hipStream_t s)
 {
     if (!gather_ok_)

@@ -309,8 +309,6 @@ public:
     // Mean over rows of the fraction of the groups that the row reaches.
     double group_coverage() const { return group_coverage_; }
     uint32_t ngroups() const { return (n_ + (1u << kGroupBits) - 1) >> kGroupBits; }
-    // Device bytes of the gather buffers (group offsets, fired lists).
-    size_t gather_bytes() const;
     // Rows per fired list (the tiles of one list hold at most this many).
     uint32_t fired_capacity() const;
     // Builds the group offsets and fired lists if not built yet.
@@ -410,8 +408,6 @@ public:
     void set_schedule(uint32_t pass_k, int layout);
     // Whether instance p propagates by gather under the current layout.
     bool use_gather(const Instance &p) const;
-    // Instances of the current tables that propagate by gather.
-    uint32_t gather_instances() const { return ngather_; }
     uint32_t pass_k() const { return pass_k_; }
     int count_layout() const { return count_layout_; }
     uint32_t count_shift() const { return cshift_; }
@@ -502,7 +498,7 @@ private:
     DataAllocator<TileDesc> d_tiles_;
     DataAllocator<uint32_t> d_qcnt_;       // count_words(2 * ntiles)
     DataAllocator<GroupDesc> d_groups_;    // gather groups of the gather instances
-    uint32_t ngroups_ = 0, ngather_ = 0;
+    uint32_t ngroups_ = 0;
     void launch_gather_k(uint32_t par);    // k_gather of parity par, if any instance gathers
     hipGraphExec_t graph_[2] = {nullptr, nullptr};   // by starting parity
     int graph_steps_[2] = {0, 0};

@@ -234,7 +234,6 @@ void Engine::rebuild()
     std::vector<InstDev> table;
     std::vector<TileDesc> tiles;
     std::vector<GroupDesc> groups;
-    uint32_t ngather = 0;
     std::vector<SampleDev> samples;
     std::vector<SampleRecDev> recs;
     tile_base_.assign(insts_.size(), 0u);
@@ -272,7 +271,6 @@ void Engine::rebuild()
             p->ensure_gather(stream_);
             for (uint32_t g = 0; g < p->ngroups(); ++g)
                 groups.push_back(GroupDesc{idx, g, std::min<uint32_t>(1u << kGroupBits, p->n() - (g << kGroupBits)), 0u});
-            ++ngather;
         }
         table.push_back(p->dev(tile_base_[slot], gat));
         recs.push_back(p->sample_rec());
@@ -291,7 +289,6 @@ void Engine::rebuild()
     ninst_ = (uint32_t)table.size();
     ntiles_ = (uint32_t)tiles.size();
     ngroups_ = (uint32_t)groups.size();
-    ngather_ = ngather;
     d_groups_.resize(std::max<size_t>(groups.size(), 1));
     nsamples_ = (uint32_t)samples.size();
     d_insts_.resize(std::max<size_t>(table.size(), 1));
