@@ -315,6 +315,14 @@ void Instance::ensure_gather(hipStream_t s)
     gather_built_ = true;
 }
 
+void Instance::release_gather()
+{
+    segidx_.resize(0);
+    fired_.resize(0);
+    fcount_.resize(0);
+    gather_built_ = false;
+}
+
 void Instance::clear_fired(hipStream_t s)
 {
     if (gather_built_)

@@ -313,6 +313,9 @@ public:
     uint32_t fired_capacity() const;
     // Builds the group offsets and fired lists if not built yet.
     void ensure_gather(hipStream_t s);
+    // Returns them (no table may still reference them).
+    void release_gather();
+    bool gather_built() const { return gather_built_; }
     // Empties both fired lists (checkpoint restore).
     void clear_fired(hipStream_t s);
 

@@ -271,6 +271,10 @@ void Engine::rebuild()
             p->ensure_gather(stream_);
             for (uint32_t g = 0; g < p->ngroups(); ++g)
                 groups.push_back(GroupDesc{idx, g, std::min<uint32_t>(1u << kGroupBits, p->n() - (g << kGroupBits)), 0u});
+        } else if (p->gather_built()) {
+            // the old tables (settled above) were the last to use them, e.g.
+            // an autotuner turn; device memory returns to the cache
+            p->release_gather();
         }
         table.push_back(p->dev(tile_base_[slot], gat));
         recs.push_back(p->sample_rec());
