@@ -18,7 +18,10 @@ instances of 20k-200k neurons, log-uniform, in-degree 200, per-instance g_ui
 uniform in 0.5-1.5 x the 10k-neuron block's, OU mean 3.0) stepped by one
 launch per step for all instances ("batched") against the same kernels run
 one instance at a time ("plain"), with every instance's counters and state
-checked identical between the two.
+checked identical between the two; and "serving": a stream of 100 requests
+(Poisson arrivals at 20/s, 10k-1M neurons, out-degree 100-1000, 100-2000 ms
+of model time) served with continuous batching, with p50/p99 completion
+latency, against its first 12 requests run alone one after the other.
 """
 import argparse
 import importlib
@@ -32,12 +35,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-DT_MS = 0.1
-N_NEURONS = 2_000_000
-N_VOXELS = 200
-IN_DEGREE = 500
-OU_MEAN, OU_SIGMA, OU_TAU = 5.0, 0.1, 4.0
-NET_SEED, SIM_SEED = 2024, 7
+WL = importlib.import_module("digital_twin_brain-open_amd.workloads")
+DT_MS = WL.DT_MS
+OU_TAU = WL.OU_TAU
 HBM_PEAK_GBS = 8000.0
 GRAPH_CHUNK = 100
 # The warm-up ends with the autotuner, in the firing regime that is timed:
@@ -46,11 +46,12 @@ GRAPH_CHUNK = 100
 # total is kept; then GRAPH_CHUNK steps capture the graph.
 AUTOTUNE_TRIAL = 32
 AUTOTUNE_STEPS = 7 * (2 + (AUTOTUNE_TRIAL + 3) // 4 * 6)   # Engine.autotune_steps(AUTOTUNE_TRIAL)
-# rocprofv3 FETCH_SIZE + WRITE_SIZE of one step's launches per workload and
-# propagation layout (at the workload's default size), scaled by a
-# device-copy calibration (each file gives its commands and factors).
-TRAFFIC_PROFILE = {("block", "dense"): os.path.join(ROOT, "profiles", "k_step_voxel2M_in500.md"),
-                   ("stress", "gather"): os.path.join(ROOT, "profiles", "k_gather_k_step_uniform500k_in5000.md")}
+# rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, scaled by a device-copy
+# calibration, of profiled runs: each summary names its window (workload,
+# neurons, propagation layout, timed steps, warm-up steps) and is used only
+# for a run of exactly that window.
+PROFILES = ("k_step_voxel2M_in500.md", "k_step_voxel2M_in500_steps20_warmup5.md",
+            "k_gather_k_step_uniform500k_in5000.md")
 
 # Algorithmic HBM bytes of one step, as the step is defined on per-neuron
 # arrays: per neuron 165 B (parameters 76 B: 7 scalars and g, V_u, decay of
@@ -79,38 +80,36 @@ GATHER_ROW_BYTES, GATHER_ROW_GROUP_BYTES = 16, 4
 GATHER_SYN_BYTES = 12
 
 
-def profiled_traffic(workload, neurons, layout):
-    """HBM bytes per step (its kernel launches) measured with rocprofv3
-    counters on the workload at its default size and this layout, or None."""
-    path = TRAFFIC_PROFILE.get((workload, layout))
-    default = N_NEURONS if workload == "block" else STRESS_NEURONS
-    if path is None or neurons != default or not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
+def profiled_traffic(workload, neurons, layout, steps, warmup):
+    """HBM bytes per launch measured with rocprofv3 counters over exactly this
+    window (workload, size, layout, timed and warm-up steps), or None."""
+    want = "window: workload=%s neurons=%d layout=%s steps=%d warmup=%d" % (workload, neurons, layout, steps,
+                                                                            warmup)
+    for name in PROFILES:
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            lines = [x.strip() for x in f]
+        if want not in lines:
+            continue
+        for line in lines:
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     return None
 
 
-ENS_INSTANCES, ENS_MIN, ENS_MAX, ENS_DEGREE, ENS_MU, ENS_SEED = 64, 20_000, 200_000, 200, 3.0, 4242
-
-
 def ensemble_bench(dtb, steps, warmup, chunk=64):
     """Batched (all instances in one launch per step, graph replay) vs plain
     (each instance alone on its own engine, same kernels, graph replay)."""
-    rng = np.random.default_rng(ENS_SEED)
-    sizes = np.exp(rng.uniform(np.log(ENS_MIN), np.log(ENS_MAX), ENS_INSTANCES)).astype(np.int64)
-    g0 = np.asarray(dtb.DEFAULT_G_UI, dtype=np.float32)
-    gs = (g0[None, :] * rng.uniform(0.5, 1.5, (ENS_INSTANCES, 4))).astype(np.float32)
     t0 = time.perf_counter()
-    nets = [dtb.gen_uniform(int(n), ENS_DEGREE, seed=ENS_SEED + 1 + i, g_ui=gs[i]) for i, n in enumerate(sizes)]
+    nets, sizes = WL.ensemble_networks()
     log("ensemble: generated %d instances, %d neurons, %d synapses in %.1f s"
         % (len(nets), int(sizes.sum()), sum(x.nsyn for x in nets), time.perf_counter() - t0))
 
     def setup(eng, i, net):
-        b = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED + i, engine=eng)
-        b.update_ou_background_stimuli(b.subblk_id, OU_TAU, ENS_MU, OU_SIGMA)
+        b = dtb.Block(net, delta_t=DT_MS, seed=WL.ENS_SIM_SEED + i, engine=eng)
+        b.update_ou_background_stimuli(b.subblk_id, OU_TAU, WL.ENS_MU, WL.ENS_SIGMA)
         eng.configure(b.inst, 1, True)
         return b
 
@@ -160,9 +159,7 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
         same = same and s1 == c1[i] and np.array_equal(b.state()["V"].view(np.uint32), vb[i])
         e.close()
     spikes = sum(c1[i][2] - c0[i][2] for i in range(len(blks)))
-    return {"workload": "%d instances, %d-%d neurons log-uniform (%d total), in-degree %d, g_ui x U(0.5,1.5), "
-                        "OU mu %.1f, dt %.1f ms" % (ENS_INSTANCES, ENS_MIN, ENS_MAX, int(sizes.sum()), ENS_DEGREE,
-                                                   ENS_MU, DT_MS),
+    return {"workload": WL.ensemble_description(sizes),
             "steps": steps, "warmup": warmup, "graph_chunk": chunk,
             "batched_schedule": schedule, "autotune": "every engine (batched, and each plain instance) "
                                                       "autotuned over %d warm-up steps" % tuned_steps,
@@ -200,6 +197,7 @@ def serving_bench(dtb, nreq, rate, plain):
                               "100-1000, 100-2000 ms of model time, OU mu 3.0" % (nreq, rate, int(sizes.mean())),
                   "plain_requests": plain, "plain_events_per_s": ev_plain / t_plain,
                   "plain_neuron_steps_per_s": ns_plain / t_plain,
+                  "speedup_vs_plain": stats["events_per_s"] / (ev_plain / t_plain),
                   "plain_same_results": bool(same)})
     return stats
 
@@ -208,40 +206,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# Propagation stress: 500k neurons, in-degree 5000 (2.5e9 synapses), g_ui at
-# 0.02 x the 10k-neuron block's and OU mean 12 (sigma 0.5) for bursty
-# synchronous firing near 50 Hz.
-STRESS_NEURONS, STRESS_DEGREE, STRESS_G_SCALE, STRESS_MU, STRESS_SIGMA = 500_000, 5000, 0.02, 12.0, 0.5
-
-
-def make_workload(dtb, name, neurons):
-    """(network, OU mean, OU sigma, description) of a named workload."""
+def make_workload(name, neurons):
+    """(network, OU mean, OU sigma, simulation seed, description) of a named workload."""
     if name == "block":
-        net = dtb.gen_voxel(neurons, N_VOXELS, IN_DEGREE, seed=NET_SEED)
-        # One parameter record per population, as after per-population
-        # assimilation updates: g_AMPA differs by 1e-4 relative per population.
-        sub = net.props[:, 3].astype(np.int64)
-        net.props[:, 10] = (net.props[:, 10] * (1.0 + 1e-4 * sub)).astype(np.float32)
-        desc = ("one GPU's block: %d neurons, %d voxels, in-degree %d, dt %.1f ms, OU mu %.1f"
-                % (net.n, N_VOXELS, IN_DEGREE, DT_MS, OU_MEAN))
-        return net, OU_MEAN, OU_SIGMA, desc
+        net = WL.block_network(neurons)
+        return net, WL.BLOCK_MU, WL.BLOCK_SIGMA, WL.BLOCK_SIM_SEED, WL.block_description(net)
     if name == "stress":
-        g = (np.asarray(dtb.DEFAULT_G_UI, dtype=np.float32) * np.float32(STRESS_G_SCALE)).astype(np.float32)
-        net = dtb.gen_uniform(neurons, STRESS_DEGREE, seed=NET_SEED, g_ui=g)
-        desc = ("propagation stress: %d neurons, in-degree %d, g_ui x %.2f, dt %.1f ms, OU mu %.1f sigma %.1f"
-                % (net.n, STRESS_DEGREE, STRESS_G_SCALE, DT_MS, STRESS_MU, STRESS_SIGMA))
-        return net, STRESS_MU, STRESS_SIGMA, desc
+        net = WL.stress_network(neurons)
+        return net, WL.STRESS_MU, WL.STRESS_SIGMA, WL.STRESS_SIM_SEED, WL.stress_description(net)
     raise ValueError(name)
 
 
-def cpu_baseline(dtb, net, blk, steps, mu, sigma):
+def cpu_baseline(net, blk, steps, mu, sigma, seed):
     """The CPU oracle (a C port of the reference step, OpenMP) on this host,
     started from the GPU's state after warm-up so that it runs the same
     dynamics; returns events/s over `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     orc = importlib.import_module("oracle")
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    on = orc.OracleNet.from_network(net, dt=DT_MS, seed=SIM_SEED)
+    on = orc.OracleNet.from_network(net, dt=DT_MS, seed=seed)
     on.set_ou(on.pop_ids, mu, sigma, OU_TAU)
     sim = orc.Sim(on, 32, nthreads=threads)
     st = blk.state()
@@ -283,12 +266,16 @@ def main():
                     help="only run device copies of this size (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 10000; stress 2000)")
-    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default 1000: the block fires after a few hundred; the autotuner runs at the end)")
+    ap.add_argument("--warmup", type=int, default=-1,
+                    help="untimed steps (default 1000: the block passes its initial volley and settles after a "
+                         "few hundred; the autotuner runs at the end)")
     ap.add_argument("--cpu-steps", type=int, default=1000, help="CPU oracle sample length (0 = skip)")
     ap.add_argument("--workload", choices=("block", "stress", "serving"), default="block",
                     help="block: the 2M-neuron block (headline); stress: 500k neurons at in-degree 5000; "
                          "serving: a request stream with continuous batching")
-    ap.add_argument("--requests", type=int, default=SERVE_REQUESTS, help="serving: requests in the stream")
+    ap.add_argument("--requests", type=int, default=SERVE_REQUESTS,
+                    help="serving requests (the serving workload, and the block line's serving leg; 0 = skip "
+                         "the leg)")
     ap.add_argument("--neurons", type=int, default=0, help="override the workload's neuron count")
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     args = ap.parse_args()
@@ -318,20 +305,20 @@ def main():
         print(json.dumps(res), flush=True)
         return
     if args.neurons == 0:
-        args.neurons = N_NEURONS if args.workload == "block" else STRESS_NEURONS
+        args.neurons = WL.BLOCK_NEURONS if args.workload == "block" else WL.STRESS_NEURONS
     if args.steps <= 0:
         args.steps = 10000 if args.workload == "block" else 2000
     if args.warmup < 0:
         args.warmup = 1000
     t0 = time.perf_counter()
-    net, mu, sigma, desc = make_workload(dtb, args.workload, args.neurons)
+    net, mu, sigma, sim_seed, desc = make_workload(args.workload, args.neurons)
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
-    blk = dtb.Block(net, delta_t=DT_MS, seed=SIM_SEED, engine=eng)
+    blk = dtb.Block(net, delta_t=DT_MS, seed=sim_seed, engine=eng)
     blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, mu, sigma)
     eng.configure(blk.inst, 1, True)
-    log("uploaded in %.1f s" % (time.perf_counter() - t0))
+    log("uploaded in %.1f s (pull propagation %s)" % (time.perf_counter() - t0, eng.pull_info(blk.inst)))
 
     # warm-up: steps into the firing regime, the autotuner's trial steps,
     # then steps that also capture the graph the timed region replays
@@ -347,16 +334,19 @@ def main():
     schedule = {"pass_synapses": 64 * pk, "count_layout": layout_used, "autotuned": bool(k_best),
                 "trial_us_per_step": {"%s/%d" % (lay, 64 * kk): round(us, 2) for (lay, kk), us in table.items()}}
     step0, ev0, sp0 = eng.counters(blk.inst)
+    prop0 = eng.propagation_stats()
 
-    # Timed region: K steps replayed from HIP graphs of GRAPH_CHUNK steps,
-    # between two HIP events on the engine stream (the step kernels and the
-    # gaps between them).
+    # Timed region: K steps (replayed from HIP graphs of GRAPH_CHUNK steps;
+    # steps past the last whole chunk run eagerly), with HIP events on the
+    # engine stream before the first step and after every step (eager) or
+    # chunk (replayed): the step kernels and the gaps between them.
     t0 = time.perf_counter()
-    ms_step = eng.run_span(args.steps, graph_chunk=GRAPH_CHUNK)
+    ms_span, ms_steps = eng.run_span_steps(args.steps, graph_chunk=GRAPH_CHUNK)
     eng.sync()
     wall = time.perf_counter() - t0
     eng.check()
     step1, ev1, sp1 = eng.counters(blk.inst)
+    prop1 = eng.propagation_stats()
     assert step1 - step0 == args.steps
     events = ev1 - ev0
     spikes = sp1 - sp0
@@ -369,7 +359,7 @@ def main():
     rate_hz = spikes / (net.n * args.steps * DT_MS * 1e-3)
 
     ev_step, sp_step = events / args.steps, spikes / args.steps
-    us = ms_step * 1e3 / args.steps
+    us = ms_span * 1e3 / args.steps
     algo_bytes = ALGO_NEURON_BYTES * net.n + ALGO_ROW_BYTES * sp_step + ALGO_SYN_BYTES * ev_step
     gather = layout_used == "gather"
     if gather:
@@ -380,7 +370,8 @@ def main():
         layout_bytes = LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step + LAYOUT_SYN_BYTES * ev_step
     kname = "k_gather+k_step" if gather else "k_step"
     achieved = algo_bytes / (us * 1e-6) / 1e9
-    traffic = profiled_traffic(args.workload, args.neurons, layout_used)
+    traffic = profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup)
+    eager = args.steps < GRAPH_CHUNK
 
     res = {
         "metric": "synaptic events/s (%s)" % ("2M-neuron block" if args.workload == "block" else
@@ -395,13 +386,21 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded voxel network, random weights k*2^-24)",
+        "data": "synthetic (seeded %s network, random weights k*2^-24)" % ("voxel" if args.workload == "block"
+                                                                         else "uniform"),
         "config": {"workload": desc, "neurons": net.n, "synapses": net.nsyn, "parallelism": "single GPU"},
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,
-        "events_per_step": events / args.steps,
+        "events_per_step": ev_step,
         "kernel_us_per_step": {kname: us},
         "kernel_us_isolated": {kname: us_iso},
+        "step_ms_max": float(ms_steps.max()),
+        "steps_over_1ms": int((ms_steps > 1.0).sum()),
+        "step_time_resolution": "one step (eager launches)" if eager else
+                                "one graph of %d steps (its mean)" % GRAPH_CHUNK,
+        "propagation": {"pull_tiles": prop1["pulled"] - prop0["pulled"],
+                        "deferred_tiles": prop1["deferred"] - prop0["deferred"],
+                        "synapses_past_queue_capacity": prop1["spilled"] - prop0["spilled"]},
         "schedule": schedule,
         "kernel_algo_bytes_per_step": {kname: algo_bytes},
         "roofline": {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -414,7 +413,7 @@ def main():
     }
     if args.cpu_steps > 0:
         t0 = time.perf_counter()
-        res["cpu_baseline"] = cpu_baseline(dtb, net, blk, args.cpu_steps, mu, sigma)
+        res["cpu_baseline"] = cpu_baseline(net, blk, args.cpu_steps, mu, sigma, sim_seed)
         log("cpu baseline %.1f s" % (time.perf_counter() - t0))
     else:
         res["cpu_baseline"] = None
@@ -424,6 +423,10 @@ def main():
         t0 = time.perf_counter()
         res["ensemble"] = ensemble_bench(dtb, args.ensemble_steps, min(args.warmup, 800))
         log("ensemble %.1f s" % (time.perf_counter() - t0))
+    if args.requests > 0 and args.workload == "block":
+        t0 = time.perf_counter()
+        res["serving"] = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN)
+        log("serving %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(res), flush=True)
 
 

@@ -108,12 +108,18 @@ _SIGS = {
     "dtb_read_props": (c_int, [c_void_p, c_int, c_void_p]),
     "dtb_engine_queue_capacity": (c_int, [c_void_p, c_uint32, c_uint32]),
     "dtb_engine_schedule": (c_int, [c_void_p, c_int32, c_int32]),
+    "dtb_engine_pull": (c_int, [c_void_p, c_int32, c_uint32]),
+    "dtb_engine_get_pull": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_uint32)]),
+    "dtb_instance_pull_info": (c_int, [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                       POINTER(c_uint64)]),
     "dtb_engine_get_schedule": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
     "dtb_engine_autotune": (c_int, [c_void_p, c_int64, POINTER(c_int32), POINTER(c_int32), c_void_p]),
     "dtb_instance_propagation": (c_int, [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_float)]),
     "dtb_instance_tile_synapses": (c_int, [c_void_p, c_int, c_void_p]),
     "dtb_engine_spilled": (c_int, [c_void_p, POINTER(c_uint64)]),
+    "dtb_engine_propagation_stats": (c_int, [c_void_p, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint64)]),
     "dtb_engine_run_span": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float)]),
+    "dtb_engine_run_span_steps": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float), c_void_p]),
     "dtb_engine_time_kernels": (c_int, [c_void_p, c_int64, POINTER(c_float)]),
     "dtb_engine_check": (c_int, [c_void_p]),
     "dtb_read_state": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -325,6 +325,14 @@ int dtb_engine_run_span(dtb_engine_t h, int64_t steps, int32_t graph_chunk, floa
     });
 }
 
+int dtb_engine_run_span_steps(dtb_engine_t h, int64_t steps, int32_t graph_chunk, float *ms, float *ms_steps)
+{
+    return guarded([&] {
+        need(steps >= 0, "negative step count");
+        engine(h).run_span_units(steps, graph_chunk, ms, ms_steps);
+    });
+}
+
 int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step)
 {
     return guarded([&] {
@@ -344,6 +352,39 @@ int dtb_engine_check(dtb_engine_t h)
 int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift)
 {
     return guarded([&] { engine(h).set_queue_capacity(floor, shift); });
+}
+
+int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min)
+{
+    return guarded([&] { engine(h).set_pull(on != 0, defer_min); });
+}
+
+int dtb_engine_get_pull(dtb_engine_t h, int32_t *on, uint32_t *defer_min)
+{
+    return guarded([&] {
+        const dtb::Engine &e = engine(h);
+        if (on)
+            *on = e.pull_on() ? 1 : 0;
+        if (defer_min)
+            *defer_min = e.defer_min();
+    });
+}
+
+int dtb_instance_pull_info(dtb_engine_t h, int inst, int32_t *allowed, int32_t *built, int32_t *in_use,
+                           uint64_t *bytes)
+{
+    return guarded([&] {
+        dtb::Engine &e = engine(h);
+        const dtb::Instance *p = e.get(inst);
+        if (allowed)
+            *allowed = p->pull_allowed() ? 1 : 0;
+        if (built)
+            *built = p->pull_built() ? 1 : 0;
+        if (in_use)
+            *in_use = e.uses_pull(*p) ? 1 : 0;
+        if (bytes)
+            *bytes = p->pull_bytes();
+    });
 }
 
 int dtb_engine_schedule(dtb_engine_t h, int32_t pass_k, int32_t layout)
@@ -415,6 +456,19 @@ int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses)
     });
 }
 
+int dtb_engine_propagation_stats(dtb_engine_t h, uint64_t *spilled, uint64_t *deferred_tiles, uint64_t *pulled_tiles)
+{
+    return guarded([&] {
+        const dtb::StepCtl c = engine(h).propagation_stats();
+        if (spilled)
+            *spilled = c.spilled;
+        if (deferred_tiles)
+            *deferred_tiles = c.deferred;
+        if (pulled_tiles)
+            *pulled_tiles = c.pulled;
+    });
+}
+
 int dtb_read_state(dtb_engine_t h, int inst, float *v, int32_t *t_ref_count, uint8_t *f, uint8_t *emitted,
                    float *i_ou, float *j, float *i_syn)
 {
@@ -444,7 +498,8 @@ int dtb_set_step(dtb_engine_t h, int inst, int64_t step)
     return guarded([&] {
         need(step >= 0, "negative step");
         dtb::Engine &e = engine(h);
-        e.sync();
+        e.get(inst);
+        e.settle();   // pending input (incl. deferred rows, keyed by step) is folded first
         e.get(inst)->set_step(step, e.stream());
     });
 }

@@ -329,6 +329,70 @@ void Instance::clear_fired(hipStream_t s)
         DTB_HIP(hipMemsetAsync(fcount_.mutable_gpu_data(), 0, fcount_.size(), s));
 }
 
+uint64_t Instance::pull_bytes() const
+{
+    const uint64_t ntile = (n_ + kBlock - 1) / kBlock;
+    const uint64_t nw = (n_ + kWave - 1) / kWave;
+    return (uint64_t)nsyn_ * (sizeof(uint32_t) + sizeof(float2)) + (ntile + 1) * sizeof(int64_t) +
+           4 * nw * sizeof(unsigned long long) + 2 * sizeof(int64_t);
+}
+
+void Instance::ensure_pull(hipStream_t s)
+{
+    if (!pull_allowed())
+        throw Error(kInvalidParameter, "this network does not allow pull propagation");
+    if (pull_built_)
+        return;
+    const uint32_t ntile = (n_ + kBlock - 1) / kBlock;
+    const size_t nw = (n_ + kWave - 1) / kWave;
+    try {
+        // first in-edge of every target tile: prefix sums of the synapses
+        // targeting the tiles
+        std::vector<int64_t> ptr((size_t)ntile + 1, 0);
+        for (uint32_t t = 0; t < ntile; ++t)
+            ptr[t + 1] = ptr[t] + tile_syn_.at(t);
+        if (ptr[ntile] != nsyn_)
+            throw Error(kInternal, "tile synapse counts do not add up to the synapses");
+        cptr_.resize((size_t)ntile + 1);
+        cptr_.upload(ptr.data(), ptr.size(), s);
+        ckey_.resize((size_t)nsyn_);
+        cw_.resize((size_t)nsyn_);
+        ebits_.resize(2 * nw);
+        dbits_.resize(2 * nw);
+        dstamp_.resize(2);
+        ebits_.mutable_gpu_data();
+        dbits_.mutable_gpu_data();
+        dstamp_.mutable_gpu_data();
+        DataAllocator<unsigned long long> cursor(ntile);
+        std::vector<unsigned long long> cur(ptr.begin(), ptr.end() - 1);
+        cursor.upload(cur.data(), ntile, s);
+        DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), n_, cursor.mutable_gpu_data(),
+                                  ckey_.mutable_gpu_data(), cw_.mutable_gpu_data(), s));
+        DTB_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        release_pull();
+        throw;
+    }
+    pull_built_ = true;
+}
+
+void Instance::release_pull()
+{
+    ckey_.resize(0);
+    cw_.resize(0);
+    cptr_.resize(0);
+    ebits_.resize(0);
+    dbits_.resize(0);
+    dstamp_.resize(0);
+    pull_built_ = false;
+}
+
+void Instance::clear_deferred(hipStream_t s)
+{
+    if (pull_built_)
+        DTB_HIP(hipMemsetAsync(dstamp_.mutable_gpu_data(), 0, dstamp_.size(), s));
+}
+
 uint32_t Instance::tile_synapses(uint32_t tile) const
 {
     return tile_syn_.at(tile);
@@ -858,9 +922,20 @@ void Instance::clear_spill(hipStream_t s)
     DTB_HIP(hipMemsetAsync(spill_.mutable_gpu_data(), 0, spill_.size(), s));
 }
 
-InstDev Instance::dev(uint32_t tile_base, bool gather)
+InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer_min)
 {
     InstDev d{};
+    if (pull) {
+        if (!pull_built_ || gather)
+            throw Error(kInternal, "pull propagation needs the in-edges and no gather");
+        d.ckey = ckey_.gpu_data();
+        d.cw = cw_.gpu_data();
+        d.cptr = cptr_.gpu_data();
+        d.ebits = ebits_.mutable_gpu_data();
+        d.dbits = dbits_.mutable_gpu_data();
+        d.dstamp = dstamp_.mutable_gpu_data();
+        d.defer_min = defer_min;
+    }
     if (gather) {
         if (!gather_built_)
             throw Error(kInvalidParameter, "gather buffers not built");

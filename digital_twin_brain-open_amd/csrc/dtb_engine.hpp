@@ -280,8 +280,10 @@ public:
     void set_step(int64_t step, hipStream_t s);
     void counters(int64_t *step, uint64_t *events, uint64_t *spikes, hipStream_t s);
 
-    // Kernel descriptor; `gather`: propagate by gather (ensure_gather first).
-    InstDev dev(uint32_t tile_base, bool gather);
+    // Kernel descriptor; `gather`: propagate by gather (ensure_gather first);
+    // defer_min > 0 with the in-edges built: pull propagation (a tile with
+    // at least defer_min emitted rows defers them to the pull).
+    InstDev dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer_min);
     // Distinct parameter records (classes); n() when stored per neuron.
     uint32_t param_classes() const { return (uint32_t)pA_.count(); }
     SampleRecDev sample_rec();
@@ -318,6 +320,19 @@ public:
     bool gather_built() const { return gather_built_; }
     // Empties both fired lists (checkpoint restore).
     void clear_fired(hipStream_t s);
+
+    // Pull propagation: a copy of the synapses grouped by target tile (the
+    // in-edges, 12 bytes per synapse), through which a tile whose queue
+    // overflowed, or whose sources deferred their rows, takes its input.
+    // Possible for instances of at most kPullMaxNeurons neurons.
+    bool pull_allowed() const { return n_ <= kPullMaxNeurons && nsyn_ > 0; }
+    // Device bytes ensure_pull allocates.
+    uint64_t pull_bytes() const;
+    void ensure_pull(hipStream_t s);
+    void release_pull();
+    bool pull_built() const { return pull_built_; }
+    // Forgets deferred rows (after the pending input was folded or dropped).
+    void clear_deferred(hipStream_t s);
 
 private:
     void init_common(const float *props, float dt, uint64_t seed, hipStream_t s);
@@ -375,6 +390,12 @@ private:
     DataAllocator<uint16_t> segidx_;      // [n][ngroups + 1] group offsets (built on demand)
     DataAllocator<uint32_t> fired_;       // [2][kFiredLists][fired_capacity()] fired rows
     DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]
+    bool pull_built_ = false;             // ckey_, cw_, cptr_, bitmaps built
+    DataAllocator<uint32_t> ckey_;        // [nsyn] in-edges: source row | target in tile | channel bit
+    DataAllocator<float2> cw_;            // [nsyn] their weight pairs
+    DataAllocator<int64_t> cptr_;         // [ntiles + 1] first in-edge of each target tile
+    DataAllocator<unsigned long long> ebits_, dbits_;   // [2][ceil(n / 64)] emitted / deferred rows
+    DataAllocator<int64_t> dstamp_;       // [2] consuming step of deferred rows
     uint32_t gamma_calls_ = 0;            // gamma draw streams used
     std::vector<uint32_t> tile_cls_;      // per tile: 1 + shared parameter class, or 0
 };
@@ -399,10 +420,11 @@ public:
     // Runs `steps` steps of every active instance.  graph_chunk > 1 captures
     // that many steps into one HIP graph and replays it.
     void run(int64_t steps, int graph_chunk);
-    // True if a queue bound was ever exceeded (a bug, never expected).
-    // Synapses delivered through the spill accumulators since the engine was
-    // made (queues past their capacity; not an error).
-    uint64_t spilled();
+    // Since the engine was made: synapses past a queue's capacity (spilled,
+    // or dropped and taken by the pull; not an error), tiles that deferred
+    // their rows, tiles that pulled.
+    StepCtl propagation_stats();
+    uint64_t spilled() { return propagation_stats().spilled; }
     // Step-kernel schedule: synapses per lane in a binning pass (1, 2, 4) and
     // the propagation layout: queues with dense or spread counts
     // (kLayoutAuto: by row coverage), or gather (kLayoutGather: every
@@ -432,6 +454,16 @@ public:
     // Some instance propagates by gather under the current layout.
     bool any_gather() const;
     void autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, float *us);
+    // Pull propagation: on = every instance that allows it gets its
+    // in-edges (when added, or at the next table build) and propagates
+    // through them where queues overflow or tiles defer; defer_min = emitted
+    // rows from which a tile defers (0: tiles never defer, overflowing
+    // queues still pull).  Off: queues only, past-capacity synapses through
+    // the spill accumulators.  Every setting gives the same results.
+    void set_pull(bool on, uint32_t defer_min);
+    bool pull_on() const { return pull_on_; }
+    uint32_t defer_min() const { return defer_min_; }
+    bool uses_pull(const Instance &p) const { return pull_on_ && p.pull_built() && !use_gather(p); }
     // Queue capacity per tile: min(synapses targeting it, floor +
     // (synapses >> shift)); shift 0 keeps the exact worst case.
     void set_queue_capacity(uint32_t floor, uint32_t shift);
@@ -471,6 +503,11 @@ public:
     // run(steps, graph_chunk) between two events on the engine stream; *ms =
     // the GPU time between them (the step kernels and the gaps between).
     void run_span(int64_t steps, int graph_chunk, float *ms);
+    // run(steps, graph_chunk) with an event after every unit (a step run
+    // eagerly or a replayed graph chunk); *ms = the GPU time from before the
+    // first unit to after the last, ms_steps[steps] (optional) = each step's
+    // share of its unit's time (a chunk's time divided by its steps).
+    void run_span_units(int64_t steps, int graph_chunk, float *ms, float *ms_steps);
 
 private:
     void rebuild();
@@ -494,6 +531,10 @@ private:
     int count_layout_ = kLayoutAuto;       // requested layout
     uint32_t pass_k_ = kPassKDefault;      // synapses per lane in a binning pass
     uint32_t queue_floor_ = 4096, queue_shift_ = 3;   // queue_capacity
+    bool tuning_ = false;                  // autotune running: gather buffers are kept
+    bool pull_on_ = true;                  // pull propagation where in-edges fit
+    uint32_t defer_min_ = kDeferMinDefault;
+    void try_pull(Instance &p, hipStream_t s);   // builds p's in-edges if on and they fit
     DataAllocator<InstDev> d_insts_;
     DataAllocator<StepCtl> d_ctl_;
     DataAllocator<SampleDev> d_samples_;

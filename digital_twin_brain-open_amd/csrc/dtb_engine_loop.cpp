@@ -40,6 +40,8 @@ int Engine::add(std::unique_ptr<Instance> inst)
     inst->ensure_queue(queue_slots(*inst));
     if (use_gather(*inst))
         inst->ensure_gather(admit_stream_);
+    else
+        try_pull(*inst, admit_stream_);
     for (size_t i = 0; i < insts_.size(); ++i)
         if (!insts_[i]) {
             insts_[i] = std::move(inst);
@@ -133,6 +135,14 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
     // loads every kernel variant (and builds the gather offsets).
     const int64_t rounds = (trial_steps + kAutotuneWindow - 1) / kAutotuneWindow;
     double t_sum[7] = {0, 0, 0, 0, 0, 0, 0};
+    // The gather buffers stay built between the gather turns (a table build
+    // would otherwise return them at every other turn and build them again);
+    // the first table build after the tuning returns them unless gather won.
+    struct Tuning {
+        bool &flag;
+        explicit Tuning(bool &f) : flag(f) { flag = true; }
+        ~Tuning() { flag = false; }
+    } tuning(tuning_);
     for (int64_t r = -1; r < rounds; ++r) {
         for (int s = 0; s < 7; ++s) {
             if (s < nc)
@@ -167,18 +177,42 @@ void Engine::autotune(int64_t trial_steps, uint32_t *best_k, int *best_layout, f
     const uint32_t bk = ks[bi];
     const int bl = layouts[bi];
     set_schedule(bk, bl);
+    dirty_ = true;   // the next table build returns gather buffers no longer used
     if (best_k)
         *best_k = bk;
     if (best_layout)
         *best_layout = bl;
 }
 
-uint64_t Engine::spilled()
+void Engine::set_pull(bool on, uint32_t defer_min)
+{
+    if (on != pull_on_ || defer_min != defer_min_) {
+        pull_on_ = on;
+        defer_min_ = defer_min;
+        dirty_ = true;
+    }
+}
+
+void Engine::try_pull(Instance &p, hipStream_t s)
+{
+    if (!pull_on_ || p.pull_built() || !p.pull_allowed())
+        return;
+    try {
+        p.ensure_pull(s);
+    } catch (const Error &e) {
+        // without device memory for the in-edges the instance keeps queues
+        // with spill accumulators (same results, slower volleys)
+        if (e.code != kOutOfMemory)
+            throw;
+    }
+}
+
+StepCtl Engine::propagation_stats()
 {
     StepCtl c{};
     d_ctl_.download(&c, 1, stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
-    return c.spilled;
+    return c;
 }
 
 uint64_t Engine::queue_slots(const Instance &p) const
@@ -271,12 +305,15 @@ void Engine::rebuild()
             p->ensure_gather(stream_);
             for (uint32_t g = 0; g < p->ngroups(); ++g)
                 groups.push_back(GroupDesc{idx, g, std::min<uint32_t>(1u << kGroupBits, p->n() - (g << kGroupBits)), 0u});
-        } else if (p->gather_built()) {
-            // the old tables (settled above) were the last to use them, e.g.
-            // an autotuner turn; device memory returns to the cache
+        } else if (p->gather_built() && !tuning_) {
+            // the old tables (settled above) were the last to use them;
+            // device memory returns to the cache
             p->release_gather();
         }
-        table.push_back(p->dev(tile_base_[slot], gat));
+        if (!gat)
+            try_pull(*p, stream_);
+        const bool pl = !gat && pull_on_ && p->pull_built();
+        table.push_back(p->dev(tile_base_[slot], gat, pl, defer_min_));
         recs.push_back(p->sample_rec());
         if (recs.back().nsample > 0) {
             const auto &ids = p->samples();
@@ -406,6 +443,10 @@ void Engine::settle()
     DTB_HIP(hipSetDevice(device_));
     launch_gather_k(par_);
     DTB_HIP(launch_settle(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_, stream_));
+    // the deferred rows were folded with the rest of the pending input
+    for (auto &p : insts_)
+        if (p)
+            p->clear_deferred(stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
     pending_ = false;
 }
@@ -435,6 +476,7 @@ void Engine::clear_queues(int id)
     }
     get(id)->clear_spill(stream_);
     get(id)->clear_fired(stream_);
+    get(id)->clear_deferred(stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -517,6 +559,58 @@ void Engine::run_span(int64_t steps, int graph_chunk, float *ms)
     DTB_HIP(hipEventDestroy(e1));
     if (ms)
         *ms = t;
+}
+
+void Engine::run_span_units(int64_t steps, int graph_chunk, float *ms, float *ms_steps)
+{
+    DTB_HIP(hipSetDevice(device_));
+    if (dirty_)
+        rebuild();
+    if (steps <= 0) {
+        if (ms)
+            *ms = 0.0f;
+        return;
+    }
+    // one event before the first step and one after every unit: a step run
+    // eagerly, or a replayed graph chunk
+    std::vector<hipEvent_t> ev;
+    std::vector<int64_t> len;
+    auto mark = [&]() {
+        hipEvent_t e = nullptr;
+        DTB_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        DTB_HIP(hipEventRecord(e, stream_));
+    };
+    try {
+        mark();
+        for (int64_t done = 0; done < steps;) {
+            const bool chunk = graph_chunk > 1 && steps - done >= graph_chunk;
+            const int64_t u = chunk ? graph_chunk : 1;
+            run(u, chunk ? graph_chunk : 0);
+            mark();
+            len.push_back(u);
+            done += u;
+        }
+        DTB_HIP(hipEventSynchronize(ev.back()));
+        float total = 0.0f;
+        int64_t k = 0;
+        for (size_t i = 0; i < len.size(); ++i) {
+            float t = 0.0f;
+            DTB_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+            total += t;
+            for (int64_t j = 0; j < len[i]; ++j)
+                if (ms_steps)
+                    ms_steps[k++] = t / (float)len[i];
+        }
+        if (ms)
+            *ms = total;
+    } catch (...) {
+        for (auto e : ev)
+            (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto e : ev)
+        DTB_HIP(hipEventDestroy(e));
 }
 
 void Engine::sync()

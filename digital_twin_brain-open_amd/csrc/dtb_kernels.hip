@@ -146,6 +146,50 @@ __device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDe
     }
 }
 
+// Pull propagation: adds into acc (the tile's Q23.40 LDS sums) the weight
+// pairs of tile t's in-edges whose source row was emitted by the launch that
+// filled set `set`: every emitted row (`all`: the queue overflowed and is
+// ignored) or only the rows of tiles that deferred them (dbits).  The
+// in-edges are read coalesced, the source's bit from a per-wave bitmap
+// (n / 8 bytes, cache resident), and the weight pair only for a set bit.
+// Called by the whole workgroup.
+__device__ __forceinline__ void pull_tile(unsigned long long *acc, const InstDev &I, uint32_t set, uint32_t t,
+                                          bool all, uint32_t tid)
+{
+    constexpr uint32_t U = 4;   // in-edges per thread in flight
+    const size_t nw = (I.n + kWave - 1) / kWave;
+    const auto bits = gp(all ? I.ebits : I.dbits) + (size_t)set * nw;
+    const int64_t b = gp(I.cptr)[t];
+    const uint32_t len = (uint32_t)(gp(I.cptr)[t + 1] - b);   // a tile has fewer than 2^32 in-edges
+    // uniform bases, 32-bit offsets per lane
+    const auto key = gp(I.ckey) + b;
+    const auto cw = gp(reinterpret_cast<const unsigned long long *>(I.cw)) + b;
+    for (uint32_t k0 = tid; k0 < len; k0 += kBlock * U) {
+        uint32_t kv[U];
+        bool hit[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t j = k0 + u * kBlock;
+            hit[u] = j < len;
+            kv[u] = key[hit[u] ? j : len - 1u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t r = kv[u] & kPullRowMask;
+            hit[u] = hit[u] && ((bits[r >> 6] >> (r & 63u)) & 1ull) != 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (hit[u]) {
+                const unsigned long long wb = cw[k0 + u * kBlock];
+                const uint32_t slot = ((kv[u] >> kPullRowBits) & 0xffu) * 4u + ((kv[u] & kKindBit) ? 2u : 0u);
+                atomicAdd(&acc[slot], f32_to_q40(__uint_as_float((uint32_t)wb)));
+                atomicAdd(&acc[slot + 1u], f32_to_q40(__uint_as_float((uint32_t)(wb >> 32))));
+            }
+        }
+    }
+}
+
 // A statistic's value as the exact integer rint(x * 2^24).
 __device__ __forceinline__ long long stat_q(float x)
 {
@@ -231,6 +275,18 @@ __device__ __forceinline__ uint32_t wave_max_scan(uint32_t v)
     return v;
 }
 
+// Adds the wave's synapses past their queue's capacity (`over` per lane) to
+// the engine's diagnostic count: one atomic per wave, and none when no lane
+// has any (the usual case).
+__device__ __forceinline__ void count_over(StepCtl *__restrict__ ctl, uint32_t over, uint32_t lane)
+{
+    if (__ballot(over != 0u) != 0ull) {
+        const unsigned long long s = wave_sum_u64((unsigned long long)over);
+        if (lane == 0)
+            atomicAdd(&ctl->spilled, s);
+    }
+}
+
 // bin_pass with the loads and reservations of a lane's synapses one after the
 // other (measured faster for one synapse per lane).
 template <uint32_t K>
@@ -276,6 +332,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
         const uint32_t before = wave_shr1(sc, 0u);
         uint32_t open = (before != 0u) ? before - 1u : c0;   // segment open at the lane's first synapse
         uint32_t st[K];
+        uint32_t over = 0;   // synapses of this lane's segments past their queue's capacity
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             const uint32_t j = j0 + k;
@@ -290,12 +347,12 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
                 const uint32_t base = atomicAdd(&qcnt[count_slot(qset + tg, cshift)], n);
                 const TileDesc td = tiles[tg];
                 const uint32_t room = td.qcap > base ? min(n, td.qcap - base) : 0u;
-                if (room < n)
-                    atomicAdd(&ctl->spilled, (unsigned long long)(n - room));
+                over += n - room;
                 seg_dst[open - c0] = td.qoff + base;
                 seg_room[open - c0] = room;
             }
         }
+        count_over(ctl, over, lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -310,7 +367,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
                     e.w0 = wv[k].x;
                     e.w1 = wv[k].y;
                     queue[seg_dst[st[k] - c0] + at] = e;
-                } else {
+                } else if (spill != nullptr) {
                     unsigned long long *a = spill + (size_t)(cv[k] & kColMask) * 4u + ((cv[k] & kKindBit) ? 2u : 0u);
                     atomicAdd(a, f32_to_q40(wv[k].x));
                     atomicAdd(a + 1, f32_to_q40(wv[k].y));
@@ -411,17 +468,18 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
             qcap[k] = tiles[tg].qcap;
         }
     }
+    uint32_t over = 0;   // synapses of this lane's segments past their queue's capacity
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         if (end[k]) {
             const uint32_t n = j0 + k + 1u - st[k];
             const uint32_t room = qcap[k] > base[k] ? min(n, qcap[k] - base[k]) : 0u;
-            if (room < n)
-                atomicAdd(&ctl->spilled, (unsigned long long)(n - room));
+            over += n - room;
             seg_dst[st[k] - c0] = qoff[k] + base[k];
             seg_room[st[k] - c0] = room;
         }
     }
+    count_over(ctl, over, lane);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -435,7 +493,7 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
                 e.w0 = wv[k].x;
                 e.w1 = wv[k].y;
                 queue[seg_dst[st[k] - c0] + at] = e;
-            } else {
+            } else if (spill != nullptr) {
                 unsigned long long *a = spill + (size_t)(cv[k] & kColMask) * 4u + ((cv[k] & kKindBit) ? 2u : 0u);
                 atomicAdd(a, f32_to_q40(wv[k].x));
                 atomicAdd(a + 1, f32_to_q40(wv[k].y));
@@ -498,6 +556,9 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     else
         head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
     const int64_t step = gp(I.step)[par];
+    // pull propagation (never with gather): rows deferred into this set
+    const bool pull_on = I.ckey != nullptr;
+    const int64_t dst = pull_on ? gp(I.dstamp)[par] : -1;
 
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
@@ -505,11 +566,20 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     acc[tid * 4 + 3] = 0ull;
     if (tid == 0)
         nrows = 0u;
-    const uint32_t cnt = gat ? 0u : min(cnt_raw, d.qcap);
+    // A queue past its capacity is ignored when the tile can pull: every
+    // emitted row then comes through the in-edges.
+    const bool over = !gat && cnt_raw > d.qcap;
+    const bool pull = pull_on && (over || dst == step + 1);
+    const uint32_t cnt = (gat || (pull_on && over)) ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
+    if (pull) {
+        pull_tile(acc, I, par, d.begin >> kTileBits, over, tid);
+        if (tid == 0)
+            atomicAdd(&ctl->pulled, 1ull);
+    }
     __syncthreads();
-    if (!gat && cnt_raw > d.qcap && active)
+    if (!pull_on && over && active)
         take_spill(acc, I, par, li, tid, true);   // synapses past the queue's capacity
     if (tid == 0)
         qs.count[cslot_in] = 0u;   // this launch consumed the queue
@@ -609,6 +679,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     // round-robin to the four waves, so one long row does not keep a single
     // wave busy while the others idle.
     const unsigned long long m = __ballot(emit);
+    const bool wave_in = d.begin + w * kWave < I.n;   // this wave's neurons exist
+    const size_t nw = (I.n + kWave - 1) / kWave;
+    const size_t bw = (size_t)(par ^ 1u) * nw + (d.begin >> 6) + w;   // this wave's bitmap word of set par ^ 1
+    if (pull_on && lane == 0 && wave_in)
+        gp(I.ebits)[bw] = m;
     if (m != 0ull) {
         const uint32_t cnt_w = (uint32_t)__popcll(m);
         const int leader = __ffsll((long long)m) - 1;
@@ -675,8 +750,22 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     }
     __syncthreads();
     const uint32_t nr = nrows;
+    // A tile with many emitted rows (a synchronous volley) leaves them to
+    // the consumers' pull: binning them would cost one reservation per
+    // (row, target tile) and overflow the queues.
+    const bool defer = pull_on && I.defer_min != 0u && nr >= I.defer_min;
+    if (pull_on) {
+        if (lane == 0 && wave_in)
+            gp(I.dbits)[bw] = defer ? m : 0ull;
+        if (defer && tid == 0) {
+            gp(I.dstamp)[par ^ 1u] = step + 2;   // consumer's step + 1 (0: none)
+            atomicAdd(&ctl->deferred, 1ull);
+        }
+    }
+    const uint32_t nbin = defer ? 0u : nr;
+    unsigned long long *spill_out = pull_on ? nullptr : I.spill + (size_t)(par ^ 1u) * I.n * 4u;
     uint32_t pass = 0;
-    for (uint32_t r = 0; r < nr; ++r) {
+    for (uint32_t r = 0; r < nbin; ++r) {
         const long long rb_s = row_rb[r];
         const uint32_t len_s = row_len[r];
         for (uint32_t c0 = 0; c0 < len_s; c0 += kWave * K, ++pass)
@@ -685,12 +774,12 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
                 if constexpr (K == 1)
                     bin_pass_serial<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                        (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
-                                       I.spill + (size_t)(par ^ 1u) * I.n * 4u, ctl, seg_dst + w * kWave * K,
+                                       spill_out, ctl, seg_dst + w * kWave * K,
                                        seg_room + w * kWave * K, lane);
                 else
                     bin_pass<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                 (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
-                                I.spill + (size_t)(par ^ 1u) * I.n * 4u, ctl, seg_dst + w * kWave * K,
+                                spill_out, ctl, seg_dst + w * kWave * K,
                                 seg_room + w * kWave * K, lane);
             }
     }
@@ -869,6 +958,32 @@ __global__ __launch_bounds__(kBlock) void k_build_segidx(const int64_t *__restri
         s[g] = (uint16_t)len;
 }
 
+// In-edges of the pull propagation, one wave per source row: every synapse
+// of row r goes to its target tile's range of ckey / cw at the next free
+// position (cursor[t], starting at the tile's first in-edge).  The order
+// within a tile follows the atomics; the pull's sums are exact integers, so
+// it does not matter.
+__global__ __launch_bounds__(kBlock) void k_build_pull(const int64_t *__restrict__ rowptr,
+                                                       const uint32_t *__restrict__ col,
+                                                       const float2 *__restrict__ w, uint32_t n,
+                                                       unsigned long long *__restrict__ cursor,
+                                                       uint32_t *__restrict__ ckey, float2 *__restrict__ cw)
+{
+    const uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    if (r >= n)
+        return;
+    const int64_t rb = rowptr[r];
+    const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
+    for (uint32_t j = lane; j < len; j += kWave) {
+        const uint32_t c = col[rb + j];
+        const uint32_t t = (c & kColMask) >> kTileBits;
+        const unsigned long long pos = atomicAdd(&cursor[t], 1ull);
+        ckey[pos] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
+        cw[pos] = w[rb + j];
+    }
+}
+
 // Writes the pending delta (the runs queued in queue set `set` for the next
 // k_step) of every neuron of every tile without consuming the queues: used
 // before sampling I_syn and before reading the state back.
@@ -888,12 +1003,17 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt_raw = qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)];
-    const uint32_t cnt = min(cnt_raw, d.qcap);
+    const bool over = cnt_raw > d.qcap;
+    const bool pull_on = I.ckey != nullptr;
+    const bool pull = pull_on && (over || gp(I.dstamp)[set] == gp(I.step)[set] + 1);
+    const uint32_t cnt = (pull_on && over) ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
+    if (pull)
+        pull_tile(acc, I, set, d.begin >> kTileBits, over, tid);
     __syncthreads();
     if (tid < d.count) {
-        if (cnt_raw > d.qcap)
+        if (!pull_on && over)
             take_spill(acc, I, set, d.begin + tid, tid, false);
         st4(I.dlt, d.begin + tid, acc_to_delta(acc, tid));
     }
@@ -933,13 +1053,18 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     acc[tid * 4 + 2] = 0ull;
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt_raw = *cnt_p;
-    const uint32_t cnt = min(cnt_raw, d.qcap);
+    const bool over = cnt_raw > d.qcap;
+    const bool pull_on = I.ckey != nullptr;
+    const bool pull = pull_on && (over || gp(I.dstamp)[set] == gp(I.step)[set] + 1);
+    const uint32_t cnt = (pull_on && over) ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
+    if (pull)
+        pull_tile(acc, I, set, d.begin >> kTileBits, over, tid);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
-        if (cnt_raw > d.qcap)
+        if (!pull_on && over)
             take_spill(acc, I, set, li, tid, true);
         const uint32_t fl = gp(I.flag)[li];
         if (!(fl & kFlagFolded)) {
@@ -1105,6 +1230,17 @@ hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint3
     const uint32_t rows_per_block = kBlock / kWave;
     const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
     hipLaunchKernelGGL(k_build_segidx, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, n, ngroups, segidx);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const float2 *w, uint32_t n,
+                             unsigned long long *cursor, uint32_t *ckey, float2 *cw, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    const uint32_t rows_per_block = kBlock / kWave;
+    const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
+    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, n, cursor, ckey, cw);
     return hipGetLastError();
 }
 

@@ -29,6 +29,11 @@ hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const 
 hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint32_t n, uint32_t ngroups,
                                uint16_t *segidx, hipStream_t stream);
 
+// In-edges of the pull propagation from the presynaptic-major CSR: cursor
+// [ntiles] holds each target tile's first in-edge and is advanced.
+hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const float2 *w, uint32_t n,
+                             unsigned long long *cursor, uint32_t *ckey, float2 *cw, hipStream_t stream);
+
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
                               uint32_t set, hipStream_t stream);

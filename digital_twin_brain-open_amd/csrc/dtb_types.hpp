@@ -104,6 +104,24 @@ struct alignas(16) InstDev {
     const uint16_t *segidx;     // [n][ngroups + 1]
     uint32_t *fired;            // [2][kFiredLists][fcap]
     uint32_t *fcount;           // [2][kFiredLists][kFiredCountStride]
+    // Pull propagation (null ckey: off; never together with gather).  The
+    // in-edges of target tile t are [cptr[t], cptr[t + 1]) of ckey / cw:
+    // ckey = source row (bits 0..22) | target's index in the tile (bits
+    // 23..30) | channel pair (bit 31), cw = weight pair.  Per wave of 64
+    // neurons, ebits[s] holds the rows emitted by the launch that filled
+    // queue set s and dbits[s] those of them whose tile deferred its
+    // propagation to the pull; dstamp[s] = 1 + the step index of the launch
+    // that consumes set s when some tile deferred into it (0: none; a stale
+    // stamp never equals a later consumer's).  A consuming tile pulls
+    // when its queue overflowed (then it ignores the queue and takes every
+    // emitted row through its in-edges) or when rows were deferred (then it
+    // adds the deferred rows to its queue).
+    const uint32_t *ckey;       // [S]
+    const float2 *cw;           // [S]
+    const int64_t *cptr;        // [ntiles + 1]
+    unsigned long long *ebits;  // [2][ceil(n / 64)]
+    unsigned long long *dbits;  // [2][ceil(n / 64)]
+    int64_t *dstamp;            // [2]
     const PopDev *pops;         // [npop]
     // recording (device buffers, row = step - rec_step0)
     uint32_t *rec_counts;       // [rec_len][npop] spikes per population
@@ -129,8 +147,16 @@ struct alignas(16) InstDev {
     uint32_t pmask;             // parameter record of neuron i: (i & pmask) + cls[i]
     uint32_t ngroups;           // gather groups of the instance (kGroupTiles tiles each)
     uint32_t fcap;              // rows per fired list
-    uint32_t pad;
+    uint32_t defer_min;         // pull: a tile with this many emitted rows defers them (0: never)
 };
+
+// Pull propagation limits: source rows are 23-bit fields of the in-edge key.
+constexpr uint32_t kPullRowBits = 23;
+constexpr uint32_t kPullRowMask = (1u << kPullRowBits) - 1u;
+constexpr uint32_t kPullMaxNeurons = 1u << kPullRowBits;
+// Default deferral threshold: a tile whose step emitted at least this many
+// rows leaves them to the consumers' pull instead of binning them.
+constexpr uint32_t kDeferMinDefault = 64;
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
 // of one instance, and the queue of the synapses delivered to them.
@@ -232,7 +258,9 @@ struct alignas(16) SampleRecDev {
 
 // Shared per-engine diagnostics in device memory.
 struct alignas(16) StepCtl {
-    unsigned long long spilled; // synapses that went to the spill accumulators (not an error)
+    unsigned long long spilled;   // synapses past a queue's capacity (spilled, or taken by the pull)
+    unsigned long long deferred;  // tiles that left their emitted rows to the pull
+    unsigned long long pulled;    // tiles that took input through their in-edges (step launches)
     unsigned long long pad;
 };
 

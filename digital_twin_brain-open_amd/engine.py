@@ -298,6 +298,27 @@ class Engine:
         past it go through the exact spill accumulators."""
         _lib.check(self._lib.dtb_engine_queue_capacity(self._h, int(floor), int(shift)))
 
+    def pull(self, on=None, defer_min=None):
+        """Sets (when given) and returns (on, defer_min) of the pull
+        propagation: instances get their synapses grouped by target tile, a
+        tile whose queue overflowed takes every emitted row through them, and
+        a tile with at least defer_min emitted rows in a step leaves them to
+        the pull (0: never).  Every setting gives the same results."""
+        o, d = c_int32(0), ctypes.c_uint32(0)
+        _lib.check(self._lib.dtb_engine_get_pull(self._h, byref(o), byref(d)))
+        if on is not None or defer_min is not None:
+            o = c_int32(int(bool(on)) if on is not None else o.value)
+            d = ctypes.c_uint32(int(defer_min) if defer_min is not None else d.value)
+            _lib.check(self._lib.dtb_engine_pull(self._h, o, d))
+        return bool(o.value), int(d.value)
+
+    def pull_info(self, inst):
+        """{"allowed", "built", "in_use", "bytes"} of an instance's pull
+        propagation (in-edges)."""
+        a, b, u, n = c_int32(0), c_int32(0), c_int32(0), c_uint64(0)
+        _lib.check(self._lib.dtb_instance_pull_info(self._h, inst, byref(a), byref(b), byref(u), byref(n)))
+        return {"allowed": bool(a.value), "built": bool(b.value), "in_use": bool(u.value), "bytes": int(n.value)}
+
     LAYOUTS = {"auto": 0, "dense": 1, "spread": 2, "gather": 3}
 
     def schedule(self, pass_k=None, layout=None):
@@ -360,12 +381,29 @@ class Engine:
         _lib.check(self._lib.dtb_engine_spilled(self._h, byref(v)))
         return v.value
 
+    def propagation_stats(self):
+        """{"spilled": synapses past a queue's capacity, "deferred": tile steps
+        that left their rows to the pull, "pulled": tile steps that took input
+        through their in-edges}, since the engine was created."""
+        a, b, c = c_uint64(0), c_uint64(0), c_uint64(0)
+        _lib.check(self._lib.dtb_engine_propagation_stats(self._h, byref(a), byref(b), byref(c)))
+        return {"spilled": a.value, "deferred": b.value, "pulled": c.value}
+
     def run_span(self, steps, graph_chunk=0):
         """Runs `steps` steps like run(); returns the GPU milliseconds between
         events recorded on the engine stream before and after them."""
         ms = c_float(0.0)
         _lib.check(self._lib.dtb_engine_run_span(self._h, steps, graph_chunk, byref(ms)))
         return ms.value
+
+    def run_span_steps(self, steps, graph_chunk=0):
+        """run_span with an event after every step run eagerly or graph chunk
+        replayed; returns (total GPU ms, per-step ms [steps]: each step's
+        share of its unit)."""
+        ms = c_float(0.0)
+        per = np.zeros(max(int(steps), 0), dtype=np.float32)
+        _lib.check(self._lib.dtb_engine_run_span_steps(self._h, steps, graph_chunk, byref(ms), _ptr(per)))
+        return ms.value, per
 
     def time_kernels(self, steps):
         """Runs `steps` steps; returns the total ms of the step kernel

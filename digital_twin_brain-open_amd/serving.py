@@ -33,11 +33,15 @@ BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 64 + 1 + 8 + 1
 BYTES_PER_SYNAPSE = 4 + 8
 QUEUE_ENTRY_BYTES, QUEUE_FLOOR, QUEUE_SHIFT = 12, 4096, 3
 GATHER_GROUP, FIRED_LISTS = 2048, 64
+# Pull propagation: the in-edges (source | target | channel key and weight
+# pair per synapse), per tile its first in-edge, per wave of 64 neurons two
+# bitmap words per parity.
+PULL_BYTES_PER_SYNAPSE = 4 + 8
 
 
 class Request:
     __slots__ = ("rid", "arrival", "n", "k", "steps", "net_seed", "seed", "mu", "sigma", "tau", "g_ui",
-                 "admitted", "finished", "events", "spikes", "v_hash", "v_mean")
+                 "admitted", "finished", "events", "spikes", "v_hash", "v_mean", "v")
 
     def __init__(self, rid, arrival, n, k, steps, net_seed, seed, mu, sigma=0.1, tau=4.0, g_ui=DEFAULT_G_UI):
         self.rid, self.arrival, self.n, self.k, self.steps = rid, arrival, int(n), int(k), int(steps)
@@ -46,10 +50,12 @@ class Request:
         self.admitted = self.finished = None
         self.events = self.spikes = None
         self.v_hash = self.v_mean = None
+        self.v = None
 
-    def device_bytes(self, gather=False):
+    def device_bytes(self, gather=False, pull=False):
         """Device footprint; gather: also the gather buffers (2-byte offsets
-        per row and group of 2048 neurons, 64 fired lists per parity)."""
+        per row and group of 2048 neurons, 64 fired lists per parity); pull
+        (without gather): also the in-edges of the pull propagation."""
         syn = self.n * self.k
         tiles = (self.n + 255) // 256
         queue = min(syn, tiles * QUEUE_FLOOR + (syn >> QUEUE_SHIFT))   # bound on the sum over tiles
@@ -58,6 +64,8 @@ class Request:
             groups = (self.n + GATHER_GROUP - 1) // GATHER_GROUP
             fcap = (tiles + FIRED_LISTS - 1) // FIRED_LISTS * 256
             b += self.n * (groups + 1) * 2 + 2 * FIRED_LISTS * (fcap + 32) * 4
+        elif pull:
+            b += syn * PULL_BYTES_PER_SYNAPSE + (tiles + 1) * 8 + 4 * ((self.n + 63) // 64) * 8 + 16
         return b
 
     def latency(self):
@@ -91,13 +99,21 @@ def device_free_bytes(device=0):
 class Server:
     """Continuous batching of simulation requests on one engine."""
 
-    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1, reserve=True, layout="auto"):
+    def __init__(self, device=0, hbm_budget=None, chunk=100, dt=0.1, reserve=True, layout="auto", pull=True,
+                 keep_v=False):
         """layout: the engine's propagation layout (Engine.schedule); with
         "gather" every request propagates by gather and its footprint counts
-        the gather buffers."""
+        the gather buffers.  pull: requests get the in-edges of the pull
+        propagation (their initial synchronous volley then costs one pass
+        over the synapses instead of overflowing queues; 12 more bytes per
+        synapse).  keep_v: each retired request keeps its final V
+        (Request.v)."""
         self.engine = Engine(device)
         self.engine.schedule(layout=layout)
+        self.engine.pull(pull)
         self.gather = layout == "gather"
+        self.pull = bool(pull)
+        self.keep_v = bool(keep_v)
         self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes(device))
         self.chunk = int(chunk)
         self.dt = dt
@@ -125,6 +141,8 @@ class Server:
         r.events, r.spikes = events, spikes
         r.v_hash = int(np.bitwise_xor.reduce(V.view(np.uint32)))
         r.v_mean = float(V.astype(np.float64).mean())
+        if self.keep_v:
+            r.v = V
         eng.remove(b.inst)
 
     def serve(self, requests, realtime=True, max_resident=None, progress=None):
@@ -134,8 +152,9 @@ class Server:
         are filled into the Request objects.  progress: seconds between
         one-line reports on stderr (None: silent)."""
         for r in requests:
-            if r.device_bytes(self.gather) > self.budget:
-                raise ValueError("request %d needs %d bytes, more than the HBM budget" % (r.rid, r.device_bytes(self.gather)))
+            if r.device_bytes(self.gather, self.pull) > self.budget:
+                raise ValueError("request %d needs %d bytes, more than the HBM budget"
+                                 % (r.rid, r.device_bytes(self.gather, self.pull)))
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         active = {}    # inst -> (request, block, remaining steps)
         used = 0
@@ -152,7 +171,7 @@ class Server:
             nonlocal used, t_admit, wait_retire
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
             while (pending and not wait_retire and pending[0].arrival - offset <= now
-                   and used + pending[0].device_bytes(self.gather) <= self.budget
+                   and used + pending[0].device_bytes(self.gather, self.pull) <= self.budget
                    and (max_resident is None or len(active) + len(joining) < max_resident)):
                 r = pending.popleft()
                 ta = time.monotonic()
@@ -171,7 +190,7 @@ class Server:
                 t_admit += time.monotonic() - ta
                 r.admitted = time.monotonic() - t0 + offset
                 joining[b.inst] = [r, b, r.steps]
-                used += r.device_bytes(self.gather)
+                used += r.device_bytes(self.gather, self.pull)
 
         joining = {}   # admitted while a chunk runs: they step from the next chunk on
         while pending or active or joining:
@@ -213,7 +232,7 @@ class Server:
                 self._retire(r, b)
                 t_retire += time.monotonic() - ta
                 r.finished = time.monotonic() - t0 + offset
-                used -= r.device_bytes(self.gather)
+                used -= r.device_bytes(self.gather, self.pull)
                 done_count += 1
                 wait_retire = False
         wall = time.monotonic() - t0

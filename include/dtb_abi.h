@@ -188,6 +188,11 @@ int dtb_engine_sync(dtb_engine_t h);
  * the engine stream; *ms = the GPU time between them (step kernels and the
  * gaps between them), the engine synchronised on return. */
 int dtb_engine_run_span(dtb_engine_t h, int64_t steps, int32_t graph_chunk, float *ms);
+/* No counterpart in the reference: dtb_engine_run_span with an event after
+ * every unit of work (a step launched eagerly, or a replayed graph chunk);
+ * ms_steps[steps] (may be NULL) = each step's share of its unit's GPU time
+ * (per step when eager, the chunk's mean when replayed). */
+int dtb_engine_run_span_steps(dtb_engine_t h, int64_t steps, int32_t graph_chunk, float *ms, float *ms_steps);
 /* Times `steps` steps with HIP events around each step kernel (delta
  * accumulation + neuron update + spike binning); total in ms. */
 int dtb_engine_time_kernels(dtb_engine_t h, int64_t steps, float *ms_step);
@@ -201,6 +206,23 @@ int dtb_engine_check(dtb_engine_t h);
  * with the same exact integer arithmetic, so results do not depend on the
  * capacity; dtb_engine_spilled counts them.  Takes effect at the next run. */
 int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
+/* No counterpart in the reference (its scatter costs the same per synapse at
+ * any firing rate, presynaptic_voltage.cpp:137-188).  Pull propagation: with
+ * on != 0 every instance of at most 2^23 neurons gets a copy of its synapses
+ * grouped by target tile (its in-edges, 12 bytes per synapse; built on the
+ * device when the instance is added or at the next run; skipped when device
+ * memory is short).  A tile whose queue overflowed then takes every emitted
+ * row through its in-edges instead of spill atomics, and a tile with at
+ * least defer_min emitted rows in a step (a synchronous volley) leaves them
+ * to its targets' pull instead of binning them (defer_min 0: never).
+ * Default on, defer_min 64.  Every setting gives bit-identical results. */
+int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min);
+int dtb_engine_get_pull(dtb_engine_t h, int32_t *on, uint32_t *defer_min);
+/* Pull propagation of one instance: allowed (at most 2^23 neurons, has
+ * synapses), in-edges built, used under the current settings (not while the
+ * instance gathers), device bytes of the in-edges (any pointer may be NULL). */
+int dtb_instance_pull_info(dtb_engine_t h, int inst, int32_t *allowed, int32_t *built, int32_t *in_use,
+                           uint64_t *bytes);
 
 /* No counterpart in the reference (it has one schedule).  The step kernel's
  * schedule: pass_k synapses per lane in a binning pass (1, 2 or 4: passes of
@@ -234,6 +256,10 @@ int dtb_engine_autotune(dtb_engine_t h, int64_t trial_steps, int32_t *pass_k, in
 int dtb_instance_propagation(dtb_engine_t h, int inst, int32_t *gather_allowed, int32_t *gather_in_use,
                              float *group_coverage);
 int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses);
+/* Since the engine was created: synapses past a queue's capacity (spilled,
+ * or taken by the pull), tile steps that deferred their rows to the pull,
+ * tile steps that pulled (any pointer may be NULL). */
+int dtb_engine_propagation_stats(dtb_engine_t h, uint64_t *spilled, uint64_t *deferred_tiles, uint64_t *pulled_tiles);
 /* Synapses targeting each tile of 256 neurons of an instance (the queues'
  * worst case), out[ceil(n / 256)]. */
 int dtb_instance_tile_synapses(dtb_engine_t h, int inst, uint32_t *out);

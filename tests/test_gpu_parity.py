@@ -518,17 +518,20 @@ def test_edge_shapes_bit_exact(dtb, orc, gpu, case, mu, layout):
     assert cnt[2] == int(ref["raster"].sum())
 
 
+@pytest.mark.parametrize("pull", [False, True])
 @pytest.mark.parametrize("floor,shift", [(0, 31), (16, 31), (64, 4)])
-def test_queue_capacity_spill_bit_exact(dtb, orc, gpu, floor, shift):
+def test_queue_capacity_spill_bit_exact(dtb, orc, gpu, floor, shift, pull):
     """Queues smaller than their worst case: the synapses past a tile's
     capacity take the spill accumulators (every synapse with capacity 0; a
-    segment split between queue and spill with 16 entries).  Raster, sampled
-    V and I_syn, and the state stay bit-exact, also across a parameter change
-    with spikes pending (settle) and a second run."""
+    segment split between queue and spill with 16 entries), or with the pull
+    on the overflowing tile takes every emitted row through its in-edges.
+    Raster, sampled V and I_syn, and the state stay bit-exact, also across a
+    parameter change with spikes pending (settle) and a second run."""
     n, steps, seed = 10000, 300, 31
     net = dtb.gen_uniform(n, 100, seed=1)
     ids = np.arange(0, n, 7, dtype=np.uint32)
     eng = dtb.Engine(0)
+    eng.pull(pull, 0)
     eng.queue_capacity(floor, shift)
     try:
         blk = dtb.Block(net, delta_t=DT, seed=seed, engine=eng)

@@ -32,7 +32,7 @@ def test_results_same_alone_and_in_the_mix(dtb, gpu, serving, layout):
             for i, (n, k, steps) in enumerate([(30000, 200, 700), (12000, 150, 300), (50000, 300, 450),
                                                (20000, 120, 900), (9000, 500, 250), (40000, 250, 600)])]
     gather = layout == "gather"
-    budget = sum(sorted(r.device_bytes(gather) for r in reqs)[-3:])   # at most about three resident at once
+    budget = sum(sorted(r.device_bytes(gather, pull=True) for r in reqs)[-3:])   # at most about three resident at once
     srv = serving.Server(device=0, hbm_budget=budget, chunk=64, layout=layout)
     try:
         stats = srv.serve(reqs, realtime=False)
