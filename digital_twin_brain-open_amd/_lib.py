@@ -82,6 +82,9 @@ class RunOutputs(ctypes.Structure):
     ]
 
 
+# int (*)(void *user, int32_t first_iter, int32_t rows, const dtb_run_outputs *out)
+RUN_CHUNK_FN = ctypes.CFUNCTYPE(c_int, c_void_p, c_int32, c_int32, POINTER(RunOutputs))
+
 _SIGS = {
     "dtb_version": (c_int, []),
     "dtb_last_error": (c_char_p, []),
@@ -100,6 +103,7 @@ _SIGS = {
     "dtb_update_props": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int32]),
     "dtb_update_prop_cols": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "dtb_run": (c_int, [c_void_p, c_int, c_int32, c_int32, c_uint32, POINTER(RunOutputs)]),
+    "dtb_run_stream": (c_int, [c_void_p, c_int, c_int32, c_int32, c_uint32, c_int32, RUN_CHUNK_FN, c_void_p]),
     "dtb_engine_run": (c_int, [c_void_p, c_int64, c_int32]),
     "dtb_configure": (c_int, [c_void_p, c_int, c_int32, c_int32]),
     "dtb_engine_sync": (c_int, [c_void_p]),

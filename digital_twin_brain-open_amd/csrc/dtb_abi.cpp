@@ -1,6 +1,7 @@
 // dtb_abi.cpp -- extern "C" boundary of libdtb.so (declared in include/dtb_abi.h).
 #include "dtb_abi.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -236,60 +237,118 @@ int dtb_update_prop_cols(dtb_engine_t h, int inst, const uint32_t *prop_ids, con
     });
 }
 
+namespace {
+
+// One recorded run of `iters` steps of instance I (the body of dtb_run).
+void run_recorded(dtb::Engine &e, dtb::Instance *I, int32_t iters, int32_t t_steps, uint32_t flags,
+                  const dtb_run_outputs *out)
+{
+    // dist_simulator.cpp:3225-3234 parameter checks.
+    need(iters > 0, "iter must be positive");
+    const bool use_ou = (flags & DTB_RUN_USE_OU) != 0;
+    const bool want_samples =
+        (flags & (DTB_RUN_SAMPLE_SPIKE | DTB_RUN_SAMPLE_VMEMB | DTB_RUN_SAMPLE_ISYN | DTB_RUN_SAMPLE_IOU)) != 0;
+    if ((flags & DTB_RUN_SAMPLE_IOU) && !use_ou)
+        throw dtb::Error(dtb::kInvalidParameter, "i_ou samples need the OU background");
+    if (want_samples && I->samples().empty())
+        throw dtb::Error(dtb::kUnsetSampleFailure, "samples are not set");
+    e.settle();
+    I->configure_run(t_steps, use_ou, e.stream());
+    const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
+    const bool samples = want_samples && out &&
+                         (out->sample_spike || out->sample_vmemb || out->sample_isynaptic || out->sample_iou);
+    // Per-population means (RunRequest output_vmean / output_imean /
+    // output_receptor_imeas, snn.proto:63-77).
+    uint32_t stat_mask = 0;
+    float *means[dtb::kStatFields] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (out) {
+        if ((flags & DTB_RUN_VMEAN) && out->vmean) {
+            stat_mask |= dtb::kStatV;
+            means[0] = out->vmean;
+        }
+        if ((flags & DTB_RUN_IMEAN) && out->imean) {
+            stat_mask |= dtb::kStatI;
+            means[1] = out->imean;
+        }
+        if ((flags & DTB_RUN_RECEPTOR_IMEAN) && out->ampa_imean && out->nmda_imean && out->gabaa_imean &&
+            out->gabab_imean) {
+            stat_mask |= dtb::kStatReceptor;
+            means[2] = out->ampa_imean;
+            means[3] = out->nmda_imean;
+            means[4] = out->gabaa_imean;
+            means[5] = out->gabab_imean;
+        }
+    }
+    I->begin_record(iters, freq, samples, stat_mask, e.stream());
+    e.mark_dirty();
+    e.run(iters, (flags & DTB_RUN_GRAPH) ? 64 : 0);
+    if (stat_mask & (dtb::kStatI | dtb::kStatReceptor))
+        e.stat_tail();
+    e.sync();
+    if (out)
+        I->fetch_record(freq ? out->freq : nullptr, (flags & DTB_RUN_SAMPLE_SPIKE) ? out->sample_spike : nullptr,
+                        (flags & DTB_RUN_SAMPLE_VMEMB) ? out->sample_vmemb : nullptr,
+                        (flags & DTB_RUN_SAMPLE_ISYN) ? out->sample_isynaptic : nullptr,
+                        (flags & DTB_RUN_SAMPLE_IOU) ? out->sample_iou : nullptr, means, e.stream());
+    I->end_record();
+    e.mark_dirty();
+}
+
+}  // namespace
+
 int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, const dtb_run_outputs *out)
 {
     return guarded([&] {
-        // dist_simulator.cpp:3225-3234 parameter checks.
+        dtb::Engine &e = engine(h);
+        run_recorded(e, e.get(inst), iters, t_steps, flags, out);
+    });
+}
+
+int dtb_run_stream(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, int32_t chunk,
+                   dtb_run_chunk_fn fn, void *user)
+{
+    return guarded([&] {
         need(iters > 0, "iter must be positive");
+        need(chunk > 0, "chunk must be positive");
+        need(fn != nullptr, "callback is NULL");
         dtb::Engine &e = engine(h);
         dtb::Instance *I = e.get(inst);
-        const bool use_ou = (flags & DTB_RUN_USE_OU) != 0;
-        const bool want_samples =
-            (flags & (DTB_RUN_SAMPLE_SPIKE | DTB_RUN_SAMPLE_VMEMB | DTB_RUN_SAMPLE_ISYN | DTB_RUN_SAMPLE_IOU)) != 0;
-        if ((flags & DTB_RUN_SAMPLE_IOU) && !use_ou)
-            throw dtb::Error(dtb::kInvalidParameter, "i_ou samples need the OU background");
-        if (want_samples && I->samples().empty())
-            throw dtb::Error(dtb::kUnsetSampleFailure, "samples are not set");
-        e.settle();
-        I->configure_run(t_steps, use_ou, e.stream());
-        const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
-        const bool samples = want_samples && out &&
-                             (out->sample_spike || out->sample_vmemb || out->sample_isynaptic || out->sample_iou);
-        // Per-population means (RunRequest output_vmean / output_imean /
-        // output_receptor_imeas, snn.proto:63-77).
-        uint32_t stat_mask = 0;
-        float *means[dtb::kStatFields] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-        if (out) {
-            if ((flags & DTB_RUN_VMEAN) && out->vmean) {
-                stat_mask |= dtb::kStatV;
-                means[0] = out->vmean;
-            }
-            if ((flags & DTB_RUN_IMEAN) && out->imean) {
-                stat_mask |= dtb::kStatI;
-                means[1] = out->imean;
-            }
-            if ((flags & DTB_RUN_RECEPTOR_IMEAN) && out->ampa_imean && out->nmda_imean && out->gabaa_imean &&
-                out->gabab_imean) {
-                stat_mask |= dtb::kStatReceptor;
-                means[2] = out->ampa_imean;
-                means[3] = out->nmda_imean;
-                means[4] = out->gabaa_imean;
-                means[5] = out->gabab_imean;
-            }
+        const size_t np = I->npop(), ns = I->samples().size();
+        const size_t rows = (size_t)std::min(chunk, iters);
+        // host rows of one chunk, handed to the callback
+        std::vector<uint32_t> freq((flags & DTB_RUN_FREQ) ? rows * np : 0);
+        std::vector<uint8_t> spike((flags & DTB_RUN_SAMPLE_SPIKE) ? rows * ns : 0);
+        std::vector<float> vm((flags & DTB_RUN_SAMPLE_VMEMB) ? rows * ns : 0);
+        std::vector<float> is((flags & DTB_RUN_SAMPLE_ISYN) ? rows * ns : 0);
+        std::vector<float> io((flags & DTB_RUN_SAMPLE_IOU) ? rows * ns : 0);
+        std::vector<float> means[dtb::kStatFields];
+        if (flags & DTB_RUN_VMEAN)
+            means[0].resize(rows * np);
+        if (flags & DTB_RUN_IMEAN)
+            means[1].resize(rows * np);
+        if (flags & DTB_RUN_RECEPTOR_IMEAN)
+            for (int k = 2; k < dtb::kStatFields; ++k)
+                means[k].resize(rows * np);
+        auto ptr = [](auto &v) { return v.empty() ? nullptr : v.data(); };
+        dtb_run_outputs o{};
+        o.freq = ptr(freq);
+        o.sample_spike = ptr(spike);
+        o.sample_vmemb = ptr(vm);
+        o.sample_isynaptic = ptr(is);
+        o.sample_iou = ptr(io);
+        o.vmean = ptr(means[0]);
+        o.imean = ptr(means[1]);
+        o.ampa_imean = ptr(means[2]);
+        o.nmda_imean = ptr(means[3]);
+        o.gabaa_imean = ptr(means[4]);
+        o.gabab_imean = ptr(means[5]);
+        for (int32_t done = 0; done < iters;) {
+            const int32_t m = std::min(chunk, iters - done);
+            run_recorded(e, I, m, t_steps, flags, &o);
+            if (fn(user, done, m, &o) != 0)
+                break;   // the caller stopped the run
+            done += m;
         }
-        I->begin_record(iters, freq, samples, stat_mask, e.stream());
-        e.mark_dirty();
-        e.run(iters, (flags & DTB_RUN_GRAPH) ? 64 : 0);
-        if (stat_mask & (dtb::kStatI | dtb::kStatReceptor))
-            e.stat_tail();
-        e.sync();
-        if (out)
-            I->fetch_record(freq ? out->freq : nullptr, (flags & DTB_RUN_SAMPLE_SPIKE) ? out->sample_spike : nullptr,
-                            (flags & DTB_RUN_SAMPLE_VMEMB) ? out->sample_vmemb : nullptr,
-                            (flags & DTB_RUN_SAMPLE_ISYN) ? out->sample_isynaptic : nullptr,
-                            (flags & DTB_RUN_SAMPLE_IOU) ? out->sample_iou : nullptr, means, e.stream());
-        I->end_record();
-        e.mark_dirty();
     });
 }
 

@@ -500,6 +500,98 @@ class Engine:
         _lib.check(self._lib.dtb_run(self._h, inst, iters, t_steps, flags, byref(o)))
         return out
 
+    def run_block_stream(self, inst, iters, chunk=100, t_steps=1, use_ou=True, freqs=True, sample_spike=False,
+                         sample_vmemb=False, sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0,
+                         vmean=False, imean=False, receptor_imean=False, queue_chunks=2):
+        """run_block streamed: yields (step index within the run, dict of
+        that step's rows) while the engine keeps stepping.  The native run
+        (dtb_run_stream) goes on in a worker thread, chunk by chunk; at most
+        queue_chunks chunks wait for the consumer before the run pauses.
+        Closing the generator early stops the run after the current chunk.
+        Results are those of run_block."""
+        import queue
+        import threading
+
+        flags = (_lib.RUN_USE_OU if use_ou else 0) | (_lib.RUN_GRAPH if graph else 0)
+        fields = []   # (key, flag, dtype, RunOutputs field, row width)
+        if freqs:
+            fields.append(("freq", _lib.RUN_FREQ, np.uint32, "freq", npop))
+        if vmean:
+            fields.append(("vmean", _lib.RUN_VMEAN, np.float32, "vmean", npop))
+        if imean:
+            fields.append(("imean", _lib.RUN_IMEAN, np.float32, "imean", npop))
+        for key, flag, dt, field, on in (("spike", _lib.RUN_SAMPLE_SPIKE, np.uint8, "sample_spike", sample_spike),
+                                         ("vmemb", _lib.RUN_SAMPLE_VMEMB, np.float32, "sample_vmemb", sample_vmemb),
+                                         ("isynaptic", _lib.RUN_SAMPLE_ISYN, np.float32, "sample_isynaptic",
+                                          sample_isynaptic),
+                                         ("iou", _lib.RUN_SAMPLE_IOU, np.float32, "sample_iou", sample_iou)):
+            if on:
+                fields.append((key, flag, dt, field, nsample))
+        if receptor_imean:
+            flags |= _lib.RUN_RECEPTOR_IMEAN
+        for f in fields:
+            flags |= f[1]
+        rec_fields = ("ampa_imean", "nmda_imean", "gabaa_imean", "gabab_imean")
+        q = queue.Queue(maxsize=max(1, int(queue_chunks)))
+        stop = threading.Event()
+        _END = object()
+
+        def on_chunk(user, first, rows, out):
+            o = out.contents
+            data = {}
+            for key, _, dt, field, width in fields:
+                ptr = getattr(o, field)
+                data[key] = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                                  shape=(rows, width)).copy() if width else np.zeros((rows, 0), dt)
+            if receptor_imean:
+                data["receptor_imean"] = np.stack([np.ctypeslib.as_array(
+                    ctypes.cast(getattr(o, f), ctypes.POINTER(ctypes.c_float)), shape=(rows, npop)).copy()
+                    for f in rec_fields]) if npop else np.zeros((4, rows, 0), np.float32)
+            while not stop.is_set():
+                try:
+                    q.put((first, rows, data), timeout=0.1)
+                    break
+                except queue.Full:
+                    continue
+            return 1 if stop.is_set() else 0
+
+        cb = _lib.RUN_CHUNK_FN(on_chunk)
+        result = {}
+
+        def work():
+            try:
+                result["status"] = self._lib.dtb_run_stream(self._h, inst, iters, t_steps, flags, chunk, cb, None)
+                if result["status"] != 0:
+                    result["error"] = _lib.load().dtb_last_error()
+            finally:
+                q.put(_END)
+
+        th = threading.Thread(target=work, daemon=True)
+        th.start()
+        self._stream_thread = th
+        try:
+            while True:
+                item = q.get()
+                if item is _END:
+                    break
+                first, rows, data = item
+                for k in range(rows):
+                    row = {key: v[k] for key, v in data.items() if key != "receptor_imean"}
+                    if "receptor_imean" in data:
+                        row["receptor_imean"] = data["receptor_imean"][:, k]
+                    yield first + k, row
+        finally:
+            stop.set()
+            while th.is_alive():
+                try:
+                    q.get(timeout=0.1)
+                except queue.Empty:
+                    pass
+            th.join()
+        if result.get("status", 0) != 0:
+            msg = result.get("error")
+            raise _lib.DtbError(result["status"], msg.decode() if msg else "")
+
     def read_state(self, inst, n):
         st = {
             "V": np.empty(n, dtype=np.float32),
@@ -608,6 +700,19 @@ class Block:
                                      sample_isynaptic=output_sample_isynaptic, sample_iou=output_sample_iou,
                                      graph=graph, nsample=self._nsample, npop=len(self.pop_ids), vmean=vmean,
                                      imean=imean, receptor_imean=receptor_imean)
+
+    def run_stream(self, iterations, chunk=100, freqs=True, output_sample_spike=False, output_sample_vmemb=False,
+                   output_sample_iou=False, output_sample_isynaptic=False, iou=True, t_steps=1, graph=False,
+                   vmean=False, imean=False, receptor_imean=False):
+        """run() streamed: yields (step, per-step dict) while the run goes
+        on, chunk steps at a time (dist_simulator.cpp:3343-3455 sends one
+        RunResponse per step); the same values as run()."""
+        return self.engine.run_block_stream(self.inst, iterations, chunk=chunk, t_steps=t_steps, use_ou=iou,
+                                            freqs=freqs, sample_spike=output_sample_spike,
+                                            sample_vmemb=output_sample_vmemb, sample_isynaptic=output_sample_isynaptic,
+                                            sample_iou=output_sample_iou, graph=graph, nsample=self._nsample,
+                                            npop=len(self.pop_ids), vmean=vmean, imean=imean,
+                                            receptor_imean=receptor_imean)
 
     def state(self):
         st = self.engine.read_state(self.inst, self._n)

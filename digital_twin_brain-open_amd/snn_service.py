@@ -157,7 +157,12 @@ class SnnServicer:
         self.seed = seed
         self.block = None
         self.stop_event = threading.Event()
+        # One RPC at a time touches the block (the engine is driven by one
+        # thread at a time; dist_simulator.cpp:3993-4047 has one command
+        # thread per GPU).
         self._lock = threading.Lock()
+        self.run_chunk = 100     # steps per streamed chunk of a Run
+        self.run_started = self.run_finished = 0
 
     def GetVersion(self, request, context):
         return self.m.Version(version=_lib.load().dtb_version())
@@ -180,37 +185,47 @@ class SnnServicer:
                                       subblk_info=info)
 
     def Run(self, request, context):
-        b = self.block
-        if b is None:
-            yield self.m.RunResponse(status=1)
-            return
-        try:
-            out = b.run(request.iter, freqs=request.output_freq, output_sample_spike=request.output_sample_spike,
-                        output_sample_vmemb=request.output_sample_vmemb, output_sample_iou=request.output_sample_iou,
-                        output_sample_isynaptic=request.output_sample_isynaptic, iou=request.use_ou_background,
-                        t_steps=max(1, request.t_steps), vmean=request.output_vmean, imean=request.output_imean,
-                        receptor_imean=request.output_receptor_imeas)
-        except _lib.DtbError as e:
-            yield self.m.RunResponse(status=e.status if e.status <= 5 else 1)
-            return
-        freq = out.get("freq")
-        if freq is not None and request.freq_char:
-            freq = b.freq_char(freq)   # stat_result.cpp:103-130
-        elif freq is not None:
-            freq = freq.astype(np.int32)
-        rec = out.get("receptor_imean")
-        for k in range(request.iter):
-            r = self.m.RunResponse(status=0)
-            if freq is not None:
-                r.freq.append(freq[k].tobytes())
-            for key, field in (("vmean", "vmean"), ("spike", "sample_spike"), ("vmemb", "sample_vmemb"),
-                               ("iou", "sample_iou"), ("isynaptic", "sample_isynaptic"), ("imean", "imean")):
-                if key in out:
-                    getattr(r, field).append(out[key][k].tobytes())
-            if rec is not None:
-                for j, field in enumerate(("ampa_imean", "nmda_imean", "gabaa_imean", "gabab_imean")):
-                    getattr(r, field).append(rec[j][k].tobytes())
-            yield r
+        """One RunResponse per step, streamed while the block steps
+        (dist_simulator.cpp:3343-3455): the run goes on in chunks of
+        run_chunk steps and the responses of a chunk are sent while the next
+        one is stepped.  The block is held for the whole run: an Init or
+        update arriving meanwhile waits for it."""
+        with self._lock:
+            b = self.block
+            if b is None:
+                yield self.m.RunResponse(status=1)
+                return
+            self.run_started += 1
+            stream = b.run_stream(request.iter, chunk=self.run_chunk, freqs=request.output_freq,
+                                  output_sample_spike=request.output_sample_spike,
+                                  output_sample_vmemb=request.output_sample_vmemb,
+                                  output_sample_iou=request.output_sample_iou,
+                                  output_sample_isynaptic=request.output_sample_isynaptic,
+                                  iou=request.use_ou_background, t_steps=max(1, request.t_steps),
+                                  vmean=request.output_vmean, imean=request.output_imean,
+                                  receptor_imean=request.output_receptor_imeas)
+            try:
+                for _, row in stream:
+                    r = self.m.RunResponse(status=0)
+                    freq = row.get("freq")
+                    if freq is not None and request.freq_char:
+                        r.freq.append(b.freq_char(freq[None, :])[0].tobytes())   # stat_result.cpp:103-130
+                    elif freq is not None:
+                        r.freq.append(freq.astype(np.int32).tobytes())
+                    for key, field in (("vmean", "vmean"), ("spike", "sample_spike"), ("vmemb", "sample_vmemb"),
+                                       ("iou", "sample_iou"), ("isynaptic", "sample_isynaptic"), ("imean", "imean")):
+                        if key in row:
+                            getattr(r, field).append(row[key].tobytes())
+                    rec = row.get("receptor_imean")
+                    if rec is not None:
+                        for j, field in enumerate(("ampa_imean", "nmda_imean", "gabaa_imean", "gabab_imean")):
+                            getattr(r, field).append(rec[j].tobytes())
+                    yield r
+            except _lib.DtbError as e:
+                yield self.m.RunResponse(status=e.status if e.status <= 5 else 1)
+            finally:
+                stream.close()
+                self.run_finished += 1
 
     def _collect(self, it, fields):
         cols = {f: [] for f in fields}
@@ -220,66 +235,71 @@ class SnnServicer:
         return cols
 
     def Updateprop(self, it, context):
-        c = self._collect(it, ("neuron_id", "prop_id", "prop_val"))
-        ok = self.block is not None
-        if ok and c["neuron_id"]:
-            try:
-                self.block.engine.update_props(self.block.inst, np.asarray(c["neuron_id"], np.uint32),
-                                               np.asarray(c["prop_id"], np.uint32),
-                                               np.asarray(c["prop_val"], np.float32))
-            except _lib.DtbError:
-                ok = False
-        return self.m.UpdatePropResponse(success=ok)
+        with self._lock:
+            c = self._collect(it, ("neuron_id", "prop_id", "prop_val"))
+            ok = self.block is not None
+            if ok and c["neuron_id"]:
+                try:
+                    self.block.engine.update_props(self.block.inst, np.asarray(c["neuron_id"], np.uint32),
+                                                   np.asarray(c["prop_id"], np.uint32),
+                                                   np.asarray(c["prop_val"], np.float32))
+                except _lib.DtbError:
+                    ok = False
+            return self.m.UpdatePropResponse(success=ok)
 
     def Updatehyperpara(self, it, context):
-        ok = self.block is not None
-        for req in it:
-            if not ok or not req.brain_id:
-                continue
-            try:
-                self.block.engine.update_prop_cols(self.block.inst, np.asarray(req.prop_id, np.uint32),
-                                                   np.asarray(req.brain_id, np.uint32),
-                                                   np.asarray(req.hpara_val, np.float32), assign=req.assigned)
-            except _lib.DtbError:
-                ok = False
-        return self.m.UpdateHyperParaResponse(success=ok)
+        with self._lock:
+            ok = self.block is not None
+            for req in it:
+                if not ok or not req.brain_id:
+                    continue
+                try:
+                    self.block.engine.update_prop_cols(self.block.inst, np.asarray(req.prop_id, np.uint32),
+                                                       np.asarray(req.brain_id, np.uint32),
+                                                       np.asarray(req.hpara_val, np.float32), assign=req.assigned)
+                except _lib.DtbError:
+                    ok = False
+            return self.m.UpdateHyperParaResponse(success=ok)
 
     def Updatesample(self, it, context):
-        c = self._collect(it, ("sample_idx",))
-        ok = self.block is not None
-        if ok:
-            try:
-                self.block.set_samples(np.asarray(c["sample_idx"], np.uint32))
-            except _lib.DtbError:
-                ok = False
-        return self.m.UpdateSampleResponse(success=ok)
+        with self._lock:
+            c = self._collect(it, ("sample_idx",))
+            ok = self.block is not None
+            if ok:
+                try:
+                    self.block.set_samples(np.asarray(c["sample_idx"], np.uint32))
+                except _lib.DtbError:
+                    ok = False
+            return self.m.UpdateSampleResponse(success=ok)
 
     def Updateoubackgroundparam(self, it, context):
-        c = self._collect(it, ("brain_id", "mean", "deviation", "correlation_time"))
-        ok = self.block is not None
-        if ok and c["brain_id"]:
-            try:
-                self.block.update_ou_background_stimuli(np.asarray(c["brain_id"], np.uint32),
-                                                        np.asarray(c["correlation_time"], np.float32),
-                                                        np.asarray(c["mean"], np.float32),
-                                                        np.asarray(c["deviation"], np.float32))
-            except _lib.DtbError:
-                ok = False
-        return self.m.UpdateOUBackgroundParamResponse(success=ok)
+        with self._lock:
+            c = self._collect(it, ("brain_id", "mean", "deviation", "correlation_time"))
+            ok = self.block is not None
+            if ok and c["brain_id"]:
+                try:
+                    self.block.update_ou_background_stimuli(np.asarray(c["brain_id"], np.uint32),
+                                                            np.asarray(c["correlation_time"], np.float32),
+                                                            np.asarray(c["mean"], np.float32),
+                                                            np.asarray(c["deviation"], np.float32))
+                except _lib.DtbError:
+                    ok = False
+            return self.m.UpdateOUBackgroundParamResponse(success=ok)
 
     def Updategamma(self, it, context):
-        ok = self.block is not None
-        for req in it:
-            if not ok or not req.brain_id:
-                continue
-            try:
-                self.block.engine.gamma_prop_cols(self.block.inst, np.asarray(req.prop_id, np.uint32),
-                                                  np.asarray(req.brain_id, np.uint32),
-                                                  np.asarray(req.gamma_concentration, np.float32),
-                                                  np.asarray(req.gamma_rate, np.float32))
-            except _lib.DtbError:
-                ok = False
-        return self.m.UpdateGammaResponse(success=ok)
+        with self._lock:
+            ok = self.block is not None
+            for req in it:
+                if not ok or not req.brain_id:
+                    continue
+                try:
+                    self.block.engine.gamma_prop_cols(self.block.inst, np.asarray(req.prop_id, np.uint32),
+                                                      np.asarray(req.brain_id, np.uint32),
+                                                      np.asarray(req.gamma_concentration, np.float32),
+                                                      np.asarray(req.gamma_rate, np.float32))
+                except _lib.DtbError:
+                    ok = False
+            return self.m.UpdateGammaResponse(success=ok)
 
     def _unsupported(self, resp):
         def handler(it, context):
@@ -293,9 +313,10 @@ class SnnServicer:
         return self.m.ShutdownResponse(shutdown=True)
 
     def close(self):
-        if self.block is not None:
-            self.block.engine.close()
-            self.block = None
+        with self._lock:
+            if self.block is not None:
+                self.block.engine.close()
+                self.block = None
 
 
 def make_handler(servicer, msgs):

@@ -176,6 +176,19 @@ int dtb_read_props(dtb_engine_t h, int inst, float *props);
  * Every other instance of the engine advances too. */
 int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, const dtb_run_outputs *out);
 
+/* Streamed Run (tools/dist_simulator.cpp:3343-3455 writes one RunResponse
+ * per step while the loop runs, fed by the report thread :4049-4086): the
+ * run of dtb_run in chunks of `chunk` steps; after each chunk `fn` receives
+ * the chunk's rows (first_iter = index of its first step within the run,
+ * rows = its steps; the outputs selected by `flags`, row-major
+ * [rows][...], in library-owned host memory valid during the call).  Device
+ * and host memory hold one chunk, whatever iters is.  fn returns 0 to go on,
+ * anything else to stop after that chunk (the run then ends with DTB_OK).
+ * Results are bit-identical to one dtb_run of the same steps. */
+typedef int (*dtb_run_chunk_fn)(void *user, int32_t first_iter, int32_t rows, const dtb_run_outputs *out);
+int dtb_run_stream(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, int32_t chunk,
+                   dtb_run_chunk_fn fn, void *user);
+
 /* Batched stepping of every instance of the engine with its last run
  * configuration (t_steps, OU), no per-step outputs; graph_chunk > 1 replays
  * chunks of that many steps from a captured HIP graph. */

@@ -819,3 +819,46 @@ def test_gather_device_generated_stress_shape(dtb, gpu):
     assert np.array_equal(f0, f1)
     for key in ("V", "J", "i_ou", "I_syn"):
         assert np.array_equal(bits(s0[key]), bits(s1[key])), key
+
+
+def test_run_stream_equals_run(dtb, orc, gpu):
+    """Block.run_stream (dtb_run_stream, chunks of 37 steps) yields, step by
+    step, what one run() returns, and both match the oracle; closing the
+    stream early stops the run after the chunk in progress."""
+    n, steps, seed = 6000, 300, 23
+    net = dtb.gen_uniform(n, 120, seed=14)
+    ids = np.arange(0, n, 5, dtype=np.uint32)
+
+    def block():
+        b = dtb.Block(net, delta_t=DT, seed=seed)
+        b.update_ou_background_stimuli(b.subblk_id, 4.0, 1.5, 0.1)
+        b.set_samples(ids)
+        return b
+
+    kw = dict(freqs=True, output_sample_spike=True, output_sample_vmemb=True, output_sample_isynaptic=True,
+              vmean=True, imean=True, receptor_imean=True)
+    b1 = block()
+    whole = b1.run(steps, **kw)
+    s1 = b1.state()
+    b1.engine.close()
+    b2 = block()
+    rows = list(b2.run_stream(steps, chunk=37, **kw))
+    s2 = b2.state()
+    assert [k for k, _ in rows] == list(range(steps))
+    for key in ("freq", "spike", "vmemb", "isynaptic", "vmean", "imean"):
+        got = np.stack([r[key] for _, r in rows])
+        assert np.array_equal(bits(got), bits(whole[key])), key
+    rec = np.stack([r["receptor_imean"] for _, r in rows], axis=1)
+    assert np.array_equal(bits(rec), bits(whole["receptor_imean"]))
+    for k in ("V", "J", "I_syn", "i_ou"):
+        assert np.array_equal(bits(s1[k]), bits(s2[k])), k
+    # early stop: 2 steps consumed from chunks of 37 -> at most 2 chunks + 1 run
+    gen = b2.run_stream(1000, chunk=37)
+    next(gen)
+    next(gen)
+    gen.close()
+    assert s2["step"] + 37 <= b2.counters()[0] <= s2["step"] + 4 * 37
+    b2.engine.close()
+    ref, sim = run_oracle(orc, net, steps, seed, 1.5, vtrace=True)
+    assert np.array_equal(whole["spike"], ref["raster"][:, ids])
+    assert np.array_equal(bits(whole["vmemb"]), bits(ref["V"][:, ids]))

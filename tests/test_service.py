@@ -60,40 +60,52 @@ def test_version_errors_and_unsupported_calls(service):
 
 
 @pytest.mark.gpu
-def test_service_run_equals_block(dtb, gpu, service, tmp_path):
+def test_service_run_matches_oracle_and_streams(dtb, orc, gpu, service, tmp_path):
     """Init from reference block files, OU and hyperparameter updates and a
-    Run streamed over gRPC give the same per-step counts and samples as the
-    same calls on a Block."""
+    Run streamed over gRPC: per-step counts, sampled spikes and V and the
+    per-population V means equal the CPU oracle's for the same calls, and
+    the first RunResponse reaches the client while the run is still going
+    (chunks of 50 steps; 2000 steps of 2000 samples cannot all be buffered
+    by the transport)."""
     svc, client, s = service
     m = s.m
-    net = dtb.gen_uniform(4000, 100, seed=8)
+    n, steps = 4000, 2000
+    net = dtb.gen_uniform(n, 100, seed=8)
     path = dtb.save_npz(net, tmp_path, "single")
     init = list(client.Init(m.InitRequest(file_path=path, delta_t=0.1)))
-    assert init[0].status == 0 and init[0].neurons_per_block == 4000
+    assert init[0].status == 0 and init[0].neurons_per_block == n
     assert [i.subblk_id for i in init[0].subblk_info] == [0, 1]
     assert client.Updateoubackgroundparam(iter([m.UpdateOUBackgroundParamRequest(
         brain_id=[0, 1], mean=[1.5, 1.5], deviation=[0.1, 0.1], correlation_time=[4.0, 4.0])])).success
     assert client.Updatehyperpara(iter([m.UpdateHyperParaRequest(prop_id=[7], brain_id=[1], hpara_val=[1.25],
                                                                  assigned=False)])).success
-    ids = [3, 700, 3999]
-    assert client.Updatesample(iter([m.UpdateSampleRequest(sample_idx=ids)])).success
-    rs = list(client.Run(m.RunRequest(iter=150, output_freq=True, use_ou_background=True, output_sample_spike=True,
-                                      output_sample_vmemb=True, output_vmean=True)))
-    assert len(rs) == 150 and all(r.status == 0 for r in rs)
+    ids = np.arange(0, n, 2, dtype=np.uint32)
+    assert client.Updatesample(iter([m.UpdateSampleRequest(sample_idx=ids.tolist())])).success
+    s.run_chunk = 50
+    it = client.Run(m.RunRequest(iter=steps, output_freq=True, use_ou_background=True, output_sample_spike=True,
+                                 output_sample_vmemb=True, output_vmean=True))
+    first = next(it)
+    assert first.status == 0
+    assert s.run_started == 1 and s.run_finished == 0, "the first response came after the run had ended"
+    rs = [first] + list(it)
+    assert s.run_finished == 1
+    assert len(rs) == steps and all(r.status == 0 for r in rs)
     freq = np.stack([np.frombuffer(r.freq[0], dtype=np.int32) for r in rs])
+    sp = np.stack([np.frombuffer(r.sample_spike[0], dtype=np.uint8) for r in rs])
     vm = np.stack([np.frombuffer(r.sample_vmemb[0], dtype=np.float32) for r in rs])
     vmean = np.stack([np.frombuffer(r.vmean[0], dtype=np.float32) for r in rs])
 
-    blk = dtb.Block(net, delta_t=0.1, seed=0)
-    blk.update_ou_background_stimuli(np.array([0, 1]), 4.0, 1.5, 0.1)
-    blk.mul_property_by_subblk([1], [7], [1.25])
-    blk.set_samples(np.array(ids, np.uint32))
-    ref = blk.run(150, output_sample_spike=True, output_sample_vmemb=True, vmean=True)
-    blk.engine.close()
+    on = orc.OracleNet.from_network(net, dt=0.1, seed=0)
+    on.set_ou(on.pop_ids, 1.5, 0.1, 4.0)
+    sel = on.props[:, 3] == 1
+    on.props[sel, 10] = on.props[sel, 10] * np.float32(1.25)   # CONDUCT_G_AMPA of population 1
+    sim = orc.Sim(on, 32)
+    ref = sim.run(steps, raster=True, vtrace=True, pop_counts=True, stats=True)
     assert np.array_equal(freq, ref["freq"].astype(np.int32))
-    assert np.array_equal(vm.view(np.uint32), ref["vmemb"].view(np.uint32))
-    assert np.array_equal(vmean.view(np.uint32), ref["vmean"].view(np.uint32))
-    assert freq.sum() > 50
+    assert np.array_equal(sp, ref["raster"][:, ids])
+    assert np.array_equal(vm.view(np.uint32), ref["V"][:, ids].view(np.uint32))
+    assert np.array_equal(vmean.view(np.uint32), ref["stat_means"][:, :, 0].view(np.uint32))
+    assert freq.sum() > 500
 
 
 def test_freq_char_rounds_half_away_and_excludes_blocked(dtb):
