@@ -277,6 +277,9 @@ def main():
                     help="serving requests (the serving workload, and the block line's serving leg; 0 = skip "
                          "the leg)")
     ap.add_argument("--neurons", type=int, default=0, help="override the workload's neuron count")
+    ap.add_argument("--weights", choices=("float32", "float16", "uint8"), default="float32",
+                    help="weight pairs kept on the device (uint8: codes round(255 w), as the reference's uint8 "
+                         "blocks; 2 bytes per synapse instead of 8)")
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
@@ -312,6 +315,9 @@ def main():
         args.warmup = 1000
     t0 = time.perf_counter()
     net, mu, sigma, sim_seed, desc = make_workload(args.workload, args.neurons)
+    if args.weights != "float32":
+        net = net.with_weights(args.weights)
+        desc += ", %s weights" % args.weights
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
@@ -360,17 +366,22 @@ def main():
 
     ev_step, sp_step = events / args.steps, spikes / args.steps
     us = ms_span * 1e3 / args.steps
-    algo_bytes = ALGO_NEURON_BYTES * net.n + ALGO_ROW_BYTES * sp_step + ALGO_SYN_BYTES * ev_step
+    # a delivered synapse reads its column index and its weight pair as stored
+    wpair = net.w.dtype.itemsize * 2
+    syn_bytes = ALGO_SYN_BYTES - 8 + wpair
+    algo_bytes = ALGO_NEURON_BYTES * net.n + ALGO_ROW_BYTES * sp_step + syn_bytes * ev_step
     gather = layout_used == "gather"
     if gather:
         ngroups = (net.n + 2047) // 2048
         layout_bytes = (GATHER_NEURON_BYTES * net.n + (GATHER_ROW_BYTES + GATHER_ROW_GROUP_BYTES * ngroups) * sp_step
-                        + GATHER_SYN_BYTES * ev_step)
+                        + (GATHER_SYN_BYTES - 8 + wpair) * ev_step)
     else:
-        layout_bytes = LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step + LAYOUT_SYN_BYTES * ev_step
+        layout_bytes = (LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step
+                        + (LAYOUT_SYN_BYTES - 8 + wpair) * ev_step)
     kname = "k_gather+k_step" if gather else "k_step"
     achieved = algo_bytes / (us * 1e-6) / 1e9
-    traffic = profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup)
+    traffic = (profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup)
+               if args.weights == "float32" else None)
     eager = args.steps < GRAPH_CHUNK
 
     res = {
@@ -386,8 +397,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded %s network, random weights k*2^-24)" % ("voxel" if args.workload == "block"
-                                                                         else "uniform"),
+        "data": "synthetic (seeded %s network, random weights k*2^-24%s)" % (
+            "voxel" if args.workload == "block" else "uniform",
+            "" if args.weights == "float32" else ", stored as %s" % args.weights),
         "config": {"workload": desc, "neurons": net.n, "synapses": net.nsyn, "parallelism": "single GPU"},
         "realtime_factor": rtf,
         "mean_rate_hz": rate_hz,

@@ -63,6 +63,7 @@ class Network(ctypes.Structure):
         ("colinds", c_void_p),
         ("weights", c_void_p),
         ("connkinds", c_void_p),
+        ("weight_type", c_int32),
     ]
 
 
@@ -142,6 +143,8 @@ _SIGS = {
                                         POINTER(c_int)]),
     "dtb_npz_size": (c_int, [c_char_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_int32)]),
     "dtb_npz_read": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dtb_npz_weight_type": (c_int, [c_char_p, POINTER(c_int32)]),
+    "dtb_npz_read_typed": (c_int, [c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(sorted(_SIGS))

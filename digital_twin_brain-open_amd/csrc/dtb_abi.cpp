@@ -140,6 +140,10 @@ int dtb_instance_add(dtb_engine_t h, const dtb_network *net, float dt, uint64_t 
         nh.col = net->colinds;
         nh.w = net->weights;
         nh.kind_bits = net->connkinds;
+        need(net->weight_type >= DTB_WEIGHTS_F32 && net->weight_type <= DTB_WEIGHTS_U8, "unknown weight type");
+        nh.wfmt = net->weight_type == DTB_WEIGHTS_F16 ? dtb::kWeightF16
+                  : net->weight_type == DTB_WEIGHTS_U8 ? dtb::kWeightU8
+                                                       : dtb::kWeightF32;
         auto inst = std::make_unique<dtb::Instance>(nh, dt, seed, e.admit_stream());
         *inst_id = e.add(std::move(inst));
     });
@@ -665,6 +669,28 @@ int dtb_npz_read(const char *path, float *props, int64_t *rowptr, uint32_t *coli
     return guarded([&] {
         need(path && props && rowptr && colinds && weights && connkinds, "output arrays are required");
         dtb::npz_block_read(path, props, rowptr, colinds, weights, connkinds, block_begin);
+    });
+}
+
+int dtb_npz_weight_type(const char *path, int32_t *weight_type)
+{
+    return guarded([&] {
+        need(path && weight_type, "path and weight_type are required");
+        const uint32_t f = dtb::npz_weight_format(path);
+        *weight_type = f == dtb::kWeightF16 ? DTB_WEIGHTS_F16 : f == dtb::kWeightU8 ? DTB_WEIGHTS_U8 : DTB_WEIGHTS_F32;
+    });
+}
+
+int dtb_npz_read_typed(const char *path, float *props, int64_t *rowptr, uint32_t *colinds, void *weights,
+                       int32_t weight_type, uint32_t *connkinds, int64_t *block_begin)
+{
+    return guarded([&] {
+        need(path && props && rowptr && colinds && weights && connkinds, "output arrays are required");
+        need(weight_type >= DTB_WEIGHTS_F32 && weight_type <= DTB_WEIGHTS_U8, "unknown weight type");
+        const uint32_t f = weight_type == DTB_WEIGHTS_F16 ? dtb::kWeightF16
+                           : weight_type == DTB_WEIGHTS_U8 ? dtb::kWeightU8
+                                                           : dtb::kWeightF32;
+        dtb::npz_block_read_typed(path, props, rowptr, colinds, weights, f, connkinds, block_begin);
     });
 }
 

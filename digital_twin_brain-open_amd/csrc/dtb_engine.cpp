@@ -103,8 +103,11 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
         throw Error(kInvalidParameter, "props and rowptr are required");
     if (net.nsyn < 0 || (net.nsyn > 0 && (net.col == nullptr || net.w == nullptr || net.kind_bits == nullptr)))
         throw Error(kInvalidParameter, "synapse arrays are required");
+    if (net.wfmt != kWeightF32 && net.wfmt != kWeightF16 && net.wfmt != kWeightU8)
+        throw Error(kInvalidParameter, "unknown weight format");
     n_ = (uint32_t)net.n;
     nsyn_ = net.nsyn;
+    wfmt_ = net.wfmt;
 
     // CSR validation (rows ascending, columns in range).
     if (net.rowptr[0] != 0 || net.rowptr[net.n] != net.nsyn)
@@ -137,7 +140,7 @@ Instance::Instance(const NetworkHost &net, float dt, uint64_t seed, hipStream_t 
         DTB_HIP(hipStreamSynchronize(s));
     }
     if (nsyn_ > 0)
-        w_.upload(reinterpret_cast<const float2 *>(net.w), (size_t)nsyn_, s);
+        w_.upload(static_cast<const uint8_t *>(net.w), (size_t)nsyn_ * weight_bytes(wfmt_), s);
     count_tile_synapses(net);
     DTB_HIP(hipStreamSynchronize(s));
 }
@@ -187,7 +190,7 @@ void Instance::init_common(std::vector<float> &&props, float dt, uint64_t seed, 
     spill_.resize((size_t)n_ * 8);
     rowptr_.resize((size_t)n_ + 1);
     col_.resize((size_t)std::max<int64_t>(nsyn_, 1));
-    w_.resize((size_t)std::max<int64_t>(nsyn_, 1));
+    w_.resize((size_t)std::max<int64_t>(nsyn_, 1) * weight_bytes(wfmt_));
     pops_.resize(np);
     step_.resize(2);
     counters_.resize(2 * (size_t)((n_ + kWave - 1) / kWave));
@@ -210,11 +213,12 @@ Instance::Instance(const FanoutSpec &f, float dt, uint64_t seed, hipStream_t s)
     // the generation on the same stream).
     rowptr_.resize((size_t)n_ + 1);
     col_.resize((size_t)nsyn_);
-    w_.resize((size_t)nsyn_);
+    wfmt_ = kWeightF32;
+    w_.resize((size_t)nsyn_ * weight_bytes(wfmt_));
     const uint32_t ntile = (n_ + (1u << kTileBits) - 1) >> kTileBits;
     DataAllocator<uint32_t> tsyn(ntile);
     DTB_HIP(launch_gen_fanout(n_, f.K, f.net_seed, rowptr_.mutable_gpu_data(), col_.mutable_gpu_data(),
-                              w_.mutable_gpu_data(), tsyn.mutable_gpu_data(), s));
+                              reinterpret_cast<float2 *>(w_.mutable_gpu_data()), tsyn.mutable_gpu_data(), s));
     std::vector<float> props((size_t)n_ * kNumCols);
     gen_default_props(f.n, f.g_ui, f.sub_base, props.data());
     init_common(std::move(props), dt, seed, s);
@@ -333,7 +337,7 @@ uint64_t Instance::pull_bytes() const
 {
     const uint64_t ntile = (n_ + kBlock - 1) / kBlock;
     const uint64_t nw = (n_ + kWave - 1) / kWave;
-    return (uint64_t)nsyn_ * (sizeof(uint32_t) + sizeof(float2)) + (ntile + 1) * sizeof(int64_t) +
+    return (uint64_t)nsyn_ * (sizeof(uint32_t) + weight_bytes(wfmt_)) + (ntile + 1) * sizeof(int64_t) +
            4 * nw * sizeof(unsigned long long) + 2 * sizeof(int64_t);
 }
 
@@ -356,7 +360,7 @@ void Instance::ensure_pull(hipStream_t s)
         cptr_.resize((size_t)ntile + 1);
         cptr_.upload(ptr.data(), ptr.size(), s);
         ckey_.resize((size_t)nsyn_);
-        cw_.resize((size_t)nsyn_);
+        cw_.resize((size_t)nsyn_ * weight_bytes(wfmt_));
         ebits_.resize(2 * nw);
         dbits_.resize(2 * nw);
         dstamp_.resize(2);
@@ -366,8 +370,8 @@ void Instance::ensure_pull(hipStream_t s)
         DataAllocator<unsigned long long> cursor(ntile);
         std::vector<unsigned long long> cur(ptr.begin(), ptr.end() - 1);
         cursor.upload(cur.data(), ntile, s);
-        DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), n_, cursor.mutable_gpu_data(),
-                                  ckey_.mutable_gpu_data(), cw_.mutable_gpu_data(), s));
+        DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), wfmt_, n_,
+                                  cursor.mutable_gpu_data(), ckey_.mutable_gpu_data(), cw_.mutable_gpu_data(), s));
         DTB_HIP(hipStreamSynchronize(s));
     } catch (...) {
         release_pull();
@@ -965,6 +969,7 @@ InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer
     d.rowptr = rowptr_.gpu_data();
     d.col = col_.gpu_data();
     d.w = w_.gpu_data();
+    d.wfmt = wfmt_;
     d.pops = pops_.gpu_data();
     d.rec_counts = (rec_freq_ && rec_len_ > 0) ? rec_counts_.mutable_gpu_data() : nullptr;
     d.rec_stat = (rec_stat_mask_ != 0 && rec_len_ > 0) ? rec_stat_.mutable_gpu_data() : nullptr;

@@ -196,8 +196,9 @@ struct NetworkHost {
     int64_t nsyn = 0;
     const int64_t *rowptr = nullptr;
     const uint32_t *col = nullptr;
-    const float *w = nullptr;           // [nsyn][2]
+    const void *w = nullptr;            // [nsyn][2] in format wfmt
     const uint32_t *kind_bits = nullptr;  // [ceil(nsyn/32)]
+    uint32_t wfmt = kWeightF32;         // kWeightF32 / kWeightF16 / kWeightU8
 };
 
 // Per-run options (protos/snn.proto:63-77 RunRequest).
@@ -230,6 +231,7 @@ public:
 
     uint32_t n() const { return n_; }
     int64_t nsyn() const { return nsyn_; }
+    uint32_t weight_format() const { return wfmt_; }
     uint32_t npop() const { return (uint32_t)pop_ids_.size(); }
     const std::vector<uint32_t> &pop_ids() const { return pop_ids_; }
     const std::vector<uint32_t> &pop_begin() const { return pop_begin_; }
@@ -370,7 +372,8 @@ private:
     uint64_t qslots_ = 0;
     DataAllocator<int64_t> rowptr_;
     DataAllocator<uint32_t> col_;
-    DataAllocator<float2> w_;
+    DataAllocator<uint8_t> w_;                     // [nsyn] weight pairs in format wfmt_
+    uint32_t wfmt_ = kWeightF32;
     DataAllocator<PopDev> pops_;
     DataAllocator<int64_t> step_;                   // [2]
     DataAllocator<unsigned long long> counters_;   // per wave: events, spikes
@@ -392,7 +395,7 @@ private:
     DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]
     bool pull_built_ = false;             // ckey_, cw_, cptr_, bitmaps built
     DataAllocator<uint32_t> ckey_;        // [nsyn] in-edges: source row | target in tile | channel bit
-    DataAllocator<float2> cw_;            // [nsyn] their weight pairs
+    DataAllocator<uint8_t> cw_;           // [nsyn] their weight pairs (format wfmt_)
     DataAllocator<int64_t> cptr_;         // [ntiles + 1] first in-edge of each target tile
     DataAllocator<unsigned long long> ebits_, dbits_;   // [2][ceil(n / 64)] emitted / deferred rows
     DataAllocator<int64_t> dstamp_;       // [2] consuming step of deferred rows

@@ -80,6 +80,71 @@ __device__ __forceinline__ unsigned long long f32_to_q40(float w)
 }
 
 
+// uint8 weight codes: u -> (float)u / 255.f, the correctly rounded quotient
+// (presynaptic_voltage.cpp:171-176), folded at compile time (IEEE division).
+struct U8Weights {
+    float v[256];
+};
+constexpr U8Weights make_u8_weights()
+{
+    U8Weights t{};
+    for (int u = 0; u < 256; ++u)
+        t.v[u] = (float)u / 255.0f;
+    return t;
+}
+__constant__ U8Weights c_u8_weight = make_u8_weights();
+
+// Weight pairs (presynaptic_voltage.cpp:153-176): raw bits of pair j of an
+// array in format fmt (at most 64 bits) and the pair they encode.
+__device__ __forceinline__ unsigned long long w_raw(const void *w, size_t j, uint32_t fmt)
+{
+    if (fmt == kWeightF32)
+        return gp(static_cast<const unsigned long long *>(w))[j];
+    if (fmt == kWeightF16)
+        return gp(static_cast<const uint32_t *>(w))[j];
+    return gp(static_cast<const uint16_t *>(w))[j];
+}
+
+__device__ __forceinline__ float2 w_decode(unsigned long long b, uint32_t fmt)
+{
+    if (fmt == kWeightF32)
+        return make_float2(__uint_as_float((uint32_t)b), __uint_as_float((uint32_t)(b >> 32)));
+    if (fmt == kWeightF16)
+        return make_float2((float)__builtin_bit_cast(_Float16, (uint16_t)b),
+                           (float)__builtin_bit_cast(_Float16, (uint16_t)(b >> 16)));
+    // uchar2 codes over 255
+    return make_float2(c_u8_weight.v[b & 0xffu], c_u8_weight.v[(b >> 8) & 0xffu]);
+}
+
+// K weight pairs at indices jj[] of array w (one branch on the format for
+// all K loads, so they are in flight together).
+template <uint32_t K, typename Idx>
+__device__ __forceinline__ void w_load(unsigned long long (&out)[K], const void *w, const Idx (&jj)[K],
+                                       uint32_t fmt)
+{
+    if (fmt == kWeightF32) {
+        const auto p = gp(static_cast<const unsigned long long *>(w));
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            out[k] = p[jj[k]];
+    } else if (fmt == kWeightF16) {
+        const auto p = gp(static_cast<const uint32_t *>(w));
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            out[k] = p[jj[k]];
+    } else {
+        const auto p = gp(static_cast<const uint16_t *>(w));
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k)
+            out[k] = p[jj[k]];
+    }
+}
+
+__device__ __forceinline__ const void *w_row(const void *w, int64_t first, uint32_t fmt)
+{
+    return static_cast<const char *>(w) + (size_t)first * weight_bytes(fmt);
+}
+
 // Row of the parameter records that holds neuron li's parameters.
 __device__ __forceinline__ uint32_t param_index(const InstDev &I, uint32_t li)
 {
@@ -153,17 +218,11 @@ __device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDe
 // in-edges are read coalesced, the source's bit from a per-wave bitmap
 // (n / 8 bytes, cache resident), and the weight pair only for a set bit.
 // Called by the whole workgroup.
-__device__ __forceinline__ void pull_tile(unsigned long long *acc, const InstDev &I, uint32_t set, uint32_t t,
-                                          bool all, uint32_t tid)
+template <uint32_t kFmt>
+__device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_t *key, const void *cw,
+                                          const unsigned long long *bits, uint32_t len, uint32_t tid)
 {
     constexpr uint32_t U = 4;   // in-edges per thread in flight
-    const size_t nw = (I.n + kWave - 1) / kWave;
-    const auto bits = gp(all ? I.ebits : I.dbits) + (size_t)set * nw;
-    const int64_t b = gp(I.cptr)[t];
-    const uint32_t len = (uint32_t)(gp(I.cptr)[t + 1] - b);   // a tile has fewer than 2^32 in-edges
-    // uniform bases, 32-bit offsets per lane
-    const auto key = gp(I.ckey) + b;
-    const auto cw = gp(reinterpret_cast<const unsigned long long *>(I.cw)) + b;
     for (uint32_t k0 = tid; k0 < len; k0 += kBlock * U) {
         uint32_t kv[U];
         bool hit[U];
@@ -171,23 +230,44 @@ __device__ __forceinline__ void pull_tile(unsigned long long *acc, const InstDev
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t j = k0 + u * kBlock;
             hit[u] = j < len;
-            kv[u] = key[hit[u] ? j : len - 1u];
+            kv[u] = gp(key)[hit[u] ? j : len - 1u];
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t r = kv[u] & kPullRowMask;
-            hit[u] = hit[u] && ((bits[r >> 6] >> (r & 63u)) & 1ull) != 0ull;
+            hit[u] = hit[u] && ((gp(bits)[r >> 6] >> (r & 63u)) & 1ull) != 0ull;
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             if (hit[u]) {
-                const unsigned long long wb = cw[k0 + u * kBlock];
+                const float2 wp = w_decode(w_raw(cw, k0 + u * kBlock, kFmt), kFmt);
                 const uint32_t slot = ((kv[u] >> kPullRowBits) & 0xffu) * 4u + ((kv[u] & kKindBit) ? 2u : 0u);
-                atomicAdd(&acc[slot], f32_to_q40(__uint_as_float((uint32_t)wb)));
-                atomicAdd(&acc[slot + 1u], f32_to_q40(__uint_as_float((uint32_t)(wb >> 32))));
+                atomicAdd(&acc[slot], f32_to_q40(wp.x));
+                atomicAdd(&acc[slot + 1u], f32_to_q40(wp.y));
             }
         }
     }
+}
+
+// Not inlined: the pull is the rare path (volleys, overflowing queues), and
+// as a call it keeps its registers out of the step kernel's budget.
+__device__ __attribute__((noinline)) void pull_tile(unsigned long long *acc, const InstDev &I, uint32_t set,
+                                                    uint32_t t, bool all, uint32_t tid)
+{
+    const size_t nw = (I.n + kWave - 1) / kWave;
+    const unsigned long long *bits = (all ? I.ebits : I.dbits) + (size_t)set * nw;
+    const int64_t b = gp(I.cptr)[t];
+    const uint32_t len = (uint32_t)(gp(I.cptr)[t + 1] - b);   // a tile has fewer than 2^32 in-edges
+    // uniform bases, 32-bit offsets per lane; one loop per weight format
+    const uint32_t *key = I.ckey + b;
+    const uint32_t fmt = I.wfmt;
+    const void *cw = w_row(I.cw, b, fmt);
+    if (fmt == kWeightF32)
+        pull_loop<kWeightF32>(acc, key, cw, bits, len, tid);
+    else if (fmt == kWeightF16)
+        pull_loop<kWeightF16>(acc, key, cw, bits, len, tid);
+    else
+        pull_loop<kWeightU8>(acc, key, cw, bits, len, tid);
 }
 
 // A statistic's value as the exact integer rint(x * 2^24).
@@ -290,7 +370,7 @@ __device__ __forceinline__ void count_over(StepCtl *__restrict__ ctl, uint32_t o
 // bin_pass with the loads and reservations of a lane's synapses one after the
 // other (measured faster for one synapse per lane).
 template <uint32_t K>
-__device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+__device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col_all, const void *w_all, uint32_t wfmt,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, unsigned long long *__restrict__ spill,
@@ -299,7 +379,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
 {
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const auto col = gp(col_all) + row_begin;
-    const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
+    const void *wt = w_row(w_all, row_begin, wfmt);
     {
         const uint32_t j0 = c0 + lane * K;
         uint32_t cv[K], tl[K];
@@ -308,8 +388,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
         for (uint32_t k = 0; k < K; ++k) {
             if (j0 + k < len) {
                 cv[k] = col[j0 + k];
-                const unsigned long long wb = wt[j0 + k];
-                wv[k] = make_float2(__uint_as_float((uint32_t)wb), __uint_as_float((uint32_t)(wb >> 32)));
+                wv[k] = w_decode(w_raw(wt, j0 + k, wfmt), wfmt);
                 tl[k] = (cv[k] & kColMask) >> kTileBits;
             } else {
                 cv[k] = 0u;
@@ -398,7 +477,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
 // instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
 // does not depend on which path a synapse took).
 template <uint32_t K>
-__device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const float2 *__restrict__ w_all,
+__device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const void *w_all, uint32_t wfmt,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
                                         QSyn *__restrict__ queue, unsigned long long *__restrict__ spill,
@@ -407,22 +486,25 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
 {
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const auto col = gp(col_all) + row_begin;
-    const auto wt = gp(reinterpret_cast<const unsigned long long *>(w_all)) + row_begin;
+    const void *wt = w_row(w_all, row_begin, wfmt);
     const uint32_t j0 = c0 + lane * K;
     // Every lane loads K synapses without a branch (past the row's end it
     // re-reads the last one and marks it invalid), so all loads are in
     // flight together.
-    uint32_t cv[K], tl[K];
+    uint32_t cv[K], tl[K], jj[K];
     float2 wv[K];
     bool ok[K];
+    unsigned long long wr[K];
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
         ok[k] = j0 + k < len;
-        const uint32_t jj = ok[k] ? j0 + k : len - 1u;
-        cv[k] = col[jj];
-        const unsigned long long wb = wt[jj];
-        wv[k] = make_float2(__uint_as_float((uint32_t)wb), __uint_as_float((uint32_t)(wb >> 32)));
+        jj[k] = ok[k] ? j0 + k : len - 1u;
+        cv[k] = col[jj[k]];
     }
+    w_load<K, uint32_t>(wr, wt, jj, wfmt);
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k)
+        wv[k] = w_decode(wr[k], wfmt);
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k)
         tl[k] = ok[k] ? (cv[k] & kColMask) >> kTileBits : 0xfffffff0u + k;
@@ -526,10 +608,34 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     const size_t cslot_in = count_slot((size_t)par * qs.ntiles + blockIdx.x, qs.cshift);
     QSyn *__restrict__ q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
 
-    // Loads are issued in order of their dependence chains: the queue count
-    // (kernel arguments only), the parameter class, the neuron state, the
-    // parameter records (need the class), the queue head.
+    // A tile that pulls (the pull propagation, never with gather: its queue
+    // overflowed, or rows were deferred into this set; rare) accumulates its
+    // input first, before its state is loaded, so the pull's loads do not
+    // compete with the state's registers.
     const uint32_t cnt_raw = qs.count[cslot_in];
+    const int64_t step = gp(I.step)[par];
+    const bool pull_on = I.ckey != nullptr;
+    const bool over = !gat && cnt_raw > d.qcap;
+    const bool pull = pull_on && (over || gp(I.dstamp)[par] == step + 1);
+    acc[tid * 4 + 0] = 0ull;
+    acc[tid * 4 + 1] = 0ull;
+    acc[tid * 4 + 2] = 0ull;
+    acc[tid * 4 + 3] = 0ull;
+    if (tid == 0)
+        nrows = 0u;
+    if (pull) {
+        __syncthreads();
+        // An overflowed queue is ignored: every emitted row then comes
+        // through the in-edges.
+        accumulate_tile(acc, q_in + d.qoff, over ? 0u : cnt_raw, QSyn{}, 0u, tid);
+        pull_tile(acc, I, par, d.begin >> kTileBits, over, tid);
+        if (tid == 0)
+            atomicAdd(&ctl->pulled, 1ull);
+    }
+
+    // Otherwise loads are issued in order of their dependence chains: the
+    // parameter class, the neuron state, the parameter records (need the
+    // class), the queue head.
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
     const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
@@ -555,28 +661,10 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         dl = ld4(I.dlt, lc);
     else
         head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
-    const int64_t step = gp(I.step)[par];
-    // pull propagation (never with gather): rows deferred into this set
-    const bool pull_on = I.ckey != nullptr;
-    const int64_t dst = pull_on ? gp(I.dstamp)[par] : -1;
-
-    acc[tid * 4 + 0] = 0ull;
-    acc[tid * 4 + 1] = 0ull;
-    acc[tid * 4 + 2] = 0ull;
-    acc[tid * 4 + 3] = 0ull;
-    if (tid == 0)
-        nrows = 0u;
-    // A queue past its capacity is ignored when the tile can pull: every
-    // emitted row then comes through the in-edges.
-    const bool over = !gat && cnt_raw > d.qcap;
-    const bool pull = pull_on && (over || dst == step + 1);
-    const uint32_t cnt = (gat || (pull_on && over)) ? 0u : min(cnt_raw, d.qcap);
-    __syncthreads();
-    accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
-    if (pull) {
-        pull_tile(acc, I, par, d.begin >> kTileBits, over, tid);
-        if (tid == 0)
-            atomicAdd(&ctl->pulled, 1ull);
+    if (!pull) {
+        const uint32_t cnt = gat ? 0u : min(cnt_raw, d.qcap);
+        __syncthreads();
+        accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
     }
     __syncthreads();
     if (!pull_on && over && active)
@@ -772,12 +860,12 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             if ((pass & 3u) == w)
             {
                 if constexpr (K == 1)
-                    bin_pass_serial<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                    bin_pass_serial<K>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                        (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
                                        spill_out, ctl, seg_dst + w * kWave * K,
                                        seg_room + w * kWave * K, lane);
                 else
-                    bin_pass<K>(I.col, I.w, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                    bin_pass<K>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                 (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
                                 spill_out, ctl, seg_dst + w * kWave * K,
                                 seg_room + w * kWave * K, lane);
@@ -861,7 +949,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
     __syncthreads();
     const auto fired = gp(I.fired) + (size_t)par * kFiredLists * I.fcap;
     const auto col = gp(I.col);
-    const auto wt = gp(reinterpret_cast<const unsigned long long *>(I.w));
+    const uint32_t wfmt = I.wfmt;
     const uint32_t stride = I.ngroups + 1u;
     const uint32_t first = d.g << kGroupBits;
     for (uint32_t c0 = wv * kWave; c0 < nf; c0 += kGatherBlock) {
@@ -895,6 +983,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (uint32_t it = 0; it < total; it += kGatherItems * kWave) {
             uint32_t cv[kGatherItems];
+            size_t jw[kGatherItems];
             unsigned long long wb[kGatherItems];
             bool ok[kGatherItems];
 #pragma unroll
@@ -912,14 +1001,16 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
                         lo += st;
                 const size_t j = (size_t)s_base[wv][lo] + (ic - s_off[wv][lo]);
                 cv[u] = col[j];
-                wb[u] = wt[j];
+                jw[u] = j;
             }
+            w_load<kGatherItems, size_t>(wb, I.w, jw, wfmt);
 #pragma unroll
             for (uint32_t u = 0; u < kGatherItems; ++u) {
                 if (ok[u]) {
+                    const float2 wp = w_decode(wb[u], wfmt);
                     const uint32_t sl = ((cv[u] & kColMask) - first) * 4u + ((cv[u] & kKindBit) ? 2u : 0u);
-                    atomicAdd(&acc[sl], f32_to_q40(__uint_as_float((uint32_t)wb[u])));
-                    atomicAdd(&acc[sl + 1u], f32_to_q40(__uint_as_float((uint32_t)(wb[u] >> 32))));
+                    atomicAdd(&acc[sl], f32_to_q40(wp.x));
+                    atomicAdd(&acc[sl + 1u], f32_to_q40(wp.y));
                 }
             }
         }
@@ -965,9 +1056,9 @@ __global__ __launch_bounds__(kBlock) void k_build_segidx(const int64_t *__restri
 // it does not matter.
 __global__ __launch_bounds__(kBlock) void k_build_pull(const int64_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
-                                                       const float2 *__restrict__ w, uint32_t n,
+                                                       const void *__restrict__ w, uint32_t wfmt, uint32_t n,
                                                        unsigned long long *__restrict__ cursor,
-                                                       uint32_t *__restrict__ ckey, float2 *__restrict__ cw)
+                                                       uint32_t *__restrict__ ckey, void *__restrict__ cw)
 {
     const uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const uint32_t lane = __lane_id();
@@ -980,7 +1071,13 @@ __global__ __launch_bounds__(kBlock) void k_build_pull(const int64_t *__restrict
         const uint32_t t = (c & kColMask) >> kTileBits;
         const unsigned long long pos = atomicAdd(&cursor[t], 1ull);
         ckey[pos] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
-        cw[pos] = w[rb + j];
+        const unsigned long long b = w_raw(w, (size_t)(rb + j), wfmt);
+        if (wfmt == kWeightF32)
+            static_cast<unsigned long long *>(cw)[pos] = b;
+        else if (wfmt == kWeightF16)
+            static_cast<uint32_t *>(cw)[pos] = (uint32_t)b;
+        else
+            static_cast<uint16_t *>(cw)[pos] = (uint16_t)b;
     }
 }
 
@@ -1233,14 +1330,14 @@ hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint3
     return hipGetLastError();
 }
 
-hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const float2 *w, uint32_t n,
-                             unsigned long long *cursor, uint32_t *ckey, float2 *cw, hipStream_t stream)
+hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const void *w, uint32_t wfmt, uint32_t n,
+                             unsigned long long *cursor, uint32_t *ckey, void *cw, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
     const uint32_t rows_per_block = kBlock / kWave;
     const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
-    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, n, cursor, ckey, cw);
+    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, wfmt, n, cursor, ckey, cw);
     return hipGetLastError();
 }
 

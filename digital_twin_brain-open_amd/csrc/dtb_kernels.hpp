@@ -31,8 +31,8 @@ hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint3
 
 // In-edges of the pull propagation from the presynaptic-major CSR: cursor
 // [ntiles] holds each target tile's first in-edge and is advanced.
-hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const float2 *w, uint32_t n,
-                             unsigned long long *cursor, uint32_t *ckey, float2 *cw, hipStream_t stream);
+hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const void *w, uint32_t wfmt, uint32_t n,
+                             unsigned long long *cursor, uint32_t *ckey, void *cw, hipStream_t stream);
 
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,

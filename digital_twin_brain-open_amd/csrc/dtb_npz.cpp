@@ -439,9 +439,40 @@ void npz_block_size(const std::string &path, int64_t *n, int64_t *nsyn, int32_t 
         *nblocks = (int32_t)files.size();
 }
 
+uint32_t npz_weight_format(const std::string &path)
+{
+    const auto files = npz_block_files(path);
+    uint32_t fmt = kWeightF32;
+    bool first = true;
+    for (const auto &f : files) {
+        NpzArchive z(f);
+        if (block_shape(z).conns == 0)
+            continue;
+        const NpyArray w = z.get("weight", true);
+        const uint32_t m = (w.kind() == 'f' && w.itemsize() == 2) ? kWeightF16
+                           : (w.itemsize() == 1)                   ? kWeightU8
+                                                                   : kWeightF32;
+        if (!first && m != fmt)
+            bad(f, "blocks store their weights in different types");
+        fmt = m;
+        first = false;
+    }
+    return fmt;
+}
+
 void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                     uint32_t *connkinds, int64_t *block_begin)
 {
+    npz_block_read_typed(path, props, rowptr, colinds, weights, kWeightF32, connkinds, block_begin);
+}
+
+void npz_block_read_typed(const std::string &path, float *props, int64_t *rowptr, uint32_t *colinds, void *weights,
+                          uint32_t wfmt, uint32_t *connkinds, int64_t *block_begin)
+{
+    if (wfmt != kWeightF32 && wfmt != kWeightF16 && wfmt != kWeightU8)
+        throw Error(kInvalidParameter, "unknown weight type");
+    const size_t wb = weight_bytes(wfmt);   // bytes per weight pair in the output
+    uint8_t *wout = static_cast<uint8_t *>(weights);
     const auto files = npz_block_files(path);
     const size_t B = files.size();
     std::vector<uint64_t> base(B + 1, 0);
@@ -482,6 +513,10 @@ void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint
         if (!((w.kind() == 'f' && (w.itemsize() == 8 || w.itemsize() == 4 || w.itemsize() == 2)) ||
               ((w.kind() == 'u' || w.kind() == 'i') && w.itemsize() == 1)))
             bad(files[b], "weight dtype must be float64, float32, float16 or uint8");
+        if (wfmt == kWeightF16 && !(w.kind() == 'f' && w.itemsize() == 2 && w.descr[0] != '>'))
+            bad(files[b], "float16 weights requested but the block stores another type");
+        if (wfmt == kWeightU8 && w.itemsize() != 1)
+            bad(files[b], "uint8 weights requested but the block stores another type");
         const uint64_t nb = base[b + 1] - base[b];
         for (uint64_t j = 0; j < conns[b]; ++j) {
             const int64_t src_b = (int64_t)b + int_at(bid, j);
@@ -517,8 +552,13 @@ void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint
             const uint64_t src = base[(uint64_t)((int64_t)b + int_at(bid, j))] + (uint64_t)int_at(nid, j);
             const int64_t k = fill[src]++;
             colinds[k] = (uint32_t)(base[b] + (uint64_t)int_at(oid, j));
-            weights[2 * k] = weight_at(w, 2 * j);
-            weights[2 * k + 1] = weight_at(w, 2 * j + 1);
+            if (wfmt == kWeightF32) {
+                float pair[2] = {weight_at(w, 2 * j), weight_at(w, 2 * j + 1)};
+                std::memcpy(wout + (size_t)k * wb, pair, wb);
+            } else {
+                // the stored codes (uchar2 / half2), converted on the device
+                std::memcpy(wout + (size_t)k * wb, w.data + (size_t)j * wb, wb);
+            }
             kind[(size_t)k] = int_at(ch, j) == 2 ? 1 : 0;
         }
     }
@@ -532,7 +572,7 @@ void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint
         th.emplace_back([&, t]() {
             std::vector<uint32_t> idx;
             std::vector<uint32_t> c2;
-            std::vector<float> w2;
+            std::vector<uint8_t> w2;
             std::vector<uint8_t> k2;
             const uint64_t a = n * t / nt, e = n * (t + 1) / nt;
             for (uint64_t r = a; r < e; ++r) {
@@ -545,17 +585,16 @@ void npz_block_read(const std::string &path, float *props, int64_t *rowptr, uint
                 std::stable_sort(idx.begin(), idx.end(),
                                  [&](uint32_t x, uint32_t y) { return colinds[lo + x] < colinds[lo + y]; });
                 c2.resize((size_t)len);
-                w2.resize((size_t)len * 2);
+                w2.resize((size_t)len * wb);
                 k2.resize((size_t)len);
                 for (int64_t q = 0; q < len; ++q) {
                     const int64_t s = lo + idx[(size_t)q];
                     c2[(size_t)q] = colinds[s];
-                    w2[(size_t)(2 * q)] = weights[2 * s];
-                    w2[(size_t)(2 * q + 1)] = weights[2 * s + 1];
+                    std::memcpy(w2.data() + (size_t)q * wb, wout + (size_t)s * wb, wb);
                     k2[(size_t)q] = kind[(size_t)s];
                 }
                 std::memcpy(colinds + lo, c2.data(), sizeof(uint32_t) * (size_t)len);
-                std::memcpy(weights + 2 * lo, w2.data(), sizeof(float) * 2 * (size_t)len);
+                std::memcpy(wout + (size_t)lo * wb, w2.data(), wb * (size_t)len);
                 std::memcpy(kind.data() + lo, k2.data(), (size_t)len);
             }
         });

@@ -93,7 +93,7 @@ struct alignas(16) InstDev {
     // connectivity: presynaptic-major CSR, 64-bit row offsets
     const int64_t *rowptr;      // [n+1]
     const uint32_t *col;        // [S] postsynaptic id | kKindBit
-    const float2 *w;            // [S] weight pair
+    const void *w;              // [S] weight pair in format wfmt (kWeightF32 / F16 / U8)
     // Gather propagation (null: queue propagation).  segidx[r][g] = offset in
     // row r of its first synapse whose target lies in group g or later
     // (segidx[r][ngroups] = row length).  The rows with synapses fired by the
@@ -117,7 +117,7 @@ struct alignas(16) InstDev {
     // emitted row through its in-edges) or when rows were deferred (then it
     // adds the deferred rows to its queue).
     const uint32_t *ckey;       // [S]
-    const float2 *cw;           // [S]
+    const void *cw;             // [S] weight pairs, format wfmt
     const int64_t *cptr;        // [ntiles + 1]
     unsigned long long *ebits;  // [2][ceil(n / 64)]
     unsigned long long *dbits;  // [2][ceil(n / 64)]
@@ -148,7 +148,20 @@ struct alignas(16) InstDev {
     uint32_t ngroups;           // gather groups of the instance (kGroupTiles tiles each)
     uint32_t fcap;              // rows per fired list
     uint32_t defer_min;         // pull: a tile with this many emitted rows defers them (0: never)
+    uint32_t wfmt;              // weight pair format of w and cw
+    uint32_t pad2[3];
 };
+
+// Weight pair formats kept on the device (include/weights.hpp:14-21 and the
+// decode of presynaptic_voltage.cpp:153-176): two fp32 (8 bytes), two fp16
+// (4 bytes, converted exactly) or two uint8 codes u (2 bytes, weight
+// (float)u / 255.f, correctly rounded).  Double weights are stored as fp32,
+// the type the reference's float block converts them to.
+enum : uint32_t { kWeightF32 = 0, kWeightF16 = 1, kWeightU8 = 2 };
+__host__ __device__ inline uint32_t weight_bytes(uint32_t fmt)
+{
+    return fmt == kWeightF32 ? 8u : fmt == kWeightF16 ? 4u : 2u;
+}
 
 // Pull propagation limits: source rows are 23-bit fields of the in-edge key.
 constexpr uint32_t kPullRowBits = 23;

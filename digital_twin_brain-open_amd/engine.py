@@ -26,17 +26,36 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(c_void_p)
 
 
+# Weight pair types kept on the device (include/weights.hpp:14-21; the
+# decode of presynaptic_voltage.cpp:153-176): numpy dtype -> dtb_weight_type.
+WEIGHT_TYPES = {np.dtype(np.float32): 0, np.dtype(np.float16): 1, np.dtype(np.uint8): 2}
+
+
+def decode_weights(w):
+    """fp32 weight pairs of stored pairs: float16 converted exactly, uint8
+    codes u as float32(u) / float32(255) (correctly rounded, as the
+    reference's float block decodes them)."""
+    w = np.asarray(w)
+    if w.dtype == np.uint8:
+        return w.astype(np.float32) / np.float32(255.0)
+    return w.astype(np.float32)
+
+
 class NetworkArrays:
     """A block in the reference layout (src/configuration.cpp:1079-1667):
     property matrix [n][22] float32, presynaptic-major CSR (int64 rowptr,
-    uint32 postsynaptic ids), weight pair per synapse, channel bits packed
-    32 per uint32 word."""
+    uint32 postsynaptic ids), weight pair per synapse (float32, float16, or
+    uint8 codes of weight u / 255; kept in that type on the device), channel
+    bits packed 32 per uint32 word."""
 
     def __init__(self, props, rowptr, col, w, kinds):
         self.props = np.ascontiguousarray(props, dtype=np.float32)
         self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
         self.col = np.ascontiguousarray(col, dtype=np.uint32)
-        self.w = np.ascontiguousarray(w, dtype=np.float32).reshape(-1, 2)
+        w = np.asarray(w)
+        if w.dtype not in WEIGHT_TYPES:
+            w = w.astype(np.float32)
+        self.w = np.ascontiguousarray(w).reshape(-1, 2)
         self.kinds = np.ascontiguousarray(kinds, dtype=np.uint32)
         assert self.props.ndim == 2 and self.props.shape[1] == NCOL
         assert self.rowptr.shape[0] == self.n + 1
@@ -51,6 +70,32 @@ class NetworkArrays:
     def nsyn(self):
         return int(self.col.shape[0])
 
+    @property
+    def weight_type(self):
+        return WEIGHT_TYPES[self.w.dtype]
+
+    def weights_f32(self):
+        """The weight pairs as the propagation uses them (fp32)."""
+        return decode_weights(self.w)
+
+    def with_weights(self, dtype):
+        """The same network with its weight pairs stored as dtype: "uint8"
+        (codes round(255 w), clipped to 0..255, as make_block.py writes uint8
+        blocks), "float16" or "float32" (decoded)."""
+        dtype = np.dtype(dtype)
+        if dtype == self.w.dtype:
+            w = self.w
+        elif dtype == np.uint8:
+            w = np.clip(np.rint(self.weights_f32().astype(np.float64) * 255.0), 0, 255).astype(np.uint8)
+        elif dtype in (np.float16, np.float32):
+            w = self.weights_f32().astype(dtype)
+        else:
+            raise ValueError("weights are stored as float32, float16 or uint8")
+        out = NetworkArrays(self.props, self.rowptr, self.col, w, self.kinds)
+        if hasattr(self, "voxel_sizes"):
+            out.voxel_sizes = self.voxel_sizes
+        return out
+
     def kind_per_synapse(self):
         """Channel bit of every synapse as uint8 (0: AMPA/NMDA, 1: GABA_A/GABA_B)."""
         bits = np.unpackbits(self.kinds.view(np.uint8), bitorder="little")
@@ -64,7 +109,7 @@ class NetworkArrays:
 
     def struct(self):
         return _lib.Network(self.n, _ptr(self.props), self.nsyn, _ptr(self.rowptr), _ptr(self.col),
-                            _ptr(self.w), _ptr(self.kinds))
+                            _ptr(self.w), _ptr(self.kinds), self.weight_type)
 
     @staticmethod
     def from_edges(props, pre, post, w, kind):
@@ -82,8 +127,11 @@ class NetworkArrays:
         padded = np.zeros(nw * 32, dtype=np.uint8)
         padded[: len(pre)] = kind_s
         kinds = np.packbits(padded, bitorder="little").view(np.uint32)
-        return NetworkArrays(props, rowptr, np.asarray(post, dtype=np.uint32)[order],
-                             np.asarray(w, dtype=np.float32).reshape(-1, 2)[order], kinds)
+        w = np.asarray(w)
+        if w.dtype not in WEIGHT_TYPES:
+            w = w.astype(np.float32)
+        return NetworkArrays(props, rowptr, np.asarray(post, dtype=np.uint32)[order], w.reshape(-1, 2)[order],
+                             kinds)
 
 
 def gen_uniform(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
@@ -118,22 +166,33 @@ def gen_fanout(n, k, seed, g_ui=DEFAULT_G_UI, sub_base=0):
     return NetworkArrays(props, rowptr, col, w, kinds)
 
 
-def load_npz(path):
+def load_npz(path, weights="float32"):
     """Reads the reference's block files through the native reader: one
     ``block_<i>.npz`` or a directory of ``block_0.npz .. block_<B-1>.npz``
     (python/make_block.py:608-614; src/configuration.cpp:1079-1667), blocks
-    concatenated in block order.  Returns (NetworkArrays, block_begin)."""
+    concatenated in block order.  weights: "float32" converts every weight
+    type to fp32 as the reference's propagation does; "native" keeps uint8
+    and float16 blocks in their stored codes (the device then reads 2 or 4
+    bytes per weight pair instead of 8; src/configuration.cpp:1411-1435 picks
+    the type by the block path).  Returns (NetworkArrays, block_begin)."""
     lib = _lib.load()
     p = os.fsencode(os.fspath(path))
     n, s, b = c_int64(0), c_int64(0), c_int32(0)
     _lib.check(lib.dtb_npz_size(p, byref(n), byref(s), byref(b)))
+    wt = c_int32(0)
+    if weights == "native":
+        _lib.check(lib.dtb_npz_weight_type(p, byref(wt)))
+    elif weights != "float32":
+        raise ValueError("weights must be 'float32' or 'native'")
+    wdtype = {0: np.float32, 1: np.float16, 2: np.uint8}[wt.value]
     props = np.empty((n.value, NCOL), dtype=np.float32)
     rowptr = np.empty(n.value + 1, dtype=np.int64)
     col = np.empty(s.value, dtype=np.uint32)
-    w = np.empty((s.value, 2), dtype=np.float32)
+    w = np.empty((s.value, 2), dtype=wdtype)
     kinds = np.empty(max((s.value + 31) // 32, 1), dtype=np.uint32)
     begin = np.empty(b.value + 1, dtype=np.int64)
-    _lib.check(lib.dtb_npz_read(p, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w), _ptr(kinds), _ptr(begin)))
+    _lib.check(lib.dtb_npz_read_typed(p, _ptr(props), _ptr(rowptr), _ptr(col), _ptr(w), wt.value, _ptr(kinds),
+                                      _ptr(begin)))
     return NetworkArrays(props, rowptr, col, w, kinds), begin
 
 
@@ -161,10 +220,12 @@ def save_npz(net, directory, dtype="single", block_begin=None):
         order = np.lexsort((rel, onid))
         sel, rel, onid = sel[order], rel[order], onid[order]
         w = net.w[sel]
-        if wtype is np.uint8:
-            w = np.clip(np.rint(w.astype(np.float64) * 255.0), 0, 255).astype(np.uint8)
+        if w.dtype == wtype:
+            pass   # stored codes kept as they are
+        elif wtype is np.uint8:
+            w = np.clip(np.rint(decode_weights(w).astype(np.float64) * 255.0), 0, 255).astype(np.uint8)
         else:
-            w = w.astype(wtype)
+            w = decode_weights(w).astype(wtype)
         np.savez(os.path.join(out, "block_%d" % i),
                  property=net.props[begin[i]:begin[i + 1]],
                  output_neuron_idx=onid,

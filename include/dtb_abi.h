@@ -63,20 +63,28 @@ enum dtb_prop_type {
 
 typedef struct dtb_engine_s *dtb_engine_t;
 
+/* Weight pair types kept on the device (include/weights.hpp:14-21 DataType
+ * FLOAT, FLOAT16, INT8; decoded per synapse as
+ * src/stage/presynaptic_voltage.cpp:153-176 does: fp16 converted exactly,
+ * uint8 codes u as (float)u / 255.f).  DOUBLE blocks are given as float32,
+ * the type the reference's float block converts them to. */
+enum dtb_weight_type { DTB_WEIGHTS_F32 = 0, DTB_WEIGHTS_F16 = 1, DTB_WEIGHTS_U8 = 2 };
+
 /* A network in the reference block layout (src/configuration.cpp:1079-1667,
  * 1778-2156): property matrix [n][22] float32 (columns as python/
  * make_block.py:441-465), presynaptic-major CSR with 64-bit row offsets,
- * postsynaptic ids, a weight pair per synapse (include/weights.hpp:30-37,
- * DataType FLOAT) and the channel-pair bit per synapse packed in 32-bit words
- * (bit j of word j/32; src/stage/presynaptic_voltage.cpp:152). */
+ * postsynaptic ids, a weight pair per synapse (include/weights.hpp:30-37) of
+ * type weight_type and the channel-pair bit per synapse packed in 32-bit
+ * words (bit j of word j/32; src/stage/presynaptic_voltage.cpp:152). */
 typedef struct dtb_network {
     int64_t n;
     const float *props;         /* [n][22] */
     int64_t nsyn;
     const int64_t *rowptr;      /* [n+1] */
     const uint32_t *colinds;    /* [nsyn] */
-    const float *weights;       /* [nsyn][2] */
+    const void *weights;        /* [nsyn][2] float32 / float16 / uint8 */
     const uint32_t *connkinds;  /* [ceil(nsyn/32)] */
+    int32_t weight_type;        /* dtb_weight_type (0: float32) */
 } dtb_network;
 
 /* Per-step outputs of dtb_run, caller-allocated, row-major [iters][...];
@@ -336,6 +344,17 @@ int dtb_gen_gamma(uint64_t seed, uint32_t call, uint32_t first, uint32_t count, 
 int dtb_npz_size(const char *path, int64_t *n, int64_t *nsyn, int32_t *nblocks);
 int dtb_npz_read(const char *path, float *props, int64_t *rowptr, uint32_t *colinds, float *weights,
                  uint32_t *connkinds, int64_t *block_begin);
+/* The weight type a block path keeps on the device: uint8 for uint8/int8
+ * weight files, float16 for half, float32 for single and double
+ * (src/configuration.cpp:1411-1435 picks the type by the block path; the
+ * reader takes it from the stored dtype, which all files must share for
+ * uint8 or float16). */
+int dtb_npz_weight_type(const char *path, int32_t *weight_type);
+/* dtb_npz_read with the weights written as weight_type ([nsyn][2] of
+ * float32, float16 or uint8): float16 and uint8 are the stored codes,
+ * unconverted (blocks stored in another type are rejected). */
+int dtb_npz_read_typed(const char *path, float *props, int64_t *rowptr, uint32_t *colinds, void *weights,
+                       int32_t weight_type, uint32_t *connkinds, int64_t *block_begin);
 
 #ifdef __cplusplus
 }

@@ -122,7 +122,14 @@ class OracleNet:
 
     @staticmethod
     def from_network(net, dt=0.1, t_steps=1, seed=0):
-        return OracleNet(net.props, net.rowptr, net.col, net.w, net.kind_per_synapse(), dt, t_steps, seed)
+        """net: NetworkArrays; uint8 weight codes u are decoded as
+        float32(u) / float32(255) (presynaptic_voltage.cpp:171-176), float16
+        weights converted to float32 (:165-169)."""
+        w = net.w
+        if w.dtype == np.uint8:
+            w = w.astype(np.float32) / np.float32(255.0)
+        return OracleNet(net.props, net.rowptr, net.col, w.astype(np.float32), net.kind_per_synapse(), dt, t_steps,
+                         seed)
 
     def set_ou(self, brain_ids, mean, sigma, tau):
         ids = np.atleast_1d(np.asarray(brain_ids))

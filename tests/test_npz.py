@@ -137,3 +137,50 @@ def test_rejects_malformed(dtb, tmp_path):
         dtb.load_npz(d / "junk.npz")
     with pytest.raises(dtb.DtbError):
         dtb.load_npz(tmp_path / "does_not_exist")
+
+
+def test_uint8_weight_decode_is_correctly_rounded(dtb):
+    """uint8 weight codes u decode to the fp32 value nearest to u / 255
+    (presynaptic_voltage.cpp:171-176 divides in float), for all 256 codes:
+    checked with exact rational arithmetic, independently of any float
+    division."""
+    from fractions import Fraction
+    w = dtb.engine.decode_weights(np.arange(256, dtype=np.uint8))
+    for u in range(256):
+        x = Fraction(u, 255)
+        got = Fraction(float(w[u]))
+        below = np.nextafter(np.float32(w[u]), np.float32(-1))
+        above = np.nextafter(np.float32(w[u]), np.float32(2))
+        # no float32 is closer to u / 255 than the decoded value
+        assert abs(got - x) <= abs(Fraction(float(below)) - x), u
+        assert abs(got - x) <= abs(Fraction(float(above)) - x), u
+
+
+@pytest.mark.parametrize("dtype", ["uint8", "half"])
+def test_native_weight_types_round_trip(dtb, tmp_path, dtype):
+    """A network stored as uint8 or float16 block files loads with its codes
+    unchanged (weights="native"), and the fp32 load equals the decode of those
+    codes: the reader's fp32 conversion and decode_weights (the oracle's
+    decode) agree for every synapse."""
+    np_dtype = {"uint8": np.uint8, "half": np.float16}[dtype]
+    net = dtb.gen_uniform(3000, 40, seed=5).with_weights(np_dtype)
+    assert net.w.dtype == np_dtype
+    path = dtb.save_npz(net, tmp_path, dtype, block_begin=[0, 1200, 3000])
+    native, _ = dtb.load_npz(path, weights="native")
+    f32, _ = dtb.load_npz(path)
+    assert native.w.dtype == np_dtype and f32.w.dtype == np.float32
+    assert np.array_equal(native.rowptr, net.rowptr) and np.array_equal(native.col, net.col)
+    assert np.array_equal(native.w.view(np.uint8), net.w.view(np.uint8))
+    assert np.array_equal(f32.w.view(np.uint32), dtb.engine.decode_weights(native.w).view(np.uint32))
+    assert np.array_equal(native.kinds, f32.kinds)
+
+
+def test_native_load_rejects_mixed_weight_types(dtb, tmp_path):
+    net = dtb.gen_uniform(1000, 20, seed=2)
+    d = dtb.save_npz(net, tmp_path, "single", block_begin=[0, 500, 1000])
+    # replace block 1's weights by uint8 codes: the blocks then disagree
+    z = dict(np.load(os.path.join(d, "block_1.npz")))
+    z["weight"] = np.clip(np.rint(z["weight"] * 255), 0, 255).astype(np.uint8)
+    np.savez(os.path.join(d, "block_1"), **z)
+    with pytest.raises(dtb.DtbError):
+        dtb.load_npz(d, weights="native")
