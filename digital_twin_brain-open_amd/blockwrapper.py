@@ -18,6 +18,17 @@ Arguments keep the reference's meaning:
 Inputs may be numpy arrays or torch tensors; outputs are numpy arrays.
 Several block files in one directory are concatenated into one block on the
 one GPU (their connections become ordinary synapses).
+
+Population ids follow the reference client (dist_blockwrapper.py:79-117):
+each block reports its own sub-block ids; a block whose first id is the
+cortical start (0 with overlap 2, else 2) after others is rebased above the
+largest id so far (rounded up to a multiple of overlap) when allow_rebase and
+(force_rebase or that id differs from the start).  Populations that end up
+with the same id in several blocks are one population to the caller: their
+spike counts are summed and their means weighted by neuron counts
+(_merge_sbblk, :153-177), and updates by population reach every part.  On
+the device every (block, sub-block) part stays its own population (property
+column 3 holds the part's index).
 """
 import numpy as np
 
@@ -48,16 +59,89 @@ class BlockWrapper:
             net, begin = path, np.array([0, path.n], dtype=np.int64)
         else:
             net, begin = load_npz(path)
+        begin = np.asarray(begin, dtype=np.int64)
+        part_of, part_id, part_block = self._parts(net.props[:, 3], begin, allow_rebase, force_rebase, overlap)
+        props = net.props.copy()
+        props[:, 3] = part_of.astype(np.float32)   # device populations: one per (block, sub-block) part
+        net = NetworkArrays(props, net.rowptr, net.col, net.w, net.kinds)
         self._block = Block(net, delta_t=delta_t, seed=seed, device=device)
         self._neurons_per_block = np.diff(begin).astype(np.int64)
         self._block_id = np.arange(len(self._neurons_per_block), dtype=np.int64)
         self._neurons_thrush = begin.astype(np.int64)
-        self._subblk_id = self._block.subblk_id.astype(np.int64)
-        self._neurons_per_subblk = self._block.neurons_per_subblk
+        # parts (device populations, in order) -> caller population ids
+        self._part_id = part_id
+        self._part_block = part_block
+        self._part_neurons = self._block.neurons_per_subblk.astype(np.int64)
+        assert self._part_neurons.shape[0] == part_id.shape[0]
+        self._subblk_id, self._part_row = np.unique(part_id, return_inverse=True)
+        if self._part_row.shape[0] == self._subblk_id.shape[0]:
+            self._subblk_idx = None
+            self._neurons_per_subblk = self._part_neurons
+            self.merge_popu_ratio = None
+        else:
+            self._subblk_idx = self._part_row
+            self._neurons_per_subblk = np.bincount(self._part_row, weights=self._part_neurons,
+                                                   minlength=self._subblk_id.shape[0]).astype(np.int64)
+            self.merge_popu_ratio = (self._part_neurons.astype(np.float32)
+                                     / self._neurons_per_subblk[self._part_row].astype(np.float32))
         self._reset_hyper_parameter()
         self._sample_order = None
         self._sample_num = 0
         self._iterations = 0
+
+    @staticmethod
+    def _parts(sub, begin, allow_rebase, force_rebase, overlap):
+        """Per neuron its part index; per part its caller population id and
+        block (dist_blockwrapper.py:79-98: each block's sub-block ids in
+        ascending order, rebased block by block)."""
+        sub = np.asarray(sub).astype(np.int64)
+        part_of = np.empty(sub.shape[0], dtype=np.int64)
+        part_id, part_block = [], []
+        seen = []           # caller ids so far (subblk_info)
+        base = 0
+        start = 0 if overlap == 2 else 2
+        for b in range(len(begin) - 1):
+            lo, hi = int(begin[b]), int(begin[b + 1])
+            local = np.unique(sub[lo:hi])
+            for j, sid in enumerate(local):
+                if j == 0 and sid == start and seen:
+                    new_base = max(seen)
+                    if allow_rebase and (force_rebase or new_base != start):
+                        base = (new_base + 1 + overlap - 1) // overlap * overlap
+                seen.append(int(sid) + base)
+            # parts in neuron order (a sub-block is a contiguous range of its block)
+            starts = lo + np.flatnonzero(np.r_[True, sub[lo + 1:hi] != sub[lo:hi - 1]]) if hi > lo else []
+            for k, a in enumerate(starts):
+                e = starts[k + 1] if k + 1 < len(starts) else hi
+                part_of[a:e] = len(part_id)
+                part_id.append(int(sub[a]) + base)
+                part_block.append(b)
+        return part_of, np.asarray(part_id, dtype=np.int64), np.asarray(part_block, dtype=np.int64)
+
+    def _merge_sbblk(self, array, weight=None):
+        """dist_blockwrapper.py:165-177: per-part columns [..., parts] summed
+        into the caller's populations [..., populations], weighted first when
+        given."""
+        if self._subblk_idx is None:
+            return array
+        if weight is not None:
+            array = array * weight
+        out = np.zeros(array.shape[:-1] + (self._subblk_id.shape[0],), dtype=array.dtype)
+        for k, r in enumerate(self._subblk_idx):
+            out[..., r] += array[..., k]
+        return out
+
+    def _parts_of(self, population_id):
+        """Device population ids (parts) of caller population ids, with the
+        index of the caller id each part serves."""
+        pop = np.asarray(population_id, dtype=np.int64).reshape(-1)
+        rows = self._subblk_rows(pop)
+        parts, which = [], []
+        for i, r in enumerate(rows):
+            for k in np.flatnonzero(self._part_row == r):
+                parts.append(k)
+                which.append(i)
+        return np.asarray(parts, dtype=np.uint32), np.asarray(which, dtype=np.int64)
 
     def _reset_hyper_parameter(self):
         self._last_hyper_parameter = np.ones([self._subblk_id.shape[0], 20], dtype=np.float32)
@@ -104,18 +188,19 @@ class BlockWrapper:
                               receptor_imean=receptor_imean, iou=iou, t_steps=t_steps)
         self._iterations += iterations
         seq = []
+        ratio = self.merge_popu_ratio
         if freqs:
-            f = out["freq"].astype(np.int32)
+            f = self._merge_sbblk(out["freq"].astype(np.int32))
             if freq_char:
                 # stat_result.cpp:103-130 sends lroundf(255 c / (n - blocked))
                 # per population; the client scales it back by the
-                # population size (dist_blockwrapper.py:285-289).
-                n = self._neurons_per_subblk.astype(np.float32)
+                # population size (dist_blockwrapper.py:285-292).
+                n = self._part_neurons.astype(np.float32)
                 ch = self._block.freq_char(out["freq"])
-                f = (ch.astype(np.float32) / np.float32(255.0) * n).astype(np.int32)
+                f = self._merge_sbblk(ch.astype(np.float32) / np.float32(255.0) * n).astype(np.int32)
             seq.append(f)
         if vmean:
-            seq.append(out["vmean"])
+            seq.append(self._merge_sbblk(out["vmean"], ratio))
         for flag, key in ((output_sample_spike, "spike"), (output_sample_vmemb, "vmemb"),
                           (output_sample_isynaptic, "isynaptic"), (output_sample_iou, "iou")):
             if flag:
@@ -124,9 +209,9 @@ class BlockWrapper:
                     a = a[:, self._sample_order]
                 seq.append(a)
         if imean:
-            seq.append(out["imean"])
+            seq.append(self._merge_sbblk(out["imean"], ratio))
         if receptor_imean:
-            seq.extend(out["receptor_imean"][k] for k in range(4))
+            seq.extend(self._merge_sbblk(out["receptor_imean"][k], ratio) for k in range(4))
         for j in range(iterations):
             if len(seq) == 1:
                 yield seq[0][j]
@@ -148,6 +233,7 @@ class BlockWrapper:
         if bid is not None:
             nid = nid + self._neurons_thrush[int(bid)]
         self._block.engine.update_props(self._block.inst, nid.astype(np.uint32), self._prop_ids(idx[:, 1]), w)
+        self._reset_hyper_parameter()   # dist_blockwrapper.py:432
 
     def _subblk_rows(self, pop):
         rows = np.searchsorted(self._subblk_id, pop)
@@ -165,15 +251,16 @@ class BlockWrapper:
             rows = self._subblk_rows(idx[:, 0])
             hp = hp / self._last_hyper_parameter[rows, prop]
             self._last_hyper_parameter[rows, prop] *= hp
-        self._block.engine.update_prop_cols(self._block.inst, prop, idx[:, 0].astype(np.uint32), hp,
-                                            assign=False)
+        parts, which = self._parts_of(idx[:, 0])
+        self._block.engine.update_prop_cols(self._block.inst, prop[which], parts, hp[which], assign=False)
 
     def assign_property_by_subblk(self, property_idx, property_hyper_parameter):
         """dist_blockwrapper.py:517-527."""
         idx = _np(property_idx, np.int64).reshape(-1, 2)
         hp = _np(property_hyper_parameter, np.float32).reshape(-1)
-        self._block.engine.update_prop_cols(self._block.inst, self._prop_ids(idx[:, 1]),
-                                            idx[:, 0].astype(np.uint32), hp, assign=True)
+        prop = self._prop_ids(idx[:, 1])
+        parts, which = self._parts_of(idx[:, 0])
+        self._block.engine.update_prop_cols(self._block.inst, prop[which], parts, hp[which], assign=True)
 
     def gamma_property_by_subblk(self, property_idx, gamma_concentration, gamma_rate):
         """dist_blockwrapper.py:529-554: every neuron of each listed
@@ -184,8 +271,11 @@ class BlockWrapper:
         conc = np.ones_like(rate) if gamma_concentration is None else _np(gamma_concentration, np.float32).reshape(-1)
         if not ((conc >= 0).all() and (rate >= 0).all()):
             raise ValueError("gamma hyperparameters must be non-negative")
-        self._block.engine.gamma_prop_cols(self._block.inst, self._prop_ids(idx[:, 1]),
-                                           idx[:, 0].astype(np.uint32), conc, rate)
+        prop = self._prop_ids(idx[:, 1])
+        rows = self._subblk_rows(idx[:, 0])
+        self._last_hyper_parameter[rows, prop] = 1.0   # dist_blockwrapper.py:538-540
+        parts, which = self._parts_of(idx[:, 0])
+        self._block.engine.gamma_prop_cols(self._block.inst, prop[which], parts, conc[which], rate[which])
 
     def set_samples(self, sample_idx, bid=None):
         """dist_blockwrapper.py:556-581."""
@@ -205,14 +295,17 @@ class BlockWrapper:
         each listed population."""
         pop = _np(population_id, np.int64).reshape(-1)
         assert np.unique(pop).shape[0] == pop.shape[0], "population_id should be unique!"
-        rows = self._subblk_rows(pop)
+        parts, _ = self._parts_of(pop)
+        if parts.shape[0] != pop.shape[0]:
+            raise NotImplementedError("one population has been splited into more than one sub-populations in this "
+                                      "network structure and this method is invalid in this case.")
         if sample_number is None:
-            m = min(int(self._neurons_per_subblk[rows].min()) // 2, 50)
-            sample_number = np.full(rows.shape, m, dtype=np.int64)
+            m = min(int(self._part_neurons[parts].min()) // 2, 50)
+            sample_number = np.full(parts.shape, m, dtype=np.int64)
         sample_number = _np(sample_number, np.int64).reshape(-1)
-        assert (sample_number < self._neurons_per_subblk[rows]).all(), "sample number invalid, too big to sample"
-        base = np.concatenate([[0], np.cumsum(self._neurons_per_subblk)])
-        ids = np.concatenate([np.arange(base[r], base[r] + k) for r, k in zip(rows, sample_number)])
+        assert (sample_number < self._part_neurons[parts]).all(), "sample number invalid, too big to sample"
+        base = np.concatenate([[0], np.cumsum(self._part_neurons)])
+        ids = np.concatenate([np.arange(base[r], base[r] + k) for r, k in zip(parts, sample_number)])
         self.set_samples(ids)
         return int(sample_number.sum())
 
@@ -223,11 +316,11 @@ class BlockWrapper:
         pop = pop[order]
         if np.unique(pop).shape[0] != pop.shape[0]:
             raise _lib.DtbError(_lib.INVALID_PARAMETER, "population ids must be unique")
-        self._subblk_rows(pop)
         tau = _np(correlation_time, np.float32).reshape(-1)[order]
         mu = _np(mean, np.float32).reshape(-1)[order]
         sd = _np(deviation, np.float32).reshape(-1)[order]
-        self._block.update_ou_background_stimuli(pop.astype(np.uint32), tau, mu, sd)
+        parts, which = self._parts_of(pop)
+        self._block.update_ou_background_stimuli(parts, tau[which], mu[which], sd[which])
 
     def close(self):
         if self._block is not None:
