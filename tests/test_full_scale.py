@@ -54,7 +54,7 @@ def oracle_chunks(sim, net, steps, sample=None, chunk=50):
     fired count of every step, the sampled raster and V trace ([steps][m],
     or None), the synaptic events (sum of the fired rows' out-degrees) and
     the spikes."""
-    deg = np.diff(net.rowptr).astype(np.float64)
+    deg = np.diff(net.rowptr)
     freq, fired, rs, vs = [], [], [], []
     events = spikes = 0
     for k0 in range(0, steps, chunk):
@@ -63,8 +63,10 @@ def oracle_chunks(sim, net, steps, sample=None, chunk=50):
         r = out["raster"]
         freq.append(out["freq"])
         fired.append(r.sum(axis=1, dtype=np.int64))
-        events += int(round(float((r.astype(np.float64) @ deg).sum())))
-        spikes += int(r.sum(dtype=np.int64))
+        idx = np.flatnonzero(r)
+        events += int(deg[idx % net.n].sum(dtype=np.int64))
+        spikes += int(idx.shape[0])
+        del idx
         if sample is not None:
             rs.append(r[:, sample].copy())
             vs.append(out["V"][:, sample].copy())
