@@ -41,7 +41,7 @@ PULL_BYTES_PER_SYNAPSE = 4 + 8
 
 class Request:
     __slots__ = ("rid", "arrival", "n", "k", "steps", "net_seed", "seed", "mu", "sigma", "tau", "g_ui",
-                 "admitted", "finished", "events", "spikes", "v_hash", "v_mean", "v")
+                 "admitted", "finished", "events", "spikes", "v_hash", "v_mean", "v", "shed")
 
     def __init__(self, rid, arrival, n, k, steps, net_seed, seed, mu, sigma=0.1, tau=4.0, g_ui=DEFAULT_G_UI):
         self.rid, self.arrival, self.n, self.k, self.steps = rid, arrival, int(n), int(k), int(steps)
@@ -51,6 +51,7 @@ class Request:
         self.events = self.spikes = None
         self.v_hash = self.v_mean = None
         self.v = None
+        self.shed = False   # refused by admission control (waited past the server's limit)
 
     def device_bytes(self, gather=False, pull=False):
         """Device footprint; gather: also the gather buffers (2-byte offsets
@@ -145,12 +146,16 @@ class Server:
             r.v = V
         eng.remove(b.inst)
 
-    def serve(self, requests, realtime=True, max_resident=None, progress=None):
+    def serve(self, requests, realtime=True, max_resident=None, progress=None, shed_after=None):
         """Runs the stream to completion; arrival times are honoured on the
         wall clock when realtime (otherwise every request has arrived at
         start).  Returns summary statistics; per-request results and times
         are filled into the Request objects.  progress: seconds between
-        one-line reports on stderr (None: silent)."""
+        one-line reports on stderr (None: silent).  shed_after: admission
+        control under overload: a request still waiting this many seconds
+        after its arrival is refused (Request.shed) instead of queued on,
+        which bounds the latency of the served ones (None: every request is
+        served)."""
         for r in requests:
             if r.device_bytes(self.gather, self.pull) > self.budget:
                 raise ValueError("request %d needs %d bytes, more than the HBM budget"
@@ -170,6 +175,11 @@ class Server:
         def admit_arrived():
             nonlocal used, t_admit, wait_retire
             now = time.monotonic() - t0 + (0.0 if realtime else 1e30)
+            if shed_after is not None:
+                # the oldest waiting requests first (arrival order)
+                clock = time.monotonic() - t0 + offset
+                while pending and clock - pending[0].arrival > shed_after:
+                    pending.popleft().shed = True
             while (pending and not wait_retire and pending[0].arrival - offset <= now
                    and used + pending[0].device_bytes(self.gather, self.pull) <= self.budget
                    and (max_resident is None or len(active) + len(joining) < max_resident)):
@@ -205,6 +215,8 @@ class Server:
                 active.update(joining)
                 joining = {}
                 if not active:
+                    if not pending:
+                        continue   # the rest was shed
                     wait = pending[0].arrival - offset - (time.monotonic() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 0.05))
@@ -236,11 +248,12 @@ class Server:
                 done_count += 1
                 wait_retire = False
         wall = time.monotonic() - t0
-        lat = np.array([r.latency() for r in requests])
-        events = sum(r.events for r in requests)
-        spikes = sum(r.spikes for r in requests)
-        nsteps = sum(r.n * r.steps for r in requests)
-        return {"requests": len(requests), "wall_s": wall, "events": int(events),
+        served = [r for r in requests if not r.shed]
+        lat = np.array([r.latency() for r in served]) if served else np.zeros(1)
+        events = sum(r.events for r in served)
+        spikes = sum(r.spikes for r in served)
+        nsteps = max(sum(r.n * r.steps for r in served), 1)
+        return {"requests": len(requests), "shed": len(requests) - len(served), "wall_s": wall, "events": int(events),
                 "events_per_s": events / wall, "neuron_steps_per_s": nsteps / wall,
                 "mean_rate_hz": spikes / (nsteps * self.dt * 1e-3),
                 "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),

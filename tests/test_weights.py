@@ -39,7 +39,9 @@ def test_compact_weights_bit_exact(dtb, orc, gpu, tmp_path, dtype, mode):
         eng.check()
     finally:
         eng.close()
-    assert used == ("gather" if mode == "gather" else "dense")
+    # queues: the automatic count layout (dense, or spread when rows reach
+    # most tiles, as here: 150 synapses over 47 tiles)
+    assert used == "gather" if mode == "gather" else used in ("dense", "spread")
     if mode == "pull":
         assert stats["pulled"] > 0 and stats["deferred"] > 0
         assert info["bytes"] < native.nsyn * (4 + 8)   # in-edges keep the compact weights
