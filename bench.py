@@ -429,6 +429,20 @@ def main():
         log("cpu baseline %.1f s" % (time.perf_counter() - t0))
     else:
         res["cpu_baseline"] = None
+    # The reference's own CPU model (models/block_new.py, torch) beside this
+    # port on the 10k-neuron block, timed where the reference is available
+    # (it does not travel with this repository); recorded in a fixture.
+    ref_timing = os.path.join(ROOT, "tests", "golden", "block_new_timing.json")
+    if res["cpu_baseline"] is not None and os.path.exists(ref_timing):
+        with open(ref_timing) as f:
+            t = json.load(f)
+        res["cpu_baseline"]["reference_cpu_model_10k"] = {
+            "workload": t["workload"], "threads": t["threads"], "host": t["host"],
+            "block_new_dense_s_per_step": t["block_new_dense"]["s_per_step"],
+            "block_new_sparse_s_per_step": t["block_new_sparse"]["s_per_step"],
+            "port_s_per_step": t["port"]["s_per_step"],
+            "block_new_dense_events_per_s": t["block_new_dense"]["synaptic_events_per_s"],
+            "port_events_per_s": t["port"]["synaptic_events_per_s"]}
     eng.close()
     del blk, net
     if args.ensemble_steps > 0 and args.workload == "block":
