@@ -104,6 +104,7 @@ _SIGS = {
     "dtb_update_props": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int32]),
     "dtb_update_prop_cols": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int32, c_int32]),
     "dtb_run": (c_int, [c_void_p, c_int, c_int32, c_int32, c_uint32, POINTER(RunOutputs)]),
+    "dtb_run_noalloc": (c_int, [c_void_p, c_int, c_int32, c_int32, c_uint32, POINTER(RunOutputs)]),
     "dtb_run_stream": (c_int, [c_void_p, c_int, c_int32, c_int32, c_uint32, c_int32, RUN_CHUNK_FN, c_void_p]),
     "dtb_engine_run": (c_int, [c_void_p, c_int64, c_int32]),
     "dtb_configure": (c_int, [c_void_p, c_int, c_int32, c_int32]),

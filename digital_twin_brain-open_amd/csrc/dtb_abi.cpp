@@ -12,6 +12,8 @@
 #include "dtb_kernels.hpp"
 #include "dtb_npz.hpp"
 
+static_assert(DTB_NOALLOC_SCRATCH_BYTES == dtb::kScratchBytes, "the header states the static buffer's size");
+
 struct dtb_engine_s {
     std::unique_ptr<dtb::Engine> eng;
 };
@@ -305,6 +307,128 @@ int dtb_run(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t f
     return guarded([&] {
         dtb::Engine &e = engine(h);
         run_recorded(e, e.get(inst), iters, t_steps, flags, out);
+    });
+}
+
+int dtb_run_noalloc(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags,
+                    const dtb_run_outputs *out)
+{
+    return guarded([&] {
+        need(iters > 0, "iter must be positive");
+        dtb::Engine &e = engine(h);
+        dtb::Instance *I = e.get(inst);
+        need(t_steps == I->t_steps(), "dtb_run_noalloc keeps the instance's substeps: change t_steps with "
+                                      "dtb_configure (or one dtb_run) first");
+        const bool use_ou = (flags & DTB_RUN_USE_OU) != 0;
+        const bool want_samples =
+            (flags & (DTB_RUN_SAMPLE_SPIKE | DTB_RUN_SAMPLE_VMEMB | DTB_RUN_SAMPLE_ISYN | DTB_RUN_SAMPLE_IOU)) != 0;
+        if ((flags & DTB_RUN_SAMPLE_IOU) && !use_ou)
+            throw dtb::Error(dtb::kInvalidParameter, "i_ou samples need the OU background");
+        if (want_samples && I->samples().empty())
+            throw dtb::Error(dtb::kUnsetSampleFailure, "samples are not set");
+        const bool freq = (flags & DTB_RUN_FREQ) != 0 && out && out->freq;
+        const bool samples = want_samples && out &&
+                             (out->sample_spike || out->sample_vmemb || out->sample_isynaptic || out->sample_iou);
+        float *means[dtb::kStatFields] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        uint32_t stat_mask = 0;
+        if (out) {
+            if ((flags & DTB_RUN_VMEAN) && out->vmean) {
+                stat_mask |= dtb::kStatV;
+                means[0] = out->vmean;
+            }
+            if ((flags & DTB_RUN_IMEAN) && out->imean) {
+                stat_mask |= dtb::kStatI;
+                means[1] = out->imean;
+            }
+            if ((flags & DTB_RUN_RECEPTOR_IMEAN) && out->ampa_imean && out->nmda_imean && out->gabaa_imean &&
+                out->gabab_imean) {
+                stat_mask |= dtb::kStatReceptor;
+                means[2] = out->ampa_imean;
+                means[3] = out->nmda_imean;
+                means[4] = out->gabaa_imean;
+                means[5] = out->gabab_imean;
+            }
+        }
+        // The call's records in the static array, each at a 256-byte
+        // boundary: counts, statistic sums, per-population divisors and
+        // means, sample rows.
+        const size_t rows = (size_t)iters, np = I->npop(), ns = samples ? I->samples().size() : 0;
+        size_t at = 0;
+        auto carve = [&](size_t bytes) {
+            const size_t o = at;
+            at += (bytes + 255) / 256 * 256;
+            return o;
+        };
+        const size_t o_cnt = freq ? carve(rows * np * sizeof(uint32_t)) : 0;
+        const size_t o_sum = stat_mask ? carve(rows * np * dtb::kStatFields * sizeof(long long)) : 0;
+        const size_t o_div = stat_mask ? carve(np * sizeof(unsigned long long)) : 0;
+        size_t o_mean[dtb::kStatFields] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < dtb::kStatFields; ++k)
+            if (means[k])
+                o_mean[k] = carve(rows * np * sizeof(float));
+        const size_t o_spk = samples ? carve(rows * ns) : 0;
+        const size_t o_v = samples ? carve(rows * ns * sizeof(float)) : 0;
+        const size_t o_is = samples ? carve(rows * ns * sizeof(float)) : 0;
+        const size_t o_io = samples ? carve(rows * ns * sizeof(float)) : 0;
+        if (at > dtb::kScratchBytes)
+            throw dtb::Error(dtb::kInvalidParameter,
+                             "dtb_run_noalloc records " + std::to_string(at) + " bytes, more than its " +
+                                 std::to_string(dtb::kScratchBytes) + "-byte static buffer: run fewer steps per call");
+
+        dtb::devmem::NoAlloc no_alloc;   // a device allocation anywhere below throws
+        DTB_HIP(hipSetDevice(e.device()));
+        char *base = static_cast<char *>(dtb::scratch_address(e.device()));
+        e.settle();
+        I->configure_run(t_steps, use_ou, e.stream());   // same substeps: no parameter re-derivation
+        dtb::Instance::RecordBuffers b{};
+        b.counts = freq ? reinterpret_cast<uint32_t *>(base + o_cnt) : nullptr;
+        b.stat = stat_mask ? reinterpret_cast<long long *>(base + o_sum) : nullptr;
+        if (samples) {
+            b.spike = reinterpret_cast<uint8_t *>(base + o_spk);
+            b.v = reinterpret_cast<float *>(base + o_v);
+            b.isyn = reinterpret_cast<float *>(base + o_is);
+            b.iou = reinterpret_cast<float *>(base + o_io);
+        }
+        // The tables point at the static array only for this call.
+        struct Scope {
+            dtb::Engine &e;
+            dtb::Instance *I;
+            ~Scope()
+            {
+                I->end_record();
+                e.mark_dirty();
+            }
+        } scope{e, I};
+        I->begin_record_at(iters, freq, samples, stat_mask, b, e.stream());
+        e.mark_dirty();
+        e.run(iters, 0);   // eager launches: a graph would be instantiated (allocated) here
+        if (stat_mask & (dtb::kStatI | dtb::kStatReceptor))
+            e.stat_tail();
+        hipStream_t s = e.stream();
+        if (stat_mask) {
+            float *dm[dtb::kStatFields];
+            for (int k = 0; k < dtb::kStatFields; ++k)
+                dm[k] = means[k] ? reinterpret_cast<float *>(base + o_mean[k]) : nullptr;
+            DTB_HIP(dtb::launch_stat_means(I->blocked_dev(), I->pops_dev(), (uint32_t)np, b.stat, (uint32_t)rows,
+                                           reinterpret_cast<unsigned long long *>(base + o_div), dm, s));
+            for (int k = 0; k < dtb::kStatFields; ++k)
+                if (means[k])
+                    DTB_HIP(hipMemcpyAsync(means[k], dm[k], rows * np * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        if (freq)
+            DTB_HIP(hipMemcpyAsync(out->freq, b.counts, rows * np * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (samples) {
+            if ((flags & DTB_RUN_SAMPLE_SPIKE) && out->sample_spike)
+                DTB_HIP(hipMemcpyAsync(out->sample_spike, b.spike, rows * ns, hipMemcpyDeviceToHost, s));
+            if ((flags & DTB_RUN_SAMPLE_VMEMB) && out->sample_vmemb)
+                DTB_HIP(hipMemcpyAsync(out->sample_vmemb, b.v, rows * ns * sizeof(float), hipMemcpyDeviceToHost, s));
+            if ((flags & DTB_RUN_SAMPLE_ISYN) && out->sample_isynaptic)
+                DTB_HIP(hipMemcpyAsync(out->sample_isynaptic, b.isyn, rows * ns * sizeof(float),
+                                       hipMemcpyDeviceToHost, s));
+            if ((flags & DTB_RUN_SAMPLE_IOU) && out->sample_iou)
+                DTB_HIP(hipMemcpyAsync(out->sample_iou, b.iou, rows * ns * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        e.sync();
     });
 }
 

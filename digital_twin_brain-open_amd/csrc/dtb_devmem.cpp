@@ -62,6 +62,9 @@ State &state()
     return *s;
 }
 
+// NoAlloc scopes alive on this thread.
+thread_local int t_no_alloc = 0;
+
 size_t round_up(size_t bytes, size_t grain)
 {
     return (bytes + grain - 1) / grain * grain;
@@ -134,8 +137,22 @@ void new_segment(DeviceCache &c, size_t bytes, bool reserved)
 
 }  // namespace
 
+NoAlloc::NoAlloc()
+{
+    ++t_no_alloc;
+}
+
+NoAlloc::~NoAlloc()
+{
+    --t_no_alloc;
+}
+
 void *alloc(size_t bytes)
 {
+    if (t_no_alloc > 0)
+        throw Error(kInvalidParameter, "this call allocates no device memory, but the engine needs " +
+                                           std::to_string(bytes) + " bytes first (new instances, samples or "
+                                           "schedule: build the tables with one dtb_run or dtb_engine_run)");
     int device = 0;
     DTB_HIP(hipGetDevice(&device));
     const size_t want = round_up(std::max<size_t>(bytes, 1), kGrain);

@@ -736,8 +736,30 @@ void Instance::begin_record(int64_t steps, bool freq, bool samples, uint32_t sta
     }
 }
 
+void Instance::begin_record_at(int64_t steps, bool freq, bool samples, uint32_t stat_mask, const RecordBuffers &b,
+                               hipStream_t s)
+{
+    if (samples && samples_.empty())
+        throw Error(kUnsetSampleFailure, "samples requested but not set");
+    if ((freq && !b.counts) || (stat_mask != 0 && !b.stat) || (samples && !(b.spike && b.v && b.isyn && b.iou)))
+        throw Error(kInvalidParameter, "record buffer missing");
+    rec_step0_ = step(s);
+    rec_len_ = steps;
+    rec_freq_ = freq;
+    rec_samples_ = samples;
+    rec_stat_mask_ = stat_mask;
+    rec_ext_ = b;
+    rec_ext_on_ = true;
+    if (stat_mask != 0)
+        DTB_HIP(launch_zero_words(b.stat, (size_t)steps * npop() * kStatFields * sizeof(long long), s));
+    if (freq)
+        DTB_HIP(launch_zero_words(b.counts, (size_t)steps * npop() * sizeof(uint32_t), s));
+}
+
 void Instance::end_record()
 {
+    rec_ext_on_ = false;
+    rec_ext_ = RecordBuffers{};
     rec_freq_ = rec_samples_ = false;
     rec_stat_mask_ = 0;
     rec_len_ = 0;
@@ -746,6 +768,8 @@ void Instance::end_record()
 void Instance::fetch_record(uint32_t *freq, uint8_t *spike, float *v, float *isyn, float *iou, float *const *means,
                             hipStream_t s)
 {
+    if (rec_ext_on_)
+        throw Error(kInternal, "the recording is in caller buffers");
     if (means != nullptr && rec_stat_mask_ != 0) {
         // stat_result.cpp:266-391 divides by the population size minus its
         // blocked neurons: mean = (double) sum * 2^-24 / count, as float.
@@ -971,8 +995,10 @@ InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer
     d.w = w_.gpu_data();
     d.wfmt = wfmt_;
     d.pops = pops_.gpu_data();
-    d.rec_counts = (rec_freq_ && rec_len_ > 0) ? rec_counts_.mutable_gpu_data() : nullptr;
-    d.rec_stat = (rec_stat_mask_ != 0 && rec_len_ > 0) ? rec_stat_.mutable_gpu_data() : nullptr;
+    d.rec_counts = (rec_freq_ && rec_len_ > 0) ? (rec_ext_on_ ? rec_ext_.counts : rec_counts_.mutable_gpu_data())
+                                                : nullptr;
+    d.rec_stat = (rec_stat_mask_ != 0 && rec_len_ > 0) ? (rec_ext_on_ ? rec_ext_.stat : rec_stat_.mutable_gpu_data())
+                                                       : nullptr;
     d.stat_mask = rec_stat_mask_;
     d.wave_ctr = counters_.mutable_gpu_data();
     d.step = step_.mutable_gpu_data();
@@ -991,7 +1017,13 @@ InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer
 SampleRecDev Instance::sample_rec()
 {
     SampleRecDev r{};
-    if (rec_samples_ && rec_len_ > 0) {
+    if (rec_samples_ && rec_len_ > 0 && rec_ext_on_) {
+        r.spike = rec_ext_.spike;
+        r.v = rec_ext_.v;
+        r.isyn = rec_ext_.isyn;
+        r.iou = rec_ext_.iou;
+        r.nsample = (uint32_t)samples_.size();
+    } else if (rec_samples_ && rec_len_ > 0) {
         r.spike = rec_spike_.mutable_gpu_data();
         r.v = rec_v_.mutable_gpu_data();
         r.isyn = rec_isyn_.mutable_gpu_data();

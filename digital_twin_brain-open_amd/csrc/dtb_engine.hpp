@@ -102,7 +102,27 @@ void release(void *p);               // block from alloc; may still be in use by
 void reserve(size_t bytes);          // keeps at least this much for blocks of 1 MB and more
 void trim();                         // returns all unused memory to the runtime
 size_t cached_bytes(int device);     // bytes held in the cache
+// While one is alive on a thread, alloc on that thread throws instead of
+// taking memory (the entry points that promise not to allocate).
+struct NoAlloc {
+    NoAlloc();
+    ~NoAlloc();
+    NoAlloc(const NoAlloc &) = delete;
+    NoAlloc &operator=(const NoAlloc &) = delete;
+};
 }  // namespace devmem
+
+// The static device array of dtb_run_noalloc (kScratchBytes) on the current
+// device: its address, looked up once per device (dtb_noalloc.hip).
+void *scratch_address(int device);
+// Zeroes `bytes` (a multiple of 4) at p with a fill kernel.
+hipError_t launch_zero_words(void *p, size_t bytes, hipStream_t stream);
+// Per-population means of a recording: cnt[q] = non-blocked neurons of
+// population q (counted on the device from the instance's blocked flags and
+// population table), then out[k][r][q] = (float)((double)sums[r][q][k] *
+// 2^-24 / cnt[q]) for every field k with out[k] != null.
+hipError_t launch_stat_means(const uint8_t *blocked, const PopDev *pops, uint32_t npop, const long long *sums,
+                             uint32_t rows, unsigned long long *cnt, float *const *out, hipStream_t stream);
 
 // Device buffer with a lazily created, zero-filled host mirror.
 template <typename T>
@@ -262,7 +282,22 @@ public:
     // Recording window of `steps` rows starting at the current step;
     // stat_mask selects per-population means (kStatV | kStatI | kStatReceptor).
     void begin_record(int64_t steps, bool freq, bool samples, uint32_t stat_mask, hipStream_t s);
+    // The same window recorded into caller-provided device buffers (no
+    // allocation): counts [steps][npop] and sums [steps][npop][kStatFields]
+    // are zeroed here; the sample rows [steps][nsample] need no clearing.
+    struct RecordBuffers {
+        uint32_t *counts;
+        long long *stat;
+        uint8_t *spike;
+        float *v, *isyn, *iou;
+    };
+    void begin_record_at(int64_t steps, bool freq, bool samples, uint32_t stat_mask, const RecordBuffers &b,
+                         hipStream_t s);
     void end_record();
+    // Device descriptors the statistics need (the instance's blocked flags
+    // and population table).
+    const uint8_t *blocked_dev() { return blocked_.gpu_data(); }
+    const PopDev *pops_dev() { return pops_.gpu_data(); }
     uint32_t stat_mask() const { return rec_stat_mask_; }
     // means: [6] arrays [steps][npop] (vmean, imean, AMPA, NMDA, GABA_A,
     // GABA_B), any may be null.
@@ -385,6 +420,8 @@ private:
     int64_t rec_step0_ = 0;
     int64_t rec_len_ = 0;
     bool rec_freq_ = false, rec_samples_ = false;
+    bool rec_ext_on_ = false;            // recording into rec_ext_ (begin_record_at)
+    RecordBuffers rec_ext_{};
     std::vector<uint32_t> tile_syn_;      // synapses per target tile
     double row_coverage_ = 0.0;           // mean fraction of the tiles one row reaches
     double group_coverage_ = 0.0;         // mean fraction of the gather groups one row reaches

@@ -330,12 +330,24 @@ void Engine::rebuild()
     ninst_ = (uint32_t)table.size();
     ntiles_ = (uint32_t)tiles.size();
     ngroups_ = (uint32_t)groups.size();
-    d_groups_.resize(std::max<size_t>(groups.size(), 1));
     nsamples_ = (uint32_t)samples.size();
-    d_insts_.resize(std::max<size_t>(table.size(), 1));
-    d_tiles_.resize(std::max<size_t>(tiles.size(), 1));
-    d_samples_.resize(std::max<size_t>(samples.size(), 1));
-    d_recs_.resize(std::max<size_t>(recs.size(), 1));
+    // The tables only grow (the kernels get their lengths as arguments), so
+    // a rebuild after the largest set so far allocates nothing.
+    auto grow = [](auto &buf, size_t n) {
+        if (buf.count() < std::max<size_t>(n, 1))
+            buf.resize(std::max<size_t>(n, 1));
+    };
+    grow(d_groups_, groups.size());
+    grow(d_insts_, table.size());
+    grow(d_tiles_, tiles.size());
+    // sized by the samples set, recorded or not: a recording run then finds
+    // the table built
+    size_t sample_set = 0;
+    for (auto &p : insts_)
+        if (p)
+            sample_set += p->samples().size();
+    grow(d_samples_, std::max(samples.size(), sample_set));
+    grow(d_recs_, recs.size());
     // Auto: dense, unless rows reach most of their instance's tiles (the
     // reservations of many waves then meet on the same lines).  Measured with
     // the tile's passes dealt to all its waves, dense was as fast or faster
@@ -347,7 +359,7 @@ void Engine::rebuild()
         cshift_ = kCountShiftSpread;
     else
         cshift_ = (total > 0 && coverage / (double)total > kSpreadCoverage) ? kCountShiftSpread : kCountShiftDense;
-    d_qcnt_.resize(count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
+    grow(d_qcnt_, count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
     d_qcnt_.mutable_gpu_data();

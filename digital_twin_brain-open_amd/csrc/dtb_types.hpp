@@ -269,6 +269,10 @@ struct alignas(16) SampleRecDev {
     uint32_t pad[3];
 };
 
+// Size of the static device array that holds the per-step records of one
+// dtb_run_noalloc call (one copy per GPU, kept until the process ends).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+
 // Shared per-engine diagnostics in device memory.
 struct alignas(16) StepCtl {
     unsigned long long spilled;   // synapses past a queue's capacity (spilled, or taken by the pull)

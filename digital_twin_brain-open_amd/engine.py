@@ -528,7 +528,9 @@ class Engine:
 
     def run_block(self, inst, iters, t_steps=1, use_ou=True, freqs=True, sample_spike=False, sample_vmemb=False,
                   sample_isynaptic=False, sample_iou=False, graph=False, nsample=0, npop=0, vmean=False,
-                  imean=False, receptor_imean=False):
+                  imean=False, receptor_imean=False, noalloc=False):
+        """dtb_run; noalloc: dtb_run_noalloc (records in the library's static
+        device array, no device allocation; eager steps)."""
         flags = (_lib.RUN_USE_OU if use_ou else 0) | (_lib.RUN_GRAPH if graph else 0)
         out = {}
         o = _lib.RunOutputs()
@@ -558,7 +560,8 @@ class Engine:
                 flags |= flag
                 out[key] = np.zeros((iters, nsample), dtype=dt)
                 setattr(o, field, _ptr(out[key]))
-        _lib.check(self._lib.dtb_run(self._h, inst, iters, t_steps, flags, byref(o)))
+        fn = self._lib.dtb_run_noalloc if noalloc else self._lib.dtb_run
+        _lib.check(fn(self._h, inst, iters, t_steps, flags, byref(o)))
         return out
 
     def run_block_stream(self, inst, iters, chunk=100, t_steps=1, use_ou=True, freqs=True, sample_spike=False,

@@ -197,6 +197,26 @@ typedef int (*dtb_run_chunk_fn)(void *user, int32_t first_iter, int32_t rows, co
 int dtb_run_stream(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags, int32_t chunk,
                    dtb_run_chunk_fn fn, void *user);
 
+/* dtb_run without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The per-step
+ * records live in a static device array of DTB_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends) instead of buffers allocated per call; the
+ * outputs are copied into the caller's host arrays as dtb_run does, with
+ * bit-identical values.  The steps are launched one by one (DTB_RUN_GRAPH is
+ * ignored: instantiating a graph allocates), with the instance's configured
+ * t_steps (another value is rejected).  Fails with DTB_INVALID_PARAMETER,
+ * allocating nothing, when the records (counts 4 B x npop, statistic sums
+ * 48 B x npop plus 4 B per requested mean x npop, samples 13 B x nsample,
+ * per step, each array rounded up to 256 B) exceed the array, or when the
+ * engine's tables would need memory first (after instances, samples or the
+ * schedule change: one dtb_run or dtb_engine_run builds them).  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several threads or engines) race. */
+#define DTB_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dtb_run_noalloc(dtb_engine_t h, int inst, int32_t iters, int32_t t_steps, uint32_t flags,
+                    const dtb_run_outputs *out);
+
 /* Batched stepping of every instance of the engine with its last run
  * configuration (t_steps, OU), no per-step outputs; graph_chunk > 1 replays
  * chunks of that many steps from a captured HIP graph. */
