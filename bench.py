@@ -281,6 +281,10 @@ def main():
                     help="weight pairs kept on the device (uint8: codes round(255 w), as the reference's uint8 "
                          "blocks; 2 bytes per synapse instead of 8)")
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
+    ap.add_argument("--pull", type=int, choices=(0, 1), default=1,
+                    help="pull propagation for synchronous volleys (in-edges grouped by target tile)")
+    ap.add_argument("--split", type=int, choices=(0, 1), default=0,
+                    help="split binning: the fired rows' passes binned by a second launch per step")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
@@ -321,6 +325,8 @@ def main():
     log("generated %d neurons / %d synapses in %.1f s" % (net.n, net.nsyn, time.perf_counter() - t0))
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
+    eng.split(bool(args.split))
+    eng.pull(bool(args.pull))
     blk = dtb.Block(net, delta_t=DT_MS, seed=sim_seed, engine=eng)
     blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, mu, sigma)
     eng.configure(blk.inst, 1, True)
@@ -378,7 +384,7 @@ def main():
     else:
         layout_bytes = (LAYOUT_NEURON_BYTES * net.n + LAYOUT_ROW_BYTES * sp_step
                         + (LAYOUT_SYN_BYTES - 8 + wpair) * ev_step)
-    kname = "k_gather+k_step" if gather else "k_step"
+    kname = "k_gather+k_step" if gather else ("k_step+k_bin" if args.split else "k_step")
     achieved = algo_bytes / (us * 1e-6) / 1e9
     traffic = (profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup)
                if args.weights == "float32" else None)
@@ -407,6 +413,7 @@ def main():
         "kernel_us_per_step": {kname: us},
         "kernel_us_isolated": {kname: us_iso},
         "step_ms_max": float(ms_steps.max()),
+        "step_ms": [round(float(x), 4) for x in ms_steps] if args.steps <= 50 else None,
         "steps_over_1ms": int((ms_steps > 1.0).sum()),
         "step_time_resolution": "one step (eager launches)" if eager else
                                 "one graph of %d steps (its mean)" % GRAPH_CHUNK,

@@ -115,6 +115,8 @@ _SIGS = {
     "dtb_engine_queue_capacity": (c_int, [c_void_p, c_uint32, c_uint32]),
     "dtb_engine_schedule": (c_int, [c_void_p, c_int32, c_int32]),
     "dtb_engine_pull": (c_int, [c_void_p, c_int32, c_uint32]),
+    "dtb_engine_split": (c_int, [c_void_p, c_int32, c_uint32]),
+    "dtb_engine_get_split": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_uint32)]),
     "dtb_engine_get_pull": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_uint32)]),
     "dtb_instance_pull_info": (c_int, [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                        POINTER(c_uint64)]),

@@ -541,6 +541,21 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift)
     return guarded([&] { engine(h).set_queue_capacity(floor, shift); });
 }
 
+int dtb_engine_split(dtb_engine_t h, int32_t on, uint32_t capacity)
+{
+    return guarded([&] { engine(h).set_split(on != 0, capacity); });
+}
+
+int dtb_engine_get_split(dtb_engine_t h, int32_t *on, uint32_t *capacity)
+{
+    return guarded([&] {
+        need(on != nullptr, "on is NULL");
+        *on = engine(h).split() ? 1 : 0;
+        if (capacity)
+            *capacity = engine(h).split_cap();
+    });
+}
+
 int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min)
 {
     return guarded([&] { engine(h).set_pull(on != 0, defer_min); });

@@ -308,10 +308,14 @@ void Instance::ensure_gather(hipStream_t s)
         segidx_.resize((size_t)n_ * (ngroups() + 1u));
         DTB_HIP(launch_build_segidx(rowptr_.gpu_data(), col_.gpu_data(), n_, ngroups(), segidx_.mutable_gpu_data(),
                                     s));
+        grec_.resize((size_t)nsyn_ * grec_words(wfmt_));
+        DTB_HIP(launch_build_grec(col_.gpu_data(), w_.gpu_data(), wfmt_, (uint64_t)nsyn_, grec_.mutable_gpu_data(),
+                                  s));
         DTB_HIP(hipStreamSynchronize(s));
     } catch (...) {
         // out of device memory (or a failed build): nothing half-built stays
         segidx_.resize(0);
+        grec_.resize(0);
         fired_.resize(0);
         fcount_.resize(0);
         throw;
@@ -322,6 +326,7 @@ void Instance::ensure_gather(hipStream_t s)
 void Instance::release_gather()
 {
     segidx_.resize(0);
+    grec_.resize(0);
     fired_.resize(0);
     fcount_.resize(0);
     gather_built_ = false;
@@ -968,6 +973,7 @@ InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer
         if (!gather_built_)
             throw Error(kInvalidParameter, "gather buffers not built");
         d.segidx = segidx_.gpu_data();
+        d.grec = grec_.gpu_data();
         d.fired = fired_.mutable_gpu_data();
         d.fcount = fcount_.mutable_gpu_data();
         d.ngroups = ngroups();

@@ -428,6 +428,7 @@ private:
     bool gather_ok_ = false;              // gather_allowed()
     bool gather_built_ = false;           // segidx_, fired_, fcount_ built
     DataAllocator<uint16_t> segidx_;      // [n][ngroups + 1] group offsets (built on demand)
+    DataAllocator<uint32_t> grec_;        // [nsyn][grec_words] gather records (built with segidx_)
     DataAllocator<uint32_t> fired_;       // [2][kFiredLists][fired_capacity()] fired rows
     DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]
     bool pull_built_ = false;             // ckey_, cw_, cptr_, bitmaps built
@@ -501,6 +502,21 @@ public:
     // queues still pull).  Off: queues only, past-capacity synapses through
     // the spill accumulators.  Every setting gives the same results.
     void set_pull(bool on, uint32_t defer_min);
+    // Split binning: the fired rows' binning passes run in a second launch
+    // per step (k_bin) spread over every CU, instead of in the tiles whose
+    // neurons fired.  Results never depend on it.
+    // cap: entries of the pass list (0: sized by the instances); passes past
+    // it are binned by their own tile.
+    void set_split(bool on, uint32_t cap)
+    {
+        if (on != split_ || cap != split_cap_) {
+            split_ = on;
+            split_cap_ = cap;
+            dirty_ = true;
+        }
+    }
+    bool split() const { return split_; }
+    uint32_t split_cap() const { return split_cap_; }
     bool pull_on() const { return pull_on_; }
     uint32_t defer_min() const { return defer_min_; }
     bool uses_pull(const Instance &p) const { return pull_on_ && p.pull_built() && !use_gather(p); }
@@ -573,6 +589,12 @@ private:
     uint32_t queue_floor_ = 4096, queue_shift_ = 3;   // queue_capacity
     bool tuning_ = false;                  // autotune running: gather buffers are kept
     bool pull_on_ = true;                  // pull propagation where in-edges fit
+    bool split_ = false;                   // split binning (k_bin)
+    uint32_t split_cap_ = 0;               // requested pass-list entries (0: automatic)
+    DataAllocator<PassEntry> d_pass_;      // pass list of split binning
+    DataAllocator<uint32_t> d_pcount_;     // its two counts, kPassCountStride words apart
+    uint32_t pass_cap_ = 0;                // entries of d_pass_ in use (0: split binning off)
+    uint32_t bin_grid_ = 0;                // k_bin workgroups
     uint32_t defer_min_ = kDeferMinDefault;
     void try_pull(Instance &p, hipStream_t s);   // builds p's in-edges if on and they fit
     DataAllocator<InstDev> d_insts_;

@@ -257,6 +257,11 @@ QueueSet Engine::queues()
     q.count = d_qcnt_.mutable_gpu_data();
     q.ntiles = ntiles_;
     q.cshift = cshift_;
+    if (pass_cap_ > 0) {
+        q.pass = d_pass_.mutable_gpu_data();
+        q.pcount = d_pcount_.mutable_gpu_data();
+        q.pcap = pass_cap_;
+    }
     return q;
 }
 
@@ -273,6 +278,7 @@ void Engine::rebuild()
     tile_base_.assign(insts_.size(), 0u);
     uint64_t total = 0;
     double coverage = 0.0;
+    uint64_t pass_want = 0;   // split binning: pass-list entries for the queue instances
     for (size_t slot = 0; slot < insts_.size(); ++slot) {
         Instance *p = insts_[slot].get();
         if (!p)
@@ -323,6 +329,9 @@ void Engine::rebuild()
         if (!gat) {
             total += p->n();
             coverage += p->row_coverage() * p->n();
+            // a step listing one pass per 1/64 of the neurons and per 4096
+            // synapses fits; beyond, tiles bin their own passes
+            pass_want += p->n() / 64 + (uint64_t)p->nsyn() / 4096;
         }
     }
     if (total > 0xffffffffull || tiles.size() > 0xffffffffull)
@@ -360,6 +369,16 @@ void Engine::rebuild()
     else
         cshift_ = (total > 0 && coverage / (double)total > kSpreadCoverage) ? kCountShiftSpread : kCountShiftDense;
     grow(d_qcnt_, count_words(2 * std::max<size_t>(tiles.size(), 1), cshift_));
+    if (split_ && total > 0) {
+        pass_cap_ = split_cap_ != 0 ? split_cap_
+                                    : (uint32_t)std::min<uint64_t>(pass_want + 4096, (uint64_t)4 << 20);
+        grow(d_pass_, pass_cap_);
+        grow(d_pcount_, 2 * kPassCountStride);
+        bin_grid_ = std::min<uint32_t>(kBinGridMax, std::max<uint32_t>(64, (uint32_t)(total / kBlock)));
+    } else {
+        pass_cap_ = 0;
+        bin_grid_ = 0;
+    }
     // Allocate (zero-filled) everything the step kernels touch before any
     // capture: no allocation may happen while a graph is being recorded.
     d_qcnt_.mutable_gpu_data();
@@ -367,6 +386,8 @@ void Engine::rebuild()
     d_recs_.mutable_gpu_data();
     d_groups_.mutable_gpu_data();
     DTB_HIP(hipMemsetAsync(d_qcnt_.mutable_gpu_data(), 0, d_qcnt_.size(), stream_));
+    if (pass_cap_ > 0)
+        DTB_HIP(hipMemsetAsync(d_pcount_.mutable_gpu_data(), 0, d_pcount_.size(), stream_));
     d_insts_.upload(table.data(), table.size(), stream_);
     d_tiles_.upload(tiles.data(), tiles.size(), stream_);
     d_groups_.upload(groups.data(), groups.size(), stream_);
@@ -386,6 +407,8 @@ void Engine::launch_step_k()
     launch_gather_k(par_);
     DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
                                d_ctl_.mutable_gpu_data(), pass_k_, stream_));
+    DTB_HIP(launch_bin_kernel(d_tiles_.gpu_data(), bin_grid_, d_insts_.gpu_data(), queues(), par_,
+                              d_ctl_.mutable_gpu_data(), pass_k_, nullptr, nullptr, stream_));
 }
 
 void Engine::launch_aux()
@@ -519,30 +542,37 @@ void Engine::run_timed(int64_t steps, float *ms_step)
     // With gather instances the step is two kernels (k_gather, k_step),
     // each timed.
     const int64_t chunk = 256;
-    std::vector<hipEvent_t> ev((size_t)(4 * chunk));
+    std::vector<hipEvent_t> ev((size_t)(6 * chunk));
     for (auto &e : ev)
         DTB_HIP(hipEventCreate(&e));
     double ts = 0.0;
     for (int64_t base = 0; base < steps; base += chunk) {
         const int64_t m = std::min<int64_t>(chunk, steps - base);
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(4 * k)];
+            hipEvent_t *e = &ev[(size_t)(6 * k)];
             if (ngroups_ > 0)
                 DTB_HIP(launch_gather_timed(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par_, e[2], e[3],
                                             stream_));
             DTB_HIP(launch_step_kernel_timed(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
                                              d_ctl_.mutable_gpu_data(), pass_k_, e[0], e[1], stream_));
+            if (pass_cap_ > 0)
+                DTB_HIP(launch_bin_kernel(d_tiles_.gpu_data(), bin_grid_, d_insts_.gpu_data(), queues(), par_,
+                                          d_ctl_.mutable_gpu_data(), pass_k_, e[4], e[5], stream_));
             par_ ^= 1u;
             launch_aux();
         }
-        DTB_HIP(hipEventSynchronize(ev[(size_t)(4 * (m - 1) + 1)]));
+        DTB_HIP(hipEventSynchronize(ev[(size_t)(6 * (m - 1) + (pass_cap_ > 0 ? 5 : 1))]));
         for (int64_t k = 0; k < m; ++k) {
-            hipEvent_t *e = &ev[(size_t)(4 * k)];
+            hipEvent_t *e = &ev[(size_t)(6 * k)];
             float a = 0.0f;
             DTB_HIP(hipEventElapsedTime(&a, e[0], e[1]));
             ts += a;
             if (ngroups_ > 0) {
                 DTB_HIP(hipEventElapsedTime(&a, e[2], e[3]));
+                ts += a;
+            }
+            if (pass_cap_ > 0) {
+                DTB_HIP(hipEventElapsedTime(&a, e[4], e[5]));
                 ts += a;
             }
         }

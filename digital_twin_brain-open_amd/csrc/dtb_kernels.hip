@@ -216,31 +216,42 @@ __device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDe
 // filled set `set`: every emitted row (`all`: the queue overflowed and is
 // ignored) or only the rows of tiles that deferred them (dbits).  The
 // in-edges are read coalesced, the source's bit from a per-wave bitmap
-// (n / 8 bytes, cache resident), and the weight pair only for a set bit.
-// Called by the whole workgroup.
+// (n / 8 bytes, cache resident; read as 32-bit words), and the weight pair
+// only for a set bit.  A full pull is a chain of three dependent loads per
+// batch (keys, bits, weights), so each thread keeps kPullItems in-edges in
+// flight.  Called by the whole workgroup.
+constexpr uint32_t kPullItems = 6;
 template <uint32_t kFmt>
 __device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_t *key, const void *cw,
                                           const unsigned long long *bits, uint32_t len, uint32_t tid)
 {
-    constexpr uint32_t U = 4;   // in-edges per thread in flight
+    constexpr uint32_t U = kPullItems;
+    const auto b32 = gp(reinterpret_cast<const uint32_t *>(bits));
+    const auto kp = gp(key);
     for (uint32_t k0 = tid; k0 < len; k0 += kBlock * U) {
         uint32_t kv[U];
-        bool hit[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t j = k0 + u * kBlock;
-            hit[u] = j < len;
-            kv[u] = gp(key)[hit[u] ? j : len - 1u];
+            kv[u] = kp[j < len ? j : len - 1u];
         }
+        uint32_t bw[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            bw[u] = b32[(kv[u] & kPullRowMask) >> 5];
+        uint32_t hit = 0;   // bit u: in-edge k0 + u * kBlock exists and its source was emitted
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            if (k0 + u * kBlock < len && ((bw[u] >> (kv[u] & 31u)) & 1u) != 0u)
+                hit |= 1u << u;
+        unsigned long long wr[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+            wr[u] = ((hit >> u) & 1u) ? w_raw(cw, k0 + u * kBlock, kFmt) : 0ull;
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t r = kv[u] & kPullRowMask;
-            hit[u] = hit[u] && ((gp(bits)[r >> 6] >> (r & 63u)) & 1ull) != 0ull;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            if (hit[u]) {
-                const float2 wp = w_decode(w_raw(cw, k0 + u * kBlock, kFmt), kFmt);
+            if ((hit >> u) & 1u) {
+                const float2 wp = w_decode(wr[u], kFmt);
                 const uint32_t slot = ((kv[u] >> kPullRowBits) & 0xffu) * 4u + ((kv[u] & kKindBit) ? 2u : 0u);
                 atomicAdd(&acc[slot], f32_to_q40(wp.x));
                 atomicAdd(&acc[slot + 1u], f32_to_q40(wp.y));
@@ -597,7 +608,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
                                           const InstDev *__restrict__ insts, const QueueSet &qs, uint32_t par,
                                           StepCtl *__restrict__ ctl, unsigned long long *acc,
                                           unsigned long long *seg_dst, uint32_t *seg_room, long long *row_rb,
-                                          uint32_t *row_len, uint32_t &nrows)
+                                          uint32_t *row_len, uint32_t &nrows, uint32_t &s_fit)
 {
     const InstDev &I = insts[d.inst];
     const uint32_t tid = threadIdx.x;
@@ -608,15 +619,43 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     const size_t cslot_in = count_slot((size_t)par * qs.ntiles + blockIdx.x, qs.cshift);
     QSyn *__restrict__ q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
 
-    // A tile that pulls (the pull propagation, never with gather: its queue
-    // overflowed, or rows were deferred into this set; rare) accumulates its
-    // input first, before its state is loaded, so the pull's loads do not
-    // compete with the state's registers.
+    // Loads are issued in order of their dependence chains, before anything
+    // waits on them: the queue count and step index, the parameter class,
+    // the neuron state, the parameter records (need the class), the queue
+    // head, the deferral stamp.
     const uint32_t cnt_raw = qs.count[cslot_in];
     const int64_t step = gp(I.step)[par];
     const bool pull_on = I.ckey != nullptr;
+    const uint32_t li = d.begin + tid;
+    const bool active = tid < d.count;
+    const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
+    // A tile whose neurons share one parameter class (the usual case: a
+    // population spans many tiles) reads its records without the class load.
+    const uint32_t c = kUniform ? d.pcls - 1u : (lc & I.pmask) + (uint32_t)gp(I.cls)[lc];
+    float4 J = ld4(I.J, lc);
+    float V = gp(I.V)[lc];
+    int32_t t = gp(I.tcnt)[lc];
+    uint32_t fl = gp(I.flag)[lc];
+    float iou = gp(I.iou)[lc];
+    float4 pa = ld4(I.pA, c);
+    float4 pb = ld4(I.pB, c);
+    float4 g = ld4(I.gU, c);
+    float4 vu = ld4(I.vU, c);
+    float4 dc = ld4(I.dec, c);
+    // The queue is complete when the launch starts; entries past the count
+    // are ignored (the queue buffer has kQueuePrefetch slots of slack at its
+    // end, so the read needs no branch).
+    QSyn head{};
+    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (gat)
+        dl = ld4(I.dlt, lc);
+    else
+        head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
+    const int64_t stamp = pull_on ? gp(I.dstamp)[par] : 0;
+    // A tile pulls (the pull propagation, never with gather) when its queue
+    // overflowed or rows were deferred into this set: rare (volleys).
     const bool over = !gat && cnt_raw > d.qcap;
-    const bool pull = pull_on && (over || gp(I.dstamp)[par] == step + 1);
+    const bool pull = !gat && pull_on && (over || stamp == step + 1);
     acc[tid * 4 + 0] = 0ull;
     acc[tid * 4 + 1] = 0ull;
     acc[tid * 4 + 2] = 0ull;
@@ -631,37 +670,19 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         pull_tile(acc, I, par, d.begin >> kTileBits, over, tid);
         if (tid == 0)
             atomicAdd(&ctl->pulled, 1ull);
-    }
-
-    // Otherwise loads are issued in order of their dependence chains: the
-    // parameter class, the neuron state, the parameter records (need the
-    // class), the queue head.
-    const uint32_t li = d.begin + tid;
-    const bool active = tid < d.count;
-    const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
-    // A tile whose neurons share one parameter class (the usual case: a
-    // population spans many tiles) reads its records without the class load.
-    const uint32_t c = kUniform ? d.pcls - 1u : (lc & I.pmask) + (uint32_t)gp(I.cls)[lc];
-    float4 J = ld4(I.J, lc);
-    float V = gp(I.V)[lc];
-    int32_t t = gp(I.tcnt)[lc];
-    uint32_t fl = gp(I.flag)[lc];
-    float iou = gp(I.iou)[lc];
-    const float4 pa = ld4(I.pA, c);
-    const float4 pb = ld4(I.pB, c);
-    const float4 g = ld4(I.gU, c);
-    const float4 vu = ld4(I.vU, c);
-    const float4 dc = ld4(I.dec, c);
-    // The queue is complete when the launch starts; entries past the count
-    // are ignored (the queue buffer has kQueuePrefetch slots of slack at its
-    // end, so the read needs no branch).
-    QSyn head{};
-    float4 dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (gat)
-        dl = ld4(I.dlt, lc);
-    else
-        head = q_in[d.qoff + min(tid, kQueuePrefetch - 1u)];
-    if (!pull) {
+        // loaded again, so that nothing loaded above is live across the call
+        J = ld4(I.J, lc);
+        V = gp(I.V)[lc];
+        t = gp(I.tcnt)[lc];
+        fl = gp(I.flag)[lc];
+        iou = gp(I.iou)[lc];
+        pa = ld4(I.pA, c);
+        pb = ld4(I.pB, c);
+        g = ld4(I.gU, c);
+        vu = ld4(I.vU, c);
+        dc = ld4(I.dec, c);
+        dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // (no gather here)
+    } else {
         const uint32_t cnt = gat ? 0u : min(cnt_raw, d.qcap);
         __syncthreads();
         accumulate_tile(acc, q_in + d.qoff, cnt, head, kQueuePrefetch, tid);
@@ -852,12 +873,60 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     }
     const uint32_t nbin = defer ? 0u : nr;
     unsigned long long *spill_out = pull_on ? nullptr : I.spill + (size_t)(par ^ 1u) * I.n * 4u;
+    // Split binning: the tile's passes (rows in LDS order, passes in row
+    // order) go to the engine's pass list for the k_bin launch that follows,
+    // where every wave of the GPU takes one; the tile bins here only the
+    // passes past the list's capacity.
+    uint32_t first_here = 0;   // tile pass index from which this tile bins itself
+    if (qs.pass != nullptr && nbin != 0u) {
+        if (w == 0) {
+            constexpr uint32_t kPass = kWave * K;
+            uint32_t total = 0;
+            for (uint32_t r0 = 0; r0 < nbin; r0 += kWave) {
+                const uint32_t r = r0 + lane;
+                const uint32_t np = r < nbin ? (row_len[r] + kPass - 1u) / kPass : 0u;
+                total += (uint32_t)wave_sum_u64((unsigned long long)np);
+            }
+            uint32_t base = 0;
+            if (lane == 0)
+                base = atomicAdd(qs.pcount + (size_t)par * kPassCountStride, total);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
+            const uint32_t fit = qs.pcap > base ? min(total, qs.pcap - base) : 0u;
+            uint32_t off = 0;   // passes of the rows of earlier chunks
+            for (uint32_t r0 = 0; r0 < nbin; r0 += kWave) {
+                const uint32_t r = r0 + lane;
+                const uint32_t len = r < nbin ? row_len[r] : 0u;
+                const uint32_t np = (len + kPass - 1u) / kPass;
+                uint32_t incl = np;
+#pragma unroll
+                for (uint32_t o = 1; o < (uint32_t)kWave; o <<= 1) {
+                    const uint32_t v = __shfl_up(incl, o, kWave);
+                    if (lane >= o)
+                        incl += v;
+                }
+                const uint32_t ex = off + incl - np;   // this row's first tile pass
+                const long long rb = r < nbin ? row_rb[r] : 0;
+                for (uint32_t j = 0; j < np && ex + j < fit; ++j) {
+                    PassEntry e;
+                    e.first = (uint64_t)(rb + (long long)j * kPass);
+                    e.len = min(kPass, len - j * kPass);
+                    e.inst = d.inst;
+                    qs.pass[base + ex + j] = e;
+                }
+                off += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+            }
+            if (lane == 0)
+                s_fit = fit;
+        }
+        __syncthreads();
+        first_here = s_fit;
+    }
     uint32_t pass = 0;
     for (uint32_t r = 0; r < nbin; ++r) {
         const long long rb_s = row_rb[r];
         const uint32_t len_s = row_len[r];
         for (uint32_t c0 = 0; c0 < len_s; c0 += kWave * K, ++pass)
-            if ((pass & 3u) == w)
+            if ((pass & 3u) == w && pass >= first_here)
             {
                 if constexpr (K == 1)
                     bin_pass_serial<K>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
@@ -891,12 +960,50 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
     __shared__ long long row_rb[kBlock];                  // fired rows of the tile: first synapse
     __shared__ uint32_t row_len[kBlock];                  // and length
     __shared__ uint32_t nrows;
+    __shared__ uint32_t s_fit;                            // split binning: passes in the list
 
     const TileDesc d = tiles[blockIdx.x];
     if (d.pcls != 0u)
-        step_tile<true, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
+        step_tile<true, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows, s_fit);
     else
-        step_tile<false, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows);
+        step_tile<false, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows, s_fit);
+}
+
+// Split binning: bins the passes that the k_step launch of parity `par`
+// listed (into queue set par ^ 1, as k_step would have), one pass per wave
+// at a time over the whole grid, so a step's propagation is spread over
+// every CU instead of the tiles whose neurons fired.  Also empties the other
+// parity's list count (its last reader was the previous k_bin; the next
+// k_step appends to it).
+template <uint32_t K>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_bin(const TileDesc *__restrict__ tiles,
+                                                const InstDev *__restrict__ insts,
+                                                QueueSet qs,
+                                                uint32_t par,
+                                                StepCtl *__restrict__ ctl)
+{
+    __shared__ unsigned long long seg_dst[4 * kWave * K];
+    __shared__ uint32_t seg_room[4 * kWave * K];
+    const uint32_t w = threadIdx.x / kWave;
+    const uint32_t lane = __lane_id();
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        qs.pcount[(size_t)(par ^ 1u) * kPassCountStride] = 0u;
+    const uint32_t total = min(qs.pcount[(size_t)par * kPassCountStride], qs.pcap);
+    const uint32_t nwaves = gridDim.x * (kBlock / kWave);
+    for (uint32_t i = blockIdx.x * (kBlock / kWave) + w; i < total; i += nwaves) {
+        const PassEntry e = qs.pass[i];
+        const InstDev &I = insts[e.inst];
+        QSyn *q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
+        unsigned long long *spill_out = I.ckey != nullptr ? nullptr : I.spill + (size_t)(par ^ 1u) * I.n * 4u;
+        if constexpr (K == 1)
+            bin_pass_serial<K>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
+                               (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, spill_out, ctl,
+                               seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
+        else
+            bin_pass<K>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
+                        (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, spill_out, ctl,
+                        seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
+    }
 }
 
 // Gather propagation (instances with InstDev::segidx; their k_step reads
@@ -916,7 +1023,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 // last reader was the previous k_gather).  Consecutive groups go to one XCD (workgroups
 // are dealt to the 8 XCDs round-robin by index), so the segments of
 // neighbouring groups, which share cache lines, meet in one L2.
-__global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__restrict__ groups,
+__global__ __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_gather(const GroupDesc *__restrict__ groups,
                                                          const InstDev *__restrict__ insts, uint32_t par)
 {
     constexpr uint32_t kWaves = kGatherBlock / kWave;
@@ -948,7 +1055,6 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
         gp(I.fcount)[((size_t)(par ^ 1u) * kFiredLists + tid) * kFiredCountStride] = 0u;
     __syncthreads();
     const auto fired = gp(I.fired) + (size_t)par * kFiredLists * I.fcap;
-    const auto col = gp(I.col);
     const uint32_t wfmt = I.wfmt;
     const uint32_t stride = I.ngroups + 1u;
     const uint32_t first = d.g << kGroupBits;
@@ -999,11 +1105,26 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
                 for (uint32_t st = kWave / 2; st > 0; st >>= 1)
                     if (s_off[wv][lo + st] <= ic)
                         lo += st;
-                const size_t j = (size_t)s_base[wv][lo] + (ic - s_off[wv][lo]);
-                cv[u] = col[j];
-                jw[u] = j;
+                jw[u] = (size_t)s_base[wv][lo] + (ic - s_off[wv][lo]);
             }
-            w_load<kGatherItems, size_t>(wb, I.w, jw, wfmt);
+            // one record per synapse: column and weight pair side by side
+            // (3 or 2 words at a word stride of grec_words(wfmt))
+            const auto rec = gp(I.grec);
+            if (wfmt == kWeightF32) {
+#pragma unroll
+                for (uint32_t u = 0; u < kGatherItems; ++u) {
+                    const auto r = rec + jw[u] * 3u;
+                    cv[u] = r[0];
+                    wb[u] = (unsigned long long)r[1] | ((unsigned long long)r[2] << 32);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kGatherItems; ++u) {
+                    const auto r = rec + jw[u] * 2u;
+                    cv[u] = r[0];
+                    wb[u] = r[1];
+                }
+            }
 #pragma unroll
             for (uint32_t u = 0; u < kGatherItems; ++u) {
                 if (ok[u]) {
@@ -1021,6 +1142,21 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(const GroupDesc *__rest
     __syncthreads();
     for (uint32_t i = tid; i < d.count; i += kGatherBlock)
         st4(I.dlt, first + i, acc_to_delta(acc, i));
+}
+
+// Gather records: synapse j's column and weight-pair bits side by side.
+__global__ __launch_bounds__(kBlock) void k_build_grec(const uint32_t *__restrict__ col, const void *__restrict__ w,
+                                                       uint32_t wfmt, uint64_t nsyn, uint32_t *__restrict__ rec)
+{
+    const uint32_t nw = grec_words(wfmt);
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < nsyn; j += (uint64_t)gridDim.x * kBlock) {
+        const unsigned long long b = w_raw(w, j, wfmt);
+        uint32_t *r = rec + j * nw;
+        r[0] = col[j];
+        r[1] = (uint32_t)b;
+        if (nw == 3u)
+            r[2] = (uint32_t)(b >> 32);
+    }
 }
 
 // Group offsets of the gather propagation, one wave per row: segidx[r][g] =
@@ -1301,6 +1437,28 @@ hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, cons
     return hipGetLastError();
 }
 
+hipError_t launch_bin_kernel(const TileDesc *tiles, uint32_t grid, const InstDev *insts, const QueueSet &qs,
+                             uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
+                             hipStream_t stream)
+{
+    if (grid == 0 || qs.pass == nullptr)
+        return hipSuccess;
+    auto go = [&](auto kern) {
+        if (start != nullptr)
+            hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
+                                  ctl);
+        else
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
+    };
+    switch (pass_k) {
+    case 1: go(k_bin<1>); break;
+    case 2: go(k_bin<2>); break;
+    case 4: go(k_bin<4>); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_gather(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
                          hipStream_t stream)
 {
@@ -1316,6 +1474,16 @@ hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const 
     if (ngroups == 0)
         return hipSuccess;
     hipExtLaunchKernelGGL(k_gather, dim3(ngroups), dim3(kGatherBlock), 0, stream, start, stop, 0, groups, insts, par);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_grec(const uint32_t *col, const void *w, uint32_t wfmt, uint64_t nsyn, uint32_t *rec,
+                             hipStream_t stream)
+{
+    if (nsyn == 0)
+        return hipSuccess;
+    const uint64_t nb = std::min<uint64_t>((nsyn + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_build_grec, dim3((uint32_t)nb), dim3(kBlock), 0, stream, col, w, wfmt, nsyn, rec);
     return hipGetLastError();
 }
 

@@ -16,6 +16,13 @@ hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, cons
                                     uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
                                     hipStream_t stream);
 
+// Split binning: bins the passes listed by the k_step launch of parity
+// `par` (grid workgroups, each wave one pass at a time); nothing when the
+// queue set has no pass list.  Events may be null.
+hipError_t launch_bin_kernel(const TileDesc *tiles, uint32_t grid, const InstDev *insts, const QueueSet &qs,
+                             uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
+                             hipStream_t stream);
+
 // Gather propagation: delta of fired list `par` into InstDev::dlt for every
 // group (runs before the k_step launch of parity `par`, and before
 // materialize / settle of set `par`); empties fired list par ^ 1.
@@ -26,6 +33,10 @@ hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const 
 
 // Group offsets of the gather propagation ([n][ngroups + 1], rows ordered
 // by target group).
+// Gather records ([nsyn][grec_words(wfmt)]: column, weight pair bits).
+hipError_t launch_build_grec(const uint32_t *col, const void *w, uint32_t wfmt, uint64_t nsyn, uint32_t *rec,
+                             hipStream_t stream);
+
 hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint32_t n, uint32_t ngroups,
                                uint16_t *segidx, hipStream_t stream);
 

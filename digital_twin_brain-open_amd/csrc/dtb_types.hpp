@@ -102,6 +102,10 @@ struct alignas(16) InstDev {
     // more than fcap rows; its count is word (p * kFiredLists + s) *
     // kFiredCountStride of fcount (one cache line per count).
     const uint16_t *segidx;     // [n][ngroups + 1]
+    // The synapses again as one record each (column, then the weight pair's
+    // raw bits: grec_words(wfmt) 32-bit words), so a gathered segment is one
+    // contiguous stream instead of two.
+    const uint32_t *grec;       // [S][grec_words(wfmt)]
     uint32_t *fired;            // [2][kFiredLists][fcap]
     uint32_t *fcount;           // [2][kFiredLists][kFiredCountStride]
     // Pull propagation (null ckey: off; never together with gather).  The
@@ -161,6 +165,13 @@ enum : uint32_t { kWeightF32 = 0, kWeightF16 = 1, kWeightU8 = 2 };
 __host__ __device__ inline uint32_t weight_bytes(uint32_t fmt)
 {
     return fmt == kWeightF32 ? 8u : fmt == kWeightF16 ? 4u : 2u;
+}
+
+// 32-bit words of a gather record: the column and the weight pair (fp32
+// pairs 2 words; fp16 pairs and uint8 codes 1 word).
+__host__ __device__ inline uint32_t grec_words(uint32_t fmt)
+{
+    return fmt == kWeightF32 ? 3u : 2u;
 }
 
 // Pull propagation limits: source rows are 23-bit fields of the in-edge key.
@@ -245,10 +256,31 @@ __host__ __device__ inline size_t count_words(size_t n, uint32_t shift)
     return ((n + (1u << shift) - 1u) >> shift) * kCountLineWords;
 }
 
+// One binning pass of a fired row handed from k_step to k_bin (split
+// binning): synapses [first, first + len) of instance `inst`'s CSR, len at
+// most one pass.
+struct PassEntry {
+    uint64_t first;
+    uint32_t len;
+    uint32_t inst;
+};
+// Words between the two pass-list counts (each on its own 128-byte line).
+constexpr uint32_t kPassCountStride = 32;
+// Most workgroups of a k_bin launch (8 per CU on 256 CUs: all resident).
+constexpr uint32_t kBinGridMax = 2048;
+
 struct QueueSet {
     uint32_t *count;    // [2][ntiles], tile t of set p at word count_slot(p * ntiles + t, cshift)
     uint32_t ntiles;
     uint32_t cshift;    // kCountShiftDense or kCountShiftSpread
+    // Split binning (pass == null: off): k_step of parity p appends its
+    // tiles' passes to `pass` (count at pcount[p * kPassCountStride]) and the
+    // k_bin launch after it bins them; passes past pcap are binned by their
+    // own tile.
+    PassEntry *pass;
+    uint32_t *pcount;
+    uint32_t pcap;
+    uint32_t pad;
 };
 
 // Sample request (reference stat_samples_kernel, stat_result.cpp:393-433).

@@ -373,6 +373,18 @@ class Engine:
             _lib.check(self._lib.dtb_engine_pull(self._h, o, d))
         return bool(o.value), int(d.value)
 
+    def split(self, on=None, capacity=0):
+        """Sets (when given) and returns split binning: the fired rows'
+        binning passes run in a second launch per step over the whole GPU
+        instead of in the tiles whose neurons fired (capacity: pass-list
+        entries, 0 = sized by the instances; passes past it are binned by
+        their own tile).  Same results either way."""
+        if on is not None:
+            _lib.check(self._lib.dtb_engine_split(self._h, c_int32(int(bool(on))), ctypes.c_uint32(int(capacity))))
+        o, c = c_int32(0), ctypes.c_uint32(0)
+        _lib.check(self._lib.dtb_engine_get_split(self._h, byref(o), byref(c)))
+        return bool(o.value)
+
     def pull_info(self, inst):
         """{"allowed", "built", "in_use", "bytes"} of an instance's pull
         propagation (in-edges)."""

@@ -33,6 +33,8 @@ BYTES_PER_NEURON = 2 + 4 + 4 + 4 + 1 + 16 + 16 + 64 + 1 + 8 + 1
 BYTES_PER_SYNAPSE = 4 + 8
 QUEUE_ENTRY_BYTES, QUEUE_FLOOR, QUEUE_SHIFT = 12, 4096, 3
 GATHER_GROUP, FIRED_LISTS = 2048, 64
+# Gather records: each synapse's column and weight pair side by side.
+GATHER_RECORD_BYTES = 12
 # Pull propagation: the in-edges (source | target | channel key and weight
 # pair per synapse), per tile its first in-edge, per wave of 64 neurons two
 # bitmap words per parity.
@@ -55,7 +57,8 @@ class Request:
 
     def device_bytes(self, gather=False, pull=False):
         """Device footprint; gather: also the gather buffers (2-byte offsets
-        per row and group of 2048 neurons, 64 fired lists per parity); pull
+        per row and group of 2048 neurons, 64 fired lists per parity, a
+        12-byte record per synapse); pull
         (without gather): also the in-edges of the pull propagation."""
         syn = self.n * self.k
         tiles = (self.n + 255) // 256
@@ -64,7 +67,7 @@ class Request:
         if gather:
             groups = (self.n + GATHER_GROUP - 1) // GATHER_GROUP
             fcap = (tiles + FIRED_LISTS - 1) // FIRED_LISTS * 256
-            b += self.n * (groups + 1) * 2 + 2 * FIRED_LISTS * (fcap + 32) * 4
+            b += self.n * (groups + 1) * 2 + 2 * FIRED_LISTS * (fcap + 32) * 4 + syn * GATHER_RECORD_BYTES
         elif pull:
             b += syn * PULL_BYTES_PER_SYNAPSE + (tiles + 1) * 8 + 4 * ((self.n + 63) // 64) * 8 + 16
         return b

@@ -258,6 +258,14 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
  * to its targets' pull instead of binning them (defer_min 0: never).
  * Default on, defer_min 64.  Every setting gives bit-identical results. */
 int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min);
+/* Split binning (the binning stage of src/stage/presynaptic_voltage.cpp:
+ * 122-225, scheduled differently): on = the fired rows' binning passes are
+ * listed by the step kernel and binned by a second launch per step over the
+ * whole GPU, instead of by the tiles whose neurons fired.  capacity = entries
+ * of the pass list (0: sized by the instances); passes past it are binned by
+ * their own tile.  Results never depend on either. */
+int dtb_engine_split(dtb_engine_t h, int32_t on, uint32_t capacity);
+int dtb_engine_get_split(dtb_engine_t h, int32_t *on, uint32_t *capacity);
 int dtb_engine_get_pull(dtb_engine_t h, int32_t *on, uint32_t *defer_min);
 /* Pull propagation of one instance: allowed (at most 2^23 neurons, has
  * synapses), in-edges built, used under the current settings (not while the
