@@ -283,6 +283,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
     ap.add_argument("--pull", type=int, choices=(0, 1), default=1,
                     help="pull propagation for synchronous volleys (in-edges grouped by target tile)")
+    ap.add_argument("--defer-min", type=int, default=-1,
+                    help="pull: emitted rows from which a tile defers them to the pull (default: the engine's)")
     ap.add_argument("--split", type=int, choices=(0, 1), default=0,
                     help="split binning: the fired rows' passes binned by a second launch per step")
     args = ap.parse_args()
@@ -326,7 +328,7 @@ def main():
     t0 = time.perf_counter()
     eng = dtb.Engine(0)
     eng.split(bool(args.split))
-    eng.pull(bool(args.pull))
+    eng.pull(bool(args.pull), None if args.defer_min < 0 else args.defer_min)
     blk = dtb.Block(net, delta_t=DT_MS, seed=sim_seed, engine=eng)
     blk.update_ou_background_stimuli(blk.subblk_id, OU_TAU, mu, sigma)
     eng.configure(blk.inst, 1, True)

@@ -342,7 +342,7 @@ uint64_t Instance::pull_bytes() const
 {
     const uint64_t ntile = (n_ + kBlock - 1) / kBlock;
     const uint64_t nw = (n_ + kWave - 1) / kWave;
-    return (uint64_t)nsyn_ * (sizeof(uint32_t) + weight_bytes(wfmt_)) + (ntile + 1) * sizeof(int64_t) +
+    return (uint64_t)nsyn_ * sizeof(uint32_t) * grec_words(wfmt_) + (ntile + 1) * sizeof(int64_t) +
            4 * nw * sizeof(unsigned long long) + 2 * sizeof(int64_t);
 }
 
@@ -364,8 +364,7 @@ void Instance::ensure_pull(hipStream_t s)
             throw Error(kInternal, "tile synapse counts do not add up to the synapses");
         cptr_.resize((size_t)ntile + 1);
         cptr_.upload(ptr.data(), ptr.size(), s);
-        ckey_.resize((size_t)nsyn_);
-        cw_.resize((size_t)nsyn_ * weight_bytes(wfmt_));
+        crec_.resize((size_t)nsyn_ * grec_words(wfmt_));
         ebits_.resize(2 * nw);
         dbits_.resize(2 * nw);
         dstamp_.resize(2);
@@ -376,7 +375,7 @@ void Instance::ensure_pull(hipStream_t s)
         std::vector<unsigned long long> cur(ptr.begin(), ptr.end() - 1);
         cursor.upload(cur.data(), ntile, s);
         DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), wfmt_, n_,
-                                  cursor.mutable_gpu_data(), ckey_.mutable_gpu_data(), cw_.mutable_gpu_data(), s));
+                                  cursor.mutable_gpu_data(), crec_.mutable_gpu_data(), s));
         DTB_HIP(hipStreamSynchronize(s));
     } catch (...) {
         release_pull();
@@ -387,8 +386,7 @@ void Instance::ensure_pull(hipStream_t s)
 
 void Instance::release_pull()
 {
-    ckey_.resize(0);
-    cw_.resize(0);
+    crec_.resize(0);
     cptr_.resize(0);
     ebits_.resize(0);
     dbits_.resize(0);
@@ -961,8 +959,7 @@ InstDev Instance::dev(uint32_t tile_base, bool gather, bool pull, uint32_t defer
     if (pull) {
         if (!pull_built_ || gather)
             throw Error(kInternal, "pull propagation needs the in-edges and no gather");
-        d.ckey = ckey_.gpu_data();
-        d.cw = cw_.gpu_data();
+        d.crec = crec_.gpu_data();
         d.cptr = cptr_.gpu_data();
         d.ebits = ebits_.mutable_gpu_data();
         d.dbits = dbits_.mutable_gpu_data();

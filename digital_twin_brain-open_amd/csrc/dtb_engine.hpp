@@ -431,9 +431,9 @@ private:
     DataAllocator<uint32_t> grec_;        // [nsyn][grec_words] gather records (built with segidx_)
     DataAllocator<uint32_t> fired_;       // [2][kFiredLists][fired_capacity()] fired rows
     DataAllocator<uint32_t> fcount_;      // [2][kFiredLists][kFiredCountStride]
-    bool pull_built_ = false;             // ckey_, cw_, cptr_, bitmaps built
-    DataAllocator<uint32_t> ckey_;        // [nsyn] in-edges: source row | target in tile | channel bit
-    DataAllocator<uint8_t> cw_;           // [nsyn] their weight pairs (format wfmt_)
+    bool pull_built_ = false;             // crec_, cptr_, bitmaps built
+    DataAllocator<uint32_t> crec_;        // [nsyn][grec_words] in-edges: key (source row | target in tile |
+                                          // channel bit), weight pair bits
     DataAllocator<int64_t> cptr_;         // [ntiles + 1] first in-edge of each target tile
     DataAllocator<unsigned long long> ebits_, dbits_;   // [2][ceil(n / 64)] emitted / deferred rows
     DataAllocator<int64_t> dstamp_;       // [2] consuming step of deferred rows
@@ -586,6 +586,7 @@ private:
     uint32_t cshift_ = kCountShiftDense;   // queue count layout (count_slot)
     int count_layout_ = kLayoutAuto;       // requested layout
     uint32_t pass_k_ = kPassKDefault;      // synapses per lane in a binning pass
+    uint32_t wfmt_ = kWeightF32;           // weight format of every instance in the tables, or kWeightAny
     uint32_t queue_floor_ = 4096, queue_shift_ = 3;   // queue_capacity
     bool tuning_ = false;                  // autotune running: gather buffers are kept
     bool pull_on_ = true;                  // pull propagation where in-edges fit

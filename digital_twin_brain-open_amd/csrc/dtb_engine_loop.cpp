@@ -279,12 +279,14 @@ void Engine::rebuild()
     uint64_t total = 0;
     double coverage = 0.0;
     uint64_t pass_want = 0;   // split binning: pass-list entries for the queue instances
+    uint32_t fmt = kWeightAny + 1;   // the instances' common weight format (kWeightAny: mixed)
     for (size_t slot = 0; slot < insts_.size(); ++slot) {
         Instance *p = insts_[slot].get();
         if (!p)
             continue;
         const uint32_t idx = (uint32_t)table.size();
         tile_base_[slot] = (uint32_t)tiles.size();
+        fmt = fmt == kWeightAny + 1 || fmt == p->weight_format() ? p->weight_format() : kWeightAny;
         uint64_t qtotal = 0;   // this instance's queue slots per parity
         const auto &pb = p->pop_begin();
         uint32_t q = 0;
@@ -337,6 +339,7 @@ void Engine::rebuild()
     if (total > 0xffffffffull || tiles.size() > 0xffffffffull)
         throw Error(kInvalidParameter, "too many neurons in one engine");
     ninst_ = (uint32_t)table.size();
+    wfmt_ = fmt > kWeightAny ? kWeightF32 : fmt;
     ntiles_ = (uint32_t)tiles.size();
     ngroups_ = (uint32_t)groups.size();
     nsamples_ = (uint32_t)samples.size();
@@ -399,16 +402,16 @@ void Engine::rebuild()
 
 void Engine::launch_gather_k(uint32_t par)
 {
-    DTB_HIP(launch_gather(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par, stream_));
+    DTB_HIP(launch_gather(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par, wfmt_, stream_));
 }
 
 void Engine::launch_step_k()
 {
     launch_gather_k(par_);
     DTB_HIP(launch_step_kernel(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
-                               d_ctl_.mutable_gpu_data(), pass_k_, stream_));
+                               d_ctl_.mutable_gpu_data(), pass_k_, wfmt_, stream_));
     DTB_HIP(launch_bin_kernel(d_tiles_.gpu_data(), bin_grid_, d_insts_.gpu_data(), queues(), par_,
-                              d_ctl_.mutable_gpu_data(), pass_k_, nullptr, nullptr, stream_));
+                              d_ctl_.mutable_gpu_data(), pass_k_, wfmt_, nullptr, nullptr, stream_));
 }
 
 void Engine::launch_aux()
@@ -551,13 +554,13 @@ void Engine::run_timed(int64_t steps, float *ms_step)
         for (int64_t k = 0; k < m; ++k) {
             hipEvent_t *e = &ev[(size_t)(6 * k)];
             if (ngroups_ > 0)
-                DTB_HIP(launch_gather_timed(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par_, e[2], e[3],
+                DTB_HIP(launch_gather_timed(d_groups_.gpu_data(), ngroups_, d_insts_.gpu_data(), par_, wfmt_, e[2], e[3],
                                             stream_));
             DTB_HIP(launch_step_kernel_timed(d_tiles_.gpu_data(), ntiles_, d_insts_.gpu_data(), queues(), par_,
-                                             d_ctl_.mutable_gpu_data(), pass_k_, e[0], e[1], stream_));
+                                             d_ctl_.mutable_gpu_data(), pass_k_, wfmt_, e[0], e[1], stream_));
             if (pass_cap_ > 0)
                 DTB_HIP(launch_bin_kernel(d_tiles_.gpu_data(), bin_grid_, d_insts_.gpu_data(), queues(), par_,
-                                          d_ctl_.mutable_gpu_data(), pass_k_, e[4], e[5], stream_));
+                                          d_ctl_.mutable_gpu_data(), pass_k_, wfmt_, e[4], e[5], stream_));
             par_ ^= 1u;
             launch_aux();
         }

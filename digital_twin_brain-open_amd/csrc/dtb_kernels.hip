@@ -37,6 +37,8 @@
 //   the queues) for sampling and state read-back; k_sample gathers samples.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "dtb_kernels.hpp"
 #include "dtb_rng.hpp"
 
@@ -214,43 +216,43 @@ __device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDe
 // Pull propagation: adds into acc (the tile's Q23.40 LDS sums) the weight
 // pairs of tile t's in-edges whose source row was emitted by the launch that
 // filled set `set`: every emitted row (`all`: the queue overflowed and is
-// ignored) or only the rows of tiles that deferred them (dbits).  The
-// in-edges are read coalesced, the source's bit from a per-wave bitmap
-// (n / 8 bytes, cache resident; read as 32-bit words), and the weight pair
-// only for a set bit.  A full pull is a chain of three dependent loads per
-// batch (keys, bits, weights), so each thread keeps kPullItems in-edges in
-// flight.  Called by the whole workgroup.
-constexpr uint32_t kPullItems = 6;
+// ignored) or only the rows of tiles that deferred them (dbits).  An in-edge
+// is one record (key, then the weight pair's bits: kW words), read
+// coalesced; the source's bit comes from a per-wave bitmap (n / 8 bytes,
+// cache resident; read as 32-bit words), so a batch is two dependent loads
+// (records, bit words); each thread keeps kPullItems in-edges in flight.
+// Called by the whole workgroup.
+constexpr uint32_t kPullItems = 4;
 template <uint32_t kFmt>
-__device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_t *key, const void *cw,
+__device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_t *rec,
                                           const unsigned long long *bits, uint32_t len, uint32_t tid)
 {
     constexpr uint32_t U = kPullItems;
-    const auto b32 = gp(reinterpret_cast<const uint32_t *>(bits));
-    const auto kp = gp(key);
+    constexpr uint32_t kW = kFmt == kWeightF32 ? 3u : 2u;   // grec_words(kFmt)
+    // uniform bases and 32-bit byte offsets (a tile's in-edges span less
+    // than 4 GB; the bitmap less than 1 MB): one VGPR per address
+    using gbyte = const __attribute__((address_space(1))) unsigned char;
+    gbyte *rb = reinterpret_cast<gbyte *>(gp(rec));
+    gbyte *bb = reinterpret_cast<gbyte *>(gp(reinterpret_cast<const uint32_t *>(bits)));
     for (uint32_t k0 = tid; k0 < len; k0 += kBlock * U) {
         uint32_t kv[U];
+        unsigned long long wr[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
             const uint32_t j = k0 + u * kBlock;
-            kv[u] = kp[j < len ? j : len - 1u];
+            const auto r = reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(
+                rb + (j < len ? j : len - 1u) * (kW * 4u));
+            kv[u] = r[0];
+            wr[u] = kW == 3u ? ((unsigned long long)r[1] | ((unsigned long long)r[2] << 32)) : r[1];
         }
         uint32_t bw[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
-            bw[u] = b32[(kv[u] & kPullRowMask) >> 5];
-        uint32_t hit = 0;   // bit u: in-edge k0 + u * kBlock exists and its source was emitted
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            if (k0 + u * kBlock < len && ((bw[u] >> (kv[u] & 31u)) & 1u) != 0u)
-                hit |= 1u << u;
-        unsigned long long wr[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            wr[u] = ((hit >> u) & 1u) ? w_raw(cw, k0 + u * kBlock, kFmt) : 0ull;
+            bw[u] = *reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(
+                bb + ((kv[u] & kPullRowMask) >> 5) * 4u);
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
-            if ((hit >> u) & 1u) {
+            if (k0 + u * kBlock < len && ((bw[u] >> (kv[u] & 31u)) & 1u) != 0u) {
                 const float2 wp = w_decode(wr[u], kFmt);
                 const uint32_t slot = ((kv[u] >> kPullRowBits) & 0xffu) * 4u + ((kv[u] & kKindBit) ? 2u : 0u);
                 atomicAdd(&acc[slot], f32_to_q40(wp.x));
@@ -261,7 +263,11 @@ __device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_
 }
 
 // Not inlined: the pull is the rare path (volleys, overflowing queues), and
-// as a call it keeps its registers out of the step kernel's budget.
+// as a call it keeps its registers out of the step kernel's budget.  Its
+// registers are the ones the caller must keep free across the call, so a
+// kernel specialised for one weight format calls the loop of that format
+// only (kFmt; kWeightAny: the instance's format at run time).
+template <uint32_t kFmt>
 __device__ __attribute__((noinline)) void pull_tile(unsigned long long *acc, const InstDev &I, uint32_t set,
                                                     uint32_t t, bool all, uint32_t tid)
 {
@@ -269,16 +275,14 @@ __device__ __attribute__((noinline)) void pull_tile(unsigned long long *acc, con
     const unsigned long long *bits = (all ? I.ebits : I.dbits) + (size_t)set * nw;
     const int64_t b = gp(I.cptr)[t];
     const uint32_t len = (uint32_t)(gp(I.cptr)[t + 1] - b);   // a tile has fewer than 2^32 in-edges
-    // uniform bases, 32-bit offsets per lane; one loop per weight format
-    const uint32_t *key = I.ckey + b;
-    const uint32_t fmt = I.wfmt;
-    const void *cw = w_row(I.cw, b, fmt);
+    const uint32_t fmt = kFmt != kWeightAny ? kFmt : I.wfmt;
+    const uint32_t *rec = I.crec + (size_t)b * grec_words(fmt);
     if (fmt == kWeightF32)
-        pull_loop<kWeightF32>(acc, key, cw, bits, len, tid);
+        pull_loop<kWeightF32>(acc, rec, bits, len, tid);
     else if (fmt == kWeightF16)
-        pull_loop<kWeightF16>(acc, key, cw, bits, len, tid);
+        pull_loop<kWeightF16>(acc, rec, bits, len, tid);
     else
-        pull_loop<kWeightU8>(acc, key, cw, bits, len, tid);
+        pull_loop<kWeightU8>(acc, rec, bits, len, tid);
 }
 
 // A statistic's value as the exact integer rint(x * 2^24).
@@ -380,7 +384,7 @@ __device__ __forceinline__ void count_over(StepCtl *__restrict__ ctl, uint32_t o
 
 // bin_pass with the loads and reservations of a lane's synapses one after the
 // other (measured faster for one synapse per lane).
-template <uint32_t K>
+template <uint32_t K, uint32_t kFmt>
 __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col_all, const void *w_all, uint32_t wfmt,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
@@ -388,6 +392,8 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
                                         StepCtl *__restrict__ ctl, unsigned long long *seg_dst, uint32_t *seg_room,
                                         uint32_t lane)
 {
+    if (kFmt != kWeightAny)
+        wfmt = kFmt;   // the launch's instances share one format: decoded without branches
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const auto col = gp(col_all) + row_begin;
     const void *wt = w_row(w_all, row_begin, wfmt);
@@ -487,7 +493,7 @@ __device__ __forceinline__ void bin_pass_serial(const uint32_t *__restrict__ col
 // queue's capacity add their weights into the target's spill accumulator
 // instead (64-bit integer atomics, Q23.40 like the LDS sums, so the total
 // does not depend on which path a synapse took).
-template <uint32_t K>
+template <uint32_t K, uint32_t kFmt>
 __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, const void *w_all, uint32_t wfmt,
                                         int64_t row_begin, uint32_t len, uint32_t c0, uint32_t tile_base,
                                         const TileDesc *__restrict__ tiles, uint32_t *__restrict__ qcnt, size_t qset, uint32_t cshift,
@@ -495,6 +501,8 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
                                         StepCtl *__restrict__ ctl, unsigned long long *seg_dst, uint32_t *seg_room,
                                         uint32_t lane)
 {
+    if (kFmt != kWeightAny)
+        wfmt = kFmt;   // the launch's instances share one format: decoded without branches
     constexpr uint32_t kNone = 0xffffffffu;   // "no start" marker for positions
     const auto col = gp(col_all) + row_begin;
     const void *wt = w_row(w_all, row_begin, wfmt);
@@ -603,7 +611,7 @@ __device__ __forceinline__ void bin_pass(const uint32_t *__restrict__ col_all, c
 // has parameter class d.pcls - 1, so the parameter records are read once per
 // tile through the scalar unit instead of per lane.  K: synapses per lane in
 // a binning pass (a pass is 64 K synapses).
-template <bool kUniform, uint32_t K>
+template <bool kUniform, uint32_t K, uint32_t kFmt>
 __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__restrict__ tiles,
                                           const InstDev *__restrict__ insts, const QueueSet &qs, uint32_t par,
                                           StepCtl *__restrict__ ctl, unsigned long long *acc,
@@ -625,7 +633,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     // head, the deferral stamp.
     const uint32_t cnt_raw = qs.count[cslot_in];
     const int64_t step = gp(I.step)[par];
-    const bool pull_on = I.ckey != nullptr;
+    const bool pull_on = I.crec != nullptr;
     const uint32_t li = d.begin + tid;
     const bool active = tid < d.count;
     const uint32_t lc = active ? li : d.begin;   // in-range index for idle lanes
@@ -667,7 +675,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         // An overflowed queue is ignored: every emitted row then comes
         // through the in-edges.
         accumulate_tile(acc, q_in + d.qoff, over ? 0u : cnt_raw, QSyn{}, 0u, tid);
-        pull_tile(acc, I, par, d.begin >> kTileBits, over, tid);
+        pull_tile<kFmt>(acc, I, par, d.begin >> kTileBits, over, tid);
         if (tid == 0)
             atomicAdd(&ctl->pulled, 1ull);
         // loaded again, so that nothing loaded above is live across the call
@@ -929,12 +937,12 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             if ((pass & 3u) == w && pass >= first_here)
             {
                 if constexpr (K == 1)
-                    bin_pass_serial<K>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                    bin_pass_serial<K, kFmt>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                        (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
                                        spill_out, ctl, seg_dst + w * kWave * K,
                                        seg_room + w * kWave * K, lane);
                 else
-                    bin_pass<K>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
+                    bin_pass<K, kFmt>(I.col, I.w, I.wfmt, (int64_t)rb_s, len_s, c0, I.tile_base, tiles, qs.count,
                                 (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out,
                                 spill_out, ctl, seg_dst + w * kWave * K,
                                 seg_room + w * kWave * K, lane);
@@ -947,7 +955,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
 // K: synapses per lane in a binning pass (the schedule; every K gives the
 // same results).  Eight waves per SIMD for K = 1, 2; K = 4's larger LDS
 // segment tables allow six.
-template <uint32_t K>
+template <uint32_t K, uint32_t kFmt>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ? 6 : 8))) void k_step(const TileDesc *__restrict__ tiles,
                                                  const InstDev *__restrict__ insts,
                                                  QueueSet qs,
@@ -964,9 +972,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 
     const TileDesc d = tiles[blockIdx.x];
     if (d.pcls != 0u)
-        step_tile<true, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows, s_fit);
+        step_tile<true, K, kFmt>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows, s_fit);
     else
-        step_tile<false, K>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows, s_fit);
+        step_tile<false, K, kFmt>(d, tiles, insts, qs, par, ctl, acc, seg_dst, seg_room, row_rb, row_len, nrows,
+                                  s_fit);
 }
 
 // Split binning: bins the passes that the k_step launch of parity `par`
@@ -975,7 +984,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(K == 4 ?
 // every CU instead of the tiles whose neurons fired.  Also empties the other
 // parity's list count (its last reader was the previous k_bin; the next
 // k_step appends to it).
-template <uint32_t K>
+template <uint32_t K, uint32_t kFmt>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_bin(const TileDesc *__restrict__ tiles,
                                                 const InstDev *__restrict__ insts,
                                                 QueueSet qs,
@@ -994,13 +1003,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
         const PassEntry e = qs.pass[i];
         const InstDev &I = insts[e.inst];
         QSyn *q_out = I.queue + (size_t)(par ^ 1u) * I.qslots;
-        unsigned long long *spill_out = I.ckey != nullptr ? nullptr : I.spill + (size_t)(par ^ 1u) * I.n * 4u;
+        unsigned long long *spill_out = I.crec != nullptr ? nullptr : I.spill + (size_t)(par ^ 1u) * I.n * 4u;
         if constexpr (K == 1)
-            bin_pass_serial<K>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
+            bin_pass_serial<K, kFmt>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
                                (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, spill_out, ctl,
                                seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
         else
-            bin_pass<K>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
+            bin_pass<K, kFmt>(I.col, I.w, I.wfmt, (int64_t)e.first, e.len, 0u, I.tile_base, tiles, qs.count,
                         (size_t)(par ^ 1u) * qs.ntiles, qs.cshift, q_out, spill_out, ctl,
                         seg_dst + w * kWave * K, seg_room + w * kWave * K, lane);
     }
@@ -1023,6 +1032,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
 // last reader was the previous k_gather).  Consecutive groups go to one XCD (workgroups
 // are dealt to the 8 XCDs round-robin by index), so the segments of
 // neighbouring groups, which share cache lines, meet in one L2.
+template <uint32_t kFmt>
 __global__ __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_gather(const GroupDesc *__restrict__ groups,
                                                          const InstDev *__restrict__ insts, uint32_t par)
 {
@@ -1055,7 +1065,7 @@ __global__ __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(8)
         gp(I.fcount)[((size_t)(par ^ 1u) * kFiredLists + tid) * kFiredCountStride] = 0u;
     __syncthreads();
     const auto fired = gp(I.fired) + (size_t)par * kFiredLists * I.fcap;
-    const uint32_t wfmt = I.wfmt;
+    const uint32_t wfmt = kFmt != kWeightAny ? kFmt : I.wfmt;
     const uint32_t stride = I.ngroups + 1u;
     const uint32_t first = d.g << kGroupBits;
     for (uint32_t c0 = wv * kWave; c0 < nf; c0 += kGatherBlock) {
@@ -1186,34 +1196,34 @@ __global__ __launch_bounds__(kBlock) void k_build_segidx(const int64_t *__restri
 }
 
 // In-edges of the pull propagation, one wave per source row: every synapse
-// of row r goes to its target tile's range of ckey / cw at the next free
-// position (cursor[t], starting at the tile's first in-edge).  The order
+// of row r goes to its target tile's range of records at the next free
+// position (cursor[t], starting at the tile's first in-edge) as (key =
+// source row | target in tile | channel bit, weight pair bits).  The order
 // within a tile follows the atomics; the pull's sums are exact integers, so
 // it does not matter.
 __global__ __launch_bounds__(kBlock) void k_build_pull(const int64_t *__restrict__ rowptr,
                                                        const uint32_t *__restrict__ col,
                                                        const void *__restrict__ w, uint32_t wfmt, uint32_t n,
                                                        unsigned long long *__restrict__ cursor,
-                                                       uint32_t *__restrict__ ckey, void *__restrict__ cw)
+                                                       uint32_t *__restrict__ rec)
 {
     const uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const uint32_t lane = __lane_id();
     if (r >= n)
         return;
+    const uint32_t nw = grec_words(wfmt);
     const int64_t rb = rowptr[r];
     const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
     for (uint32_t j = lane; j < len; j += kWave) {
         const uint32_t c = col[rb + j];
         const uint32_t t = (c & kColMask) >> kTileBits;
         const unsigned long long pos = atomicAdd(&cursor[t], 1ull);
-        ckey[pos] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
         const unsigned long long b = w_raw(w, (size_t)(rb + j), wfmt);
-        if (wfmt == kWeightF32)
-            static_cast<unsigned long long *>(cw)[pos] = b;
-        else if (wfmt == kWeightF16)
-            static_cast<uint32_t *>(cw)[pos] = (uint32_t)b;
-        else
-            static_cast<uint16_t *>(cw)[pos] = (uint16_t)b;
+        uint32_t *e = rec + pos * nw;
+        e[0] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
+        e[1] = (uint32_t)b;
+        if (nw == 3u)
+            e[2] = (uint32_t)(b >> 32);
     }
 }
 
@@ -1237,13 +1247,13 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt_raw = qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)];
     const bool over = cnt_raw > d.qcap;
-    const bool pull_on = I.ckey != nullptr;
+    const bool pull_on = I.crec != nullptr;
     const bool pull = pull_on && (over || gp(I.dstamp)[set] == gp(I.step)[set] + 1);
     const uint32_t cnt = (pull_on && over) ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
     if (pull)
-        pull_tile(acc, I, set, d.begin >> kTileBits, over, tid);
+        pull_tile<kWeightAny>(acc, I, set, d.begin >> kTileBits, over, tid);
     __syncthreads();
     if (tid < d.count) {
         if (!pull_on && over)
@@ -1287,13 +1297,13 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
     acc[tid * 4 + 3] = 0ull;
     const uint32_t cnt_raw = *cnt_p;
     const bool over = cnt_raw > d.qcap;
-    const bool pull_on = I.ckey != nullptr;
+    const bool pull_on = I.crec != nullptr;
     const bool pull = pull_on && (over || gp(I.dstamp)[set] == gp(I.step)[set] + 1);
     const uint32_t cnt = (pull_on && over) ? 0u : min(cnt_raw, d.qcap);
     __syncthreads();
     accumulate_tile(acc, I.queue + (size_t)set * I.qslots + d.qoff, cnt, QSyn{}, 0u, tid);
     if (pull)
-        pull_tile(acc, I, set, d.begin >> kTileBits, over, tid);
+        pull_tile<kWeightAny>(acc, I, set, d.begin >> kTileBits, over, tid);
     __syncthreads();
     if (tid < d.count) {
         const uint32_t li = d.begin + tid;
@@ -1399,82 +1409,113 @@ __global__ __launch_bounds__(kBlock) void k_normals(uint64_t seed, uint32_t neur
 
 // ---- launchers ----------------------------------------------------------
 
+// Calls f(kernel) with the kernel instantiated for pass size pass_k and
+// weight format wfmt (a format of every launched instance, or kWeightAny).
+template <template <uint32_t, uint32_t> class Sel, typename F>
+hipError_t with_kernel(uint32_t pass_k, uint32_t wfmt, F f)
+{
+    auto by_fmt = [&](auto kk) -> hipError_t {
+        constexpr uint32_t K = decltype(kk)::value;
+        switch (wfmt) {
+        case kWeightF32: f(Sel<K, kWeightF32>::get()); break;
+        case kWeightF16: f(Sel<K, kWeightF16>::get()); break;
+        case kWeightU8: f(Sel<K, kWeightU8>::get()); break;
+        case kWeightAny: f(Sel<K, kWeightAny>::get()); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipSuccess;
+    };
+    switch (pass_k) {
+    case 1: return by_fmt(std::integral_constant<uint32_t, 1>{});
+    case 2: return by_fmt(std::integral_constant<uint32_t, 2>{});
+    case 4: return by_fmt(std::integral_constant<uint32_t, 4>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <uint32_t K, uint32_t F>
+struct StepSel {
+    static auto get() { return k_step<K, F>; }
+};
+template <uint32_t K, uint32_t F>
+struct BinSel {
+    static auto get() { return k_bin<K, F>; }
+};
+
 hipError_t launch_step_kernel(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                              uint32_t par, StepCtl *ctl, uint32_t pass_k, hipStream_t stream)
+                              uint32_t par, StepCtl *ctl, uint32_t pass_k, uint32_t wfmt, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    switch (pass_k) {
-    case 1: hipLaunchKernelGGL(k_step<1>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
-    case 2: hipLaunchKernelGGL(k_step<2>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
-    case 4: hipLaunchKernelGGL(k_step<4>, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const hipError_t e = with_kernel<StepSel>(pass_k, wfmt, [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(ntiles), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
+    });
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_step_kernel_timed(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,
-                                    uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
-                                    hipStream_t stream)
+                                    uint32_t par, StepCtl *ctl, uint32_t pass_k, uint32_t wfmt, hipEvent_t start,
+                                    hipEvent_t stop, hipStream_t stream)
 {
     if (ntiles == 0)
         return hipSuccess;
-    switch (pass_k) {
-    case 1:
-        hipExtLaunchKernelGGL(k_step<1>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
-                              ctl);
-        break;
-    case 2:
-        hipExtLaunchKernelGGL(k_step<2>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
-                              ctl);
-        break;
-    case 4:
-        hipExtLaunchKernelGGL(k_step<4>, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
-                              ctl);
-        break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const hipError_t e = with_kernel<StepSel>(pass_k, wfmt, [&](auto kern) {
+        hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par, ctl);
+    });
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_bin_kernel(const TileDesc *tiles, uint32_t grid, const InstDev *insts, const QueueSet &qs,
-                             uint32_t par, StepCtl *ctl, uint32_t pass_k, hipEvent_t start, hipEvent_t stop,
-                             hipStream_t stream)
+                             uint32_t par, StepCtl *ctl, uint32_t pass_k, uint32_t wfmt, hipEvent_t start,
+                             hipEvent_t stop, hipStream_t stream)
 {
     if (grid == 0 || qs.pass == nullptr)
         return hipSuccess;
-    auto go = [&](auto kern) {
+    const hipError_t e = with_kernel<BinSel>(pass_k, wfmt, [&](auto kern) {
         if (start != nullptr)
             hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, start, stop, 0, tiles, insts, qs, par,
                                   ctl);
         else
             hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, tiles, insts, qs, par, ctl);
-    };
-    switch (pass_k) {
-    case 1: go(k_bin<1>); break;
-    case 2: go(k_bin<2>); break;
-    case 4: go(k_bin<4>); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    });
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
-hipError_t launch_gather(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
+namespace {
+template <typename F>
+hipError_t with_gather(uint32_t wfmt, F f)
+{
+    switch (wfmt) {
+    case kWeightF32: f(k_gather<kWeightF32>); break;
+    case kWeightF16: f(k_gather<kWeightF16>); break;
+    case kWeightU8: f(k_gather<kWeightU8>); break;
+    case kWeightAny: f(k_gather<kWeightAny>); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_gather(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par, uint32_t wfmt,
                          hipStream_t stream)
 {
     if (ngroups == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3(ngroups), dim3(kGatherBlock), 0, stream, groups, insts, par);
-    return hipGetLastError();
+    const hipError_t e = with_gather(wfmt, [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(ngroups), dim3(kGatherBlock), 0, stream, groups, insts, par);
+    });
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_gather_timed(const GroupDesc *groups, uint32_t ngroups, const InstDev *insts, uint32_t par,
-                               hipEvent_t start, hipEvent_t stop, hipStream_t stream)
+                               uint32_t wfmt, hipEvent_t start, hipEvent_t stop, hipStream_t stream)
 {
     if (ngroups == 0)
         return hipSuccess;
-    hipExtLaunchKernelGGL(k_gather, dim3(ngroups), dim3(kGatherBlock), 0, stream, start, stop, 0, groups, insts, par);
-    return hipGetLastError();
+    const hipError_t e = with_gather(wfmt, [&](auto kern) {
+        hipExtLaunchKernelGGL(kern, dim3(ngroups), dim3(kGatherBlock), 0, stream, start, stop, 0, groups, insts, par);
+    });
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_build_grec(const uint32_t *col, const void *w, uint32_t wfmt, uint64_t nsyn, uint32_t *rec,
@@ -1499,13 +1540,13 @@ hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint3
 }
 
 hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const void *w, uint32_t wfmt, uint32_t n,
-                             unsigned long long *cursor, uint32_t *ckey, void *cw, hipStream_t stream)
+                             unsigned long long *cursor, uint32_t *rec, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
     const uint32_t rows_per_block = kBlock / kWave;
     const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
-    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, wfmt, n, cursor, ckey, cw);
+    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, wfmt, n, cursor, rec);
     return hipGetLastError();
 }
 

@@ -108,10 +108,11 @@ struct alignas(16) InstDev {
     const uint32_t *grec;       // [S][grec_words(wfmt)]
     uint32_t *fired;            // [2][kFiredLists][fcap]
     uint32_t *fcount;           // [2][kFiredLists][kFiredCountStride]
-    // Pull propagation (null ckey: off; never together with gather).  The
-    // in-edges of target tile t are [cptr[t], cptr[t + 1]) of ckey / cw:
-    // ckey = source row (bits 0..22) | target's index in the tile (bits
-    // 23..30) | channel pair (bit 31), cw = weight pair.  Per wave of 64
+    // Pull propagation (null crec: off; never together with gather).  The
+    // in-edges of target tile t are records [cptr[t], cptr[t + 1]) of crec,
+    // grec_words(wfmt) words each: key = source row (bits 0..22) | target's
+    // index in the tile (bits 23..30) | channel pair (bit 31), then the
+    // weight pair's raw bits, so one load brings both.  Per wave of 64
     // neurons, ebits[s] holds the rows emitted by the launch that filled
     // queue set s and dbits[s] those of them whose tile deferred its
     // propagation to the pull; dstamp[s] = 1 + the step index of the launch
@@ -120,8 +121,7 @@ struct alignas(16) InstDev {
     // when its queue overflowed (then it ignores the queue and takes every
     // emitted row through its in-edges) or when rows were deferred (then it
     // adds the deferred rows to its queue).
-    const uint32_t *ckey;       // [S]
-    const void *cw;             // [S] weight pairs, format wfmt
+    const uint32_t *crec;       // [S][grec_words(wfmt)]
     const int64_t *cptr;        // [ntiles + 1]
     unsigned long long *ebits;  // [2][ceil(n / 64)]
     unsigned long long *dbits;  // [2][ceil(n / 64)]
@@ -162,6 +162,9 @@ struct alignas(16) InstDev {
 // (float)u / 255.f, correctly rounded).  Double weights are stored as fp32,
 // the type the reference's float block converts them to.
 enum : uint32_t { kWeightF32 = 0, kWeightF16 = 1, kWeightU8 = 2 };
+// Kernel specialisation for launches whose instances hold different formats
+// (each weight pair is then decoded by its instance's format at run time).
+constexpr uint32_t kWeightAny = 3;
 __host__ __device__ inline uint32_t weight_bytes(uint32_t fmt)
 {
     return fmt == kWeightF32 ? 8u : fmt == kWeightF16 ? 4u : 2u;
@@ -180,7 +183,7 @@ constexpr uint32_t kPullRowMask = (1u << kPullRowBits) - 1u;
 constexpr uint32_t kPullMaxNeurons = 1u << kPullRowBits;
 // Default deferral threshold: a tile whose step emitted at least this many
 // rows leaves them to the consumers' pull instead of binning them.
-constexpr uint32_t kDeferMinDefault = 64;
+constexpr uint32_t kDeferMinDefault = 32;
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
 // of one instance, and the queue of the synapses delivered to them.

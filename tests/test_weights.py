@@ -52,3 +52,36 @@ def test_compact_weights_bit_exact(dtb, orc, gpu, tmp_path, dtype, mode):
     assert np.array_equal(bits(out["vmemb"]), bits(ref["V"][:, ids]))
     assert_state_equal(st, sim)
     assert np.array_equal(bits(out["isynaptic"][-1]), bits(st["I_syn"][ids]))
+
+
+def test_mixed_weight_types_in_one_engine(dtb, orc, gpu, tmp_path):
+    """An fp32, a half and a uint8 block stepped by the same launches (the
+    kernels then decode each instance's pairs by its own type): each against
+    its own oracle run, by queues and by gather."""
+    steps, mu = 250, 1.5
+    base = [dtb.gen_uniform(7000, 120, seed=11), dtb.gen_uniform(5000, 90, seed=12), dtb.gen_uniform(9000, 150, seed=13)]
+    nets = [base[0]]
+    for net, dtype in ((base[1], "half"), (base[2], "uint8")):
+        path = dtb.save_npz(net, tmp_path / dtype, dtype)
+        nets.append(dtb.load_npz(path, weights="native")[0])
+    assert [n.w.dtype for n in nets] == [np.float32, np.float16, np.uint8]
+    for layout in ("auto", "gather"):
+        eng = dtb.Engine(0)
+        try:
+            eng.schedule(layout=layout)
+            blks = []
+            for i, net in enumerate(nets):
+                b = dtb.Block(net, delta_t=DT, seed=40 + i, engine=eng)
+                b.update_ou_background_stimuli(b.subblk_id, 4.0, mu, 0.1)
+                eng.configure(b.inst, 1, True)
+                blks.append(b)
+            eng.run(steps, graph_chunk=50)
+            res = [(b.state(), b.counters()) for b in blks]
+            eng.check()
+        finally:
+            eng.close()
+        for i, net in enumerate(nets):
+            ref, sim = run_oracle(orc, net, steps, 40 + i, mu)
+            assert ref["raster"].sum() > 500, (layout, i)
+            assert res[i][1][2] == int(ref["raster"].sum()), (layout, i)
+            assert_state_equal(res[i][0], sim)
