@@ -869,8 +869,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     const uint32_t nr = nrows;
     // A tile with many emitted rows (a synchronous volley) leaves them to
     // the consumers' pull: binning them would cost one reservation per
-    // (row, target tile) and overflow the queues.
-    const bool defer = pull_on && I.defer_min != 0u && nr >= I.defer_min;
+    // (row, target tile) and overflow the queues.  A tile that pulls in
+    // this launch (the volley is under way) defers only from twice the
+    // count: the volley's tail, where a few tiles still emit that many,
+    // would otherwise cost the next step a pull of every in-edge.
+    const bool defer = pull_on && I.defer_min != 0u && nr >= (pull ? 2u * I.defer_min : I.defer_min);
     if (pull_on) {
         if (lane == 0 && wave_in)
             gp(I.dbits)[bw] = defer ? m : 0ull;
