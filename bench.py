@@ -48,8 +48,8 @@ AUTOTUNE_TRIAL = 32
 AUTOTUNE_STEPS = 7 * (2 + (AUTOTUNE_TRIAL + 3) // 4 * 6)   # Engine.autotune_steps(AUTOTUNE_TRIAL)
 # rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, scaled by a device-copy
 # calibration, of profiled runs: each summary names its window (workload,
-# neurons, propagation layout, timed steps, warm-up steps) and is used only
-# for a run of exactly that window.
+# neurons, propagation layout, timed steps, warm-up steps, whether the
+# autotuner ran) and is used only for a run of exactly that window.
 PROFILES = ("k_step_voxel2M_in500.md", "k_step_voxel2M_in500_steps20_warmup5.md",
             "k_gather_k_step_uniform500k_in5000.md")
 
@@ -80,11 +80,12 @@ GATHER_ROW_BYTES, GATHER_ROW_GROUP_BYTES = 16, 4
 GATHER_SYN_BYTES = 12
 
 
-def profiled_traffic(workload, neurons, layout, steps, warmup):
-    """HBM bytes per launch measured with rocprofv3 counters over exactly this
-    window (workload, size, layout, timed and warm-up steps), or None."""
-    want = "window: workload=%s neurons=%d layout=%s steps=%d warmup=%d" % (workload, neurons, layout, steps,
-                                                                            warmup)
+def profiled_traffic(workload, neurons, layout, steps, warmup, autotuned):
+    """(HBM bytes per launch measured with rocprofv3 counters over exactly
+    this window (workload, size, layout, timed and warm-up steps, whether the
+    autotuner ran), the summary that holds it), or (None, None)."""
+    want = "window: workload=%s neurons=%d layout=%s steps=%d warmup=%d autotuned=%d" % (
+        workload, neurons, layout, steps, warmup, int(autotuned))
     for name in PROFILES:
         path = os.path.join(ROOT, "profiles", name)
         if not os.path.exists(path):
@@ -95,8 +96,8 @@ def profiled_traffic(workload, neurons, layout, steps, warmup):
             continue
         for line in lines:
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    return None
+                return float(line.split(":", 1)[1]), "profiles/" + name
+    return None, None
 
 
 def ensemble_bench(dtb, steps, warmup, chunk=64):
@@ -287,6 +288,9 @@ def main():
                     help="pull: emitted rows from which a tile defers them to the pull (default: the engine's)")
     ap.add_argument("--split", type=int, choices=(0, 1), default=0,
                     help="split binning: the fired rows' passes binned by a second launch per step")
+    ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
+                    help="0: keep the default schedule (128-synapse passes, dense counts) instead of autotuning, "
+                         "so that separate profiled runs time the same kernels")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
@@ -337,7 +341,7 @@ def main():
     # warm-up: steps into the firing regime, the autotuner's trial steps,
     # then steps that also capture the graph the timed region replays
     k_best, table = None, {}
-    if args.warmup >= AUTOTUNE_STEPS + 2 * GRAPH_CHUNK:
+    if args.autotune and args.warmup >= AUTOTUNE_STEPS + 2 * GRAPH_CHUNK:
         eng.run(args.warmup - AUTOTUNE_STEPS - GRAPH_CHUNK, graph_chunk=0)
         k_best, _, table = eng.autotune(AUTOTUNE_TRIAL)
         eng.run(GRAPH_CHUNK, graph_chunk=GRAPH_CHUNK)
@@ -388,8 +392,8 @@ def main():
                         + (LAYOUT_SYN_BYTES - 8 + wpair) * ev_step)
     kname = "k_gather+k_step" if gather else ("k_step+k_bin" if args.split else "k_step")
     achieved = algo_bytes / (us * 1e-6) / 1e9
-    traffic = (profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup)
-               if args.weights == "float32" else None)
+    traffic, traffic_src = (profiled_traffic(args.workload, args.neurons, layout_used, args.steps, args.warmup,
+                                             bool(k_best)) if args.weights == "float32" else (None, None))
     eager = args.steps < GRAPH_CHUNK
 
     res = {
@@ -430,7 +434,8 @@ def main():
                      "layout_bytes_per_launch": layout_bytes,
                      "layout_achieved": layout_bytes / (us * 1e-6) / 1e9,
                      "layout_frac": layout_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                     "traffic_achieved": None if traffic is None else traffic / (us * 1e-6) / 1e9},
+                     "traffic_achieved": None if traffic is None else traffic / (us * 1e-6) / 1e9,
+                     "traffic_source": traffic_src},
     }
     if args.cpu_steps > 0:
         t0 = time.perf_counter()

@@ -371,10 +371,9 @@ void Instance::ensure_pull(hipStream_t s)
         ebits_.mutable_gpu_data();
         dbits_.mutable_gpu_data();
         dstamp_.mutable_gpu_data();
-        DataAllocator<unsigned long long> cursor(ntile);
-        std::vector<unsigned long long> cur(ptr.begin(), ptr.end() - 1);
-        cursor.upload(cur.data(), ntile, s);
-        DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), wfmt_, n_,
+        DataAllocator<uint32_t> cursor(ntile);
+        DTB_HIP(hipMemsetAsync(cursor.mutable_gpu_data(), 0, (size_t)ntile * sizeof(uint32_t), s));
+        DTB_HIP(launch_build_pull(rowptr_.gpu_data(), col_.gpu_data(), w_.gpu_data(), wfmt_, n_, cptr_.gpu_data(),
                                   cursor.mutable_gpu_data(), crec_.mutable_gpu_data(), s));
         DTB_HIP(hipStreamSynchronize(s));
     } catch (...) {

@@ -1198,35 +1198,74 @@ __global__ __launch_bounds__(kBlock) void k_build_segidx(const int64_t *__restri
         s[g] = (uint16_t)len;
 }
 
-// In-edges of the pull propagation, one wave per source row: every synapse
-// of row r goes to its target tile's range of records at the next free
-// position (cursor[t], starting at the tile's first in-edge) as (key =
-// source row | target in tile | channel bit, weight pair bits).  The order
-// within a tile follows the atomics; the pull's sums are exact integers, so
-// it does not matter.
-__global__ __launch_bounds__(kBlock) void k_build_pull(const int64_t *__restrict__ rowptr,
-                                                       const uint32_t *__restrict__ col,
-                                                       const void *__restrict__ w, uint32_t wfmt, uint32_t n,
-                                                       unsigned long long *__restrict__ cursor,
-                                                       uint32_t *__restrict__ rec)
+// In-edges of the pull propagation: every synapse of source row r goes to
+// its target tile t's range of records (from cptr[t]) as (key = source row |
+// target in tile | channel bit, weight pair bits).  A workgroup takes
+// kBuildRows consecutive rows: it counts their synapses per target tile in
+// LDS, reserves each tile's share with one atomic on the tile's cursor
+// (cursor[t]: records placed so far, from 0), then reads the rows again and
+// places every record at its tile's reserved range, ranked by an LDS
+// atomic.  One device atomic per (workgroup, target tile) instead of one
+// per synapse: on the 2M-neuron voxel block a voxel's rows share their
+// target tiles, and per-synapse atomics on the same cursors serialised.
+// The order within a tile follows the atomics; the pull's sums are exact
+// integers, so it does not matter.  Instances of more than kBuildMaxTiles
+// tiles are built in windows of that many target tiles (the rows are read
+// again for each window).
+constexpr uint32_t kBuildBlock = 1024, kBuildRows = 256, kBuildMaxTiles = 16384;
+__global__ __launch_bounds__(kBuildBlock) void k_build_pull(const int64_t *__restrict__ rowptr,
+                                                            const uint32_t *__restrict__ col,
+                                                            const void *__restrict__ w, uint32_t wfmt, uint32_t n,
+                                                            uint32_t ntiles, const int64_t *__restrict__ cptr,
+                                                            uint32_t *__restrict__ cursor, uint32_t *__restrict__ rec)
 {
-    const uint32_t r = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    extern __shared__ uint32_t hist[];   // min(ntiles, kBuildMaxTiles) words
+    const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
-    if (r >= n)
-        return;
+    const uint32_t wv = tid / kWave;
+    constexpr uint32_t kWaves = kBuildBlock / kWave;
+    const uint32_t r0 = blockIdx.x * kBuildRows;
+    const uint32_t r1 = min(n, r0 + kBuildRows);
     const uint32_t nw = grec_words(wfmt);
-    const int64_t rb = rowptr[r];
-    const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
-    for (uint32_t j = lane; j < len; j += kWave) {
-        const uint32_t c = col[rb + j];
-        const uint32_t t = (c & kColMask) >> kTileBits;
-        const unsigned long long pos = atomicAdd(&cursor[t], 1ull);
-        const unsigned long long b = w_raw(w, (size_t)(rb + j), wfmt);
-        uint32_t *e = rec + pos * nw;
-        e[0] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
-        e[1] = (uint32_t)b;
-        if (nw == 3u)
-            e[2] = (uint32_t)(b >> 32);
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += kBuildMaxTiles) {
+        const uint32_t tw = min(kBuildMaxTiles, ntiles - t0);
+        for (uint32_t t = tid; t < tw; t += kBuildBlock)
+            hist[t] = 0u;
+        __syncthreads();
+        for (uint32_t r = r0 + wv; r < r1; r += kWaves) {
+            const int64_t rb = rowptr[r];
+            const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
+            for (uint32_t j = lane; j < len; j += kWave) {
+                const uint32_t t = ((col[rb + j] & kColMask) >> kTileBits) - t0;
+                if (t < tw)
+                    atomicAdd(&hist[t], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t t = tid; t < tw; t += kBuildBlock) {
+            const uint32_t h = hist[t];
+            if (h != 0u)
+                hist[t] = atomicAdd(&cursor[t0 + t], h);
+        }
+        __syncthreads();
+        for (uint32_t r = r0 + wv; r < r1; r += kWaves) {
+            const int64_t rb = rowptr[r];
+            const uint32_t len = (uint32_t)(rowptr[r + 1] - rb);
+            for (uint32_t j = lane; j < len; j += kWave) {
+                const uint32_t c = col[rb + j];
+                const uint32_t t = ((c & kColMask) >> kTileBits) - t0;
+                if (t >= tw)
+                    continue;
+                const uint32_t off = atomicAdd(&hist[t], 1u);
+                const unsigned long long b = w_raw(w, (size_t)(rb + j), wfmt);
+                uint32_t *e = rec + ((size_t)cptr[t0 + t] + off) * nw;
+                e[0] = r | ((c & (kBlock - 1u)) << kPullRowBits) | (c & kKindBit);
+                e[1] = (uint32_t)b;
+                if (nw == 3u)
+                    e[2] = (uint32_t)(b >> 32);
+            }
+        }
+        __syncthreads();   // the window's last ranks are taken before the next window zeroes the counts
     }
 }
 
@@ -1543,13 +1582,14 @@ hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint3
 }
 
 hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const void *w, uint32_t wfmt, uint32_t n,
-                             unsigned long long *cursor, uint32_t *rec, hipStream_t stream)
+                             const int64_t *cptr, uint32_t *cursor, uint32_t *rec, hipStream_t stream)
 {
     if (n == 0)
         return hipSuccess;
-    const uint32_t rows_per_block = kBlock / kWave;
-    const uint32_t nb = (n + rows_per_block - 1) / rows_per_block;
-    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBlock), 0, stream, rowptr, col, w, wfmt, n, cursor, rec);
+    const uint32_t ntiles = (n + kBlock - 1) / kBlock;
+    const uint32_t nb = (n + kBuildRows - 1) / kBuildRows;
+    hipLaunchKernelGGL(k_build_pull, dim3(nb), dim3(kBuildBlock), min(ntiles, kBuildMaxTiles) * sizeof(uint32_t),
+                       stream, rowptr, col, w, wfmt, n, ntiles, cptr, cursor, rec);
     return hipGetLastError();
 }
 

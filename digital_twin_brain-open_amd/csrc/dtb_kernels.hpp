@@ -42,10 +42,11 @@ hipError_t launch_build_grec(const uint32_t *col, const void *w, uint32_t wfmt, 
 hipError_t launch_build_segidx(const int64_t *rowptr, const uint32_t *col, uint32_t n, uint32_t ngroups,
                                uint16_t *segidx, hipStream_t stream);
 
-// In-edges of the pull propagation from the presynaptic-major CSR: cursor
-// [ntiles] holds each target tile's first in-edge and is advanced.
+// In-edges of the pull propagation from the presynaptic-major CSR: cptr
+// [ntiles + 1] holds each target tile's first in-edge; cursor [ntiles] is
+// zeroed and counts the records placed per tile.
 hipError_t launch_build_pull(const int64_t *rowptr, const uint32_t *col, const void *w, uint32_t wfmt, uint32_t n,
-                             unsigned long long *cursor, uint32_t *rec, hipStream_t stream);
+                             const int64_t *cptr, uint32_t *cursor, uint32_t *rec, hipStream_t stream);
 
 // Pending delta of queue set `set` into InstDev::dlt (queues left intact).
 hipError_t launch_materialize(const TileDesc *tiles, uint32_t ntiles, const InstDev *insts, const QueueSet &qs,

@@ -135,6 +135,39 @@ def test_pull_volley_from_rest_matches_oracle(dtb, orc, gpu):
     assert_state_equal(st, sim)
 
 
+def test_pull_in_edges_built_in_tile_windows(dtb, orc, gpu):
+    """An instance of more than 16,384 tiles (4.3M neurons): its in-edges are
+    built in two windows of target tiles.  Deferral from one emitted row and
+    queues of capacity 0 send every synapse through the pull; bit-exact for
+    40 steps through the volley from rest."""
+    n, k, steps, seed, mu = 4_300_000, 8, 40, 9, 8.0
+    net = dtb.gen_uniform(n, k, seed=13)
+    sample = np.arange(0, n, 4099, dtype=np.uint32)
+    print("case n", n, "k", k, flush=True)
+    eng = dtb.Engine(0)
+    try:
+        eng.pull(True, 1)
+        eng.queue_capacity(0, 31)
+        blk = dtb.Block(net, delta_t=DT, seed=seed, engine=eng)
+        assert eng.pull_info(blk.inst)["built"]
+        blk.update_ou_background_stimuli(blk.subblk_id, 4.0, mu, 0.5)
+        blk.set_samples(sample)
+        out = blk.run(steps, freqs=True, output_sample_spike=True, output_sample_vmemb=True)
+        st = blk.state()
+        stats = eng.propagation_stats()
+        eng.check()
+    finally:
+        eng.close()
+    ref, sim = run_oracle(orc, net, steps, seed, mu, sigma=0.5, vtrace=True)
+    print("fired", int(ref["raster"].sum()), "stats", stats)
+    assert ref["raster"].sum() > n // 2
+    assert stats["deferred"] > 0 and stats["pulled"] > 0
+    assert np.array_equal(out["freq"], ref["freq"])
+    assert np.array_equal(out["spike"], ref["raster"][:, sample])
+    assert np.array_equal(bits(out["vmemb"]), bits(ref["V"][:, sample]))
+    assert_state_equal(st, sim)
+
+
 def test_pull_batched_instances_match_oracle(dtb, orc, gpu):
     """Instances of different sizes and g_ui in one engine, every tile
     deferring; each instance against its own oracle run."""
