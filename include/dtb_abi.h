@@ -255,8 +255,10 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
  * memory is short).  A tile whose queue overflowed then takes every emitted
  * row through its in-edges instead of spill atomics, and a tile with at
  * least defer_min emitted rows in a step (a synchronous volley) leaves them
- * to its targets' pull instead of binning them (defer_min 0: never).
- * Default on, defer_min 32.  Every setting gives bit-identical results. */
+ * to its targets' pull instead of binning them (defer_min 0: never; a tile
+ * that itself pulls in that step defers from 2 * defer_min rows, so the
+ * volley's tail is binned).  Default on, defer_min 32.  Every setting gives
+ * bit-identical results. */
 int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min);
 /* Split binning (the binning stage of src/stage/presynaptic_voltage.cpp:
  * 122-225, scheduled differently): on = the fired rows' binning passes are
