@@ -817,9 +817,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             ev += (uint32_t)__builtin_amdgcn_readlane((int)len, __ffsll((long long)r) - 1);
         uint32_t base = 0;
         if ((int)lane == leader) {
-            auto c = gp(I.wave_ctr) + 2u * ((d.begin >> 6) + w);
-            c[0] += ev;
-            c[1] += (unsigned long long)cnt_w;
+            // the wave's own counters, added without waiting for a read
+            // (a load-add-store would stall the wave before its binning)
+            unsigned long long *c = I.wave_ctr + 2u * ((d.begin >> 6) + w);
+            atomicAdd(c, ev);
+            atomicAdd(c + 1, (unsigned long long)cnt_w);
             if (!gat)
                 base = atomicAdd(&nrows, cnt_w);
         }
