@@ -21,7 +21,9 @@ one instance at a time ("plain"), with every instance's counters and state
 checked identical between the two; and "serving": a stream of 100 requests
 (Poisson arrivals at 20/s, 10k-1M neurons, out-degree 100-1000, 100-2000 ms
 of model time) served with continuous batching, with p50/p99 completion
-latency, against its first 12 requests run alone one after the other.
+latency, against its first 12 requests run alone one after the other; the
+same 100 requests are also served as a backlog (all waiting at the start)
+for the server's capacity.
 """
 import argparse
 import importlib
@@ -185,6 +187,15 @@ def serving_bench(dtb, nreq, rate, plain):
         stats = srv.serve(reqs, realtime=True, progress=10.0)
     finally:
         srv.close()
+    # The same requests as a backlog (all waiting at the start): the server's
+    # capacity, without the stream's ramp-up and its last requests running
+    # with few others resident.
+    back = serving.poisson_requests(nreq, rate, SERVE_SEED)
+    srv = serving.Server(device=0)
+    try:
+        bst = srv.serve(back, realtime=False)
+    finally:
+        srv.close()
     t_plain, ev_plain, ns_plain, same = 0.0, 0, 0, True
     for r in reqs[:plain]:
         t0 = time.perf_counter()
@@ -199,7 +210,12 @@ def serving_bench(dtb, nreq, rate, plain):
                   "plain_requests": plain, "plain_events_per_s": ev_plain / t_plain,
                   "plain_neuron_steps_per_s": ns_plain / t_plain,
                   "speedup_vs_plain": stats["events_per_s"] / (ev_plain / t_plain),
-                  "plain_same_results": bool(same)})
+                  "plain_same_results": bool(same),
+                  "backlog_events_per_s": bst["events_per_s"], "backlog_wall_s": bst["wall_s"],
+                  "backlog_mean_resident": bst["mean_resident"],
+                  "speedup_vs_plain_backlog": bst["events_per_s"] / (ev_plain / t_plain),
+                  "backlog_same_results": all((a.events, a.spikes, a.v_hash) == (b.events, b.spikes, b.v_hash)
+                                              for a, b in zip(reqs, back))})
     return stats
 
 

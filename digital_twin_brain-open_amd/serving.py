@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from ._lib import OUT_OF_MEMORY, DtbError
-from .engine import DEFAULT_G_UI, Block, Engine, device_memory_reserve
+from .engine import DEFAULT_G_UI, Block, Engine, device_memory_cached, device_memory_reserve
 
 # Device bytes per neuron of an instance: parameter class, V, i_ou,
 # refractory counter, flag, J, pending delta, spill accumulators (2 x 32),
@@ -118,7 +118,11 @@ class Server:
         self.gather = layout == "gather"
         self.pull = bool(pull)
         self.keep_v = bool(keep_v)
-        self.budget = int(hbm_budget if hbm_budget is not None else 0.85 * device_free_bytes(device))
+        # the default budget counts the device memory the library already
+        # holds unused (a previous server's reservation, removed instances)
+        # as available, not only what the HIP runtime reports free
+        self.budget = int(hbm_budget if hbm_budget is not None else
+                          0.85 * (device_free_bytes(device) + device_memory_cached(device)))
         self.chunk = int(chunk)
         self.dt = dt
         if reserve:

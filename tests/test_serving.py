@@ -59,3 +59,20 @@ def test_device_memory_is_reused_and_zeroed(dtb, gpu, serving):
     assert again == first and first[0] > 0
     dtb.device_memory_trim()
     assert dtb.device_memory_cached(0) < held
+
+
+@pytest.mark.gpu
+def test_second_server_budget_counts_the_cache(dtb, gpu, serving):
+    """A server's reservation stays with the library after it closes; the
+    next server's default budget counts it as available instead of seeing
+    only what the HIP runtime reports free."""
+    a = serving.Server(device=0)
+    budget_a = a.budget
+    a.close()
+    assert dtb.device_memory_cached(0) >= budget_a // 2
+    b = serving.Server(device=0)
+    try:
+        assert b.budget >= 0.95 * budget_a, (b.budget, budget_a)
+    finally:
+        b.close()
+    dtb.device_memory_trim()
