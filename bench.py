@@ -297,7 +297,8 @@ def main():
     ap.add_argument("--weights", choices=("float32", "float16", "uint8"), default="float32",
                     help="weight pairs kept on the device (uint8: codes round(255 w), as the reference's uint8 "
                          "blocks; 2 bytes per synapse instead of 8)")
-    ap.add_argument("--ensemble-steps", type=int, default=2000, help="ensemble comparison length (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=5000,
+                    help="ensemble comparison length, the ensemble configuration's 5000 steps (0 = skip)")
     ap.add_argument("--pull", type=int, choices=(0, 1), default=1,
                     help="pull propagation for synchronous volleys (in-edges grouped by target tile)")
     ap.add_argument("--defer-min", type=int, default=-1,
