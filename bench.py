@@ -175,7 +175,7 @@ def ensemble_bench(dtb, steps, warmup, chunk=64):
 SERVE_REQUESTS, SERVE_RATE, SERVE_SEED, SERVE_PLAIN = 100, 20.0, 31, 12
 
 
-def serving_bench(dtb, nreq, rate, plain):
+def serving_bench(dtb, nreq, rate, plain, backlog=True):
     """Serving stream (Poisson arrivals at `rate`/s, networks of 10k-1M
     neurons at out-degree 100-1000 generated on the device, 100-2000 ms of
     model time each), continuous batching under the HBM budget; then the
@@ -190,12 +190,14 @@ def serving_bench(dtb, nreq, rate, plain):
     # The same requests as a backlog (all waiting at the start): the server's
     # capacity, without the stream's ramp-up and its last requests running
     # with few others resident.
-    back = serving.poisson_requests(nreq, rate, SERVE_SEED)
-    srv = serving.Server(device=0)
-    try:
-        bst = srv.serve(back, realtime=False)
-    finally:
-        srv.close()
+    back, bst = None, None
+    if backlog:
+        back = serving.poisson_requests(nreq, rate, SERVE_SEED)
+        srv = serving.Server(device=0)
+        try:
+            bst = srv.serve(back, realtime=False)
+        finally:
+            srv.close()
     t_plain, ev_plain, ns_plain, same = 0.0, 0, 0, True
     for r in reqs[:plain]:
         t0 = time.perf_counter()
@@ -210,12 +212,13 @@ def serving_bench(dtb, nreq, rate, plain):
                   "plain_requests": plain, "plain_events_per_s": ev_plain / t_plain,
                   "plain_neuron_steps_per_s": ns_plain / t_plain,
                   "speedup_vs_plain": stats["events_per_s"] / (ev_plain / t_plain),
-                  "plain_same_results": bool(same),
-                  "backlog_events_per_s": bst["events_per_s"], "backlog_wall_s": bst["wall_s"],
-                  "backlog_mean_resident": bst["mean_resident"],
-                  "speedup_vs_plain_backlog": bst["events_per_s"] / (ev_plain / t_plain),
-                  "backlog_same_results": all((a.events, a.spikes, a.v_hash) == (b.events, b.spikes, b.v_hash)
-                                              for a, b in zip(reqs, back))})
+                  "plain_same_results": bool(same)})
+    if bst is not None:
+        stats.update({"backlog_events_per_s": bst["events_per_s"], "backlog_wall_s": bst["wall_s"],
+                      "backlog_mean_resident": bst["mean_resident"],
+                      "speedup_vs_plain_backlog": bst["events_per_s"] / (ev_plain / t_plain),
+                      "backlog_same_results": all((a.events, a.spikes, a.v_hash) == (b.events, b.spikes, b.v_hash)
+                                                  for a, b in zip(reqs, back))})
     return stats
 
 
@@ -293,6 +296,8 @@ def main():
     ap.add_argument("--requests", type=int, default=SERVE_REQUESTS,
                     help="serving requests (the serving workload, and the block line's serving leg; 0 = skip "
                          "the leg)")
+    ap.add_argument("--backlog", type=int, choices=(0, 1), default=1,
+                    help="serving: also serve the same requests as a backlog, for the server's capacity")
     ap.add_argument("--neurons", type=int, default=0, help="override the workload's neuron count")
     ap.add_argument("--weights", choices=("float32", "float16", "uint8"), default="float32",
                     help="weight pairs kept on the device (uint8: codes round(255 w), as the reference's uint8 "
@@ -326,7 +331,7 @@ def main():
 
     if args.workload == "serving":
         t0 = time.perf_counter()
-        st = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN)
+        st = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN, bool(args.backlog))
         res = {"metric": "synaptic events/s (serving)", "value": st["events_per_s"], "unit": "synaptic events/s",
                "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                "data": "synthetic (seeded fan-out networks generated on the device)",
@@ -482,7 +487,7 @@ def main():
         log("ensemble %.1f s" % (time.perf_counter() - t0))
     if args.requests > 0 and args.workload == "block":
         t0 = time.perf_counter()
-        res["serving"] = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN)
+        res["serving"] = serving_bench(dtb, args.requests, SERVE_RATE, SERVE_PLAIN, bool(args.backlog))
         log("serving %.1f s" % (time.perf_counter() - t0))
     print(json.dumps(res), flush=True)
 
