@@ -182,7 +182,16 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// Adds the queued synapses of one tile into acc[256][4] (Q23.40, LDS): every
+// The LDS sums are channel-major: channel c of row i at acc[c * rows + i].
+// A wave's 64-bit atomics go to random rows; with the rows 8 bytes apart
+// (not 32, as row-major would put them) they spread over twice as many LDS
+// banks.
+__device__ __forceinline__ uint32_t acc_at(uint32_t row, uint32_t chan, uint32_t rows)
+{
+    return chan * rows + row;
+}
+
+// Adds the queued synapses of one tile into acc[4][256] (Q23.40, LDS): every
 // thread takes every 256th entry of the tile's queue (contiguous, so the
 // reads coalesce) and adds its weight pair with 64-bit integer LDS atomics,
 // so the sum does not depend on the order.  Threads tid < npre already hold
@@ -192,10 +201,18 @@ __device__ __forceinline__ void accumulate_tile(unsigned long long *acc, const Q
 {
     for (uint32_t k = tid; k < cnt; k += kBlock) {
         const QSyn e = (k < npre) ? pre : q[k];
-        const uint32_t slot = (e.key & 0xffu) * 4u + ((e.key & kQSynKind) ? 2u : 0u);
-        atomicAdd(&acc[slot], f32_to_q40(e.w0));
-        atomicAdd(&acc[slot + 1u], f32_to_q40(e.w1));
+        const uint32_t row = e.key & 0xffu, c0 = (e.key & kQSynKind) ? 2u : 0u;
+        atomicAdd(&acc[acc_at(row, c0, kBlock)], f32_to_q40(e.w0));
+        atomicAdd(&acc[acc_at(row, c0 + 1u, kBlock)], f32_to_q40(e.w1));
     }
+}
+
+// Zeroes row tid of a tile's sums (every row by the whole workgroup).
+__device__ __forceinline__ void clear_row(unsigned long long *acc, uint32_t tid)
+{
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c)
+        acc[acc_at(tid, c, kBlock)] = 0ull;
 }
 
 // Adds neuron li's spill accumulator of queue set `set` into the neuron's own
@@ -207,7 +224,7 @@ __device__ __forceinline__ void take_spill(unsigned long long *acc, const InstDe
     const auto s = gp(I.spill) + ((size_t)set * I.n + li) * 4u;
 #pragma unroll
     for (uint32_t c = 0; c < 4; ++c) {
-        acc[tid * 4u + c] += s[c];
+        acc[acc_at(tid, c, kBlock)] += s[c];
         if (consume)
             s[c] = 0ull;
     }
@@ -254,9 +271,9 @@ __device__ __forceinline__ void pull_loop(unsigned long long *acc, const uint32_
         for (uint32_t u = 0; u < U; ++u) {
             if (k0 + u * kBlock < len && ((bw[u] >> (kv[u] & 31u)) & 1u) != 0u) {
                 const float2 wp = w_decode(wr[u], kFmt);
-                const uint32_t slot = ((kv[u] >> kPullRowBits) & 0xffu) * 4u + ((kv[u] & kKindBit) ? 2u : 0u);
-                atomicAdd(&acc[slot], f32_to_q40(wp.x));
-                atomicAdd(&acc[slot + 1u], f32_to_q40(wp.y));
+                const uint32_t row = (kv[u] >> kPullRowBits) & 0xffu, c0 = (kv[u] & kKindBit) ? 2u : 0u;
+                atomicAdd(&acc[acc_at(row, c0, kBlock)], f32_to_q40(wp.x));
+                atomicAdd(&acc[acc_at(row, c0 + 1u, kBlock)], f32_to_q40(wp.y));
             }
         }
     }
@@ -332,13 +349,13 @@ __device__ __forceinline__ uint32_t current_fields(uint32_t stat_mask)
     return ((stat_mask & kStatI) ? 2u : 0u) | ((stat_mask & kStatReceptor) ? 0x3cu : 0u);
 }
 
-__device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, uint32_t i)
+__device__ __forceinline__ float4 acc_to_delta(const unsigned long long *acc, uint32_t i, uint32_t rows = kBlock)
 {
     float4 r;
-    r.x = q40_to_f32((long long)acc[i * 4u + 0u]);
-    r.y = q40_to_f32((long long)acc[i * 4u + 1u]);
-    r.z = q40_to_f32((long long)acc[i * 4u + 2u]);
-    r.w = q40_to_f32((long long)acc[i * 4u + 3u]);
+    r.x = q40_to_f32((long long)acc[acc_at(i, 0u, rows)]);
+    r.y = q40_to_f32((long long)acc[acc_at(i, 1u, rows)]);
+    r.z = q40_to_f32((long long)acc[acc_at(i, 2u, rows)]);
+    r.w = q40_to_f32((long long)acc[acc_at(i, 3u, rows)]);
     return r;
 }
 
@@ -664,10 +681,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     // overflowed or rows were deferred into this set: rare (volleys).
     const bool over = !gat && cnt_raw > d.qcap;
     const bool pull = !gat && pull_on && (over || stamp == step + 1);
-    acc[tid * 4 + 0] = 0ull;
-    acc[tid * 4 + 1] = 0ull;
-    acc[tid * 4 + 2] = 0ull;
-    acc[tid * 4 + 3] = 0ull;
+    clear_row(acc, tid);
     if (tid == 0)
         nrows = 0u;
     if (pull) {
@@ -1144,9 +1158,9 @@ __global__ __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(8)
             for (uint32_t u = 0; u < kGatherItems; ++u) {
                 if (ok[u]) {
                     const float2 wp = w_decode(wb[u], wfmt);
-                    const uint32_t sl = ((cv[u] & kColMask) - first) * 4u + ((cv[u] & kKindBit) ? 2u : 0u);
-                    atomicAdd(&acc[sl], f32_to_q40(wp.x));
-                    atomicAdd(&acc[sl + 1u], f32_to_q40(wp.y));
+                    const uint32_t row = (cv[u] & kColMask) - first, c0 = (cv[u] & kKindBit) ? 2u : 0u;
+                    atomicAdd(&acc[acc_at(row, c0, kNeurons)], f32_to_q40(wp.x));
+                    atomicAdd(&acc[acc_at(row, c0 + 1u, kNeurons)], f32_to_q40(wp.y));
                 }
             }
         }
@@ -1156,7 +1170,7 @@ __global__ __launch_bounds__(kGatherBlock) __attribute__((amdgpu_waves_per_eu(8)
     }
     __syncthreads();
     for (uint32_t i = tid; i < d.count; i += kGatherBlock)
-        st4(I.dlt, first + i, acc_to_delta(acc, i));
+        st4(I.dlt, first + i, acc_to_delta(acc, i, kNeurons));
 }
 
 // Gather records: synapse j's column and weight-pair bits side by side.
@@ -1285,10 +1299,7 @@ __global__ __launch_bounds__(kBlock) void k_materialize(const TileDesc *__restri
     if (I.segidx != nullptr)
         return;   // gather propagation: k_gather has written dlt
     const uint32_t tid = threadIdx.x;
-    acc[tid * 4 + 0] = 0ull;
-    acc[tid * 4 + 1] = 0ull;
-    acc[tid * 4 + 2] = 0ull;
-    acc[tid * 4 + 3] = 0ull;
+    clear_row(acc, tid);
     const uint32_t cnt_raw = qs.count[count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift)];
     const bool over = cnt_raw > d.qcap;
     const bool pull_on = I.crec != nullptr;
@@ -1335,10 +1346,7 @@ __global__ __launch_bounds__(kBlock) void k_settle(const TileDesc *__restrict__ 
         return;
     }
     uint32_t *cnt_p = qs.count + count_slot((size_t)set * qs.ntiles + blockIdx.x, qs.cshift);
-    acc[tid * 4 + 0] = 0ull;
-    acc[tid * 4 + 1] = 0ull;
-    acc[tid * 4 + 2] = 0ull;
-    acc[tid * 4 + 3] = 0ull;
+    clear_row(acc, tid);
     const uint32_t cnt_raw = *cnt_p;
     const bool over = cnt_raw > d.qcap;
     const bool pull_on = I.crec != nullptr;
