@@ -667,6 +667,11 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     float4 g = ld4(I.gU, c);
     float4 vu = ld4(I.vU, c);
     float4 dc = ld4(I.dec, c);
+    // The ends of the tile's first two populations (a tile rarely spans
+    // more) arrive with the state, not after the fold.
+    const uint32_t pop_hi = min(d.pop0 + 1u, I.npop - 1u);
+    uint32_t end_a = I.pops[d.pop0].end;
+    uint32_t end_b = I.pops[pop_hi].end;
     // The queue is complete when the launch starts; entries past the count
     // are ignored (the queue buffer has kQueuePrefetch slots of slack at its
     // end, so the read needs no branch).
@@ -703,6 +708,8 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         g = ld4(I.gU, c);
         vu = ld4(I.vU, c);
         dc = ld4(I.dec, c);
+        end_a = I.pops[d.pop0].end;
+        end_b = I.pops[pop_hi].end;
         dl = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // (no gather here)
     } else {
         const uint32_t cnt = gat ? 0u : min(cnt_raw, d.qcap);
@@ -724,8 +731,13 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         if (!(fl & kFlagFolded))
             J = fold_delta(J, dc, gat ? dl : acc_to_delta(acc, tid));
         isyn = syn_current(g, vu, J, V);
-        while (p + 1 < I.npop && li >= I.pops[p].end)
-            ++p;
+        // the population of neuron li (populations in order of their ends)
+        if (li >= end_a && p + 1 < I.npop) {
+            p = pop_hi;
+            if (li >= end_b)
+                while (p + 1 < I.npop && li >= I.pops[p].end)
+                    ++p;
+        }
     }
     // stat_result.cpp:266-391: population sums, blocked neurons excluded.
     // The I_syn-based statistics belong to the previous step (its J is
