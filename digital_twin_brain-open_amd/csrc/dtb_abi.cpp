@@ -661,7 +661,7 @@ int dtb_engine_spilled(dtb_engine_t h, uint64_t *synapses)
 int dtb_engine_propagation_stats(dtb_engine_t h, uint64_t *spilled, uint64_t *deferred_tiles, uint64_t *pulled_tiles)
 {
     return guarded([&] {
-        const dtb::StepCtl c = engine(h).propagation_stats();
+        const dtb::CtlTotals c = engine(h).propagation_stats();
         if (spilled)
             *spilled = c.spilled;
         if (deferred_tiles)

@@ -464,7 +464,7 @@ public:
     // Since the engine was made: synapses past a queue's capacity (spilled,
     // or dropped and taken by the pull; not an error), tiles that deferred
     // their rows, tiles that pulled.
-    StepCtl propagation_stats();
+    CtlTotals propagation_stats();
     uint64_t spilled() { return propagation_stats().spilled; }
     // Step-kernel schedule: synapses per lane in a binning pass (1, 2, 4) and
     // the propagation layout: queues with dense or spread counts

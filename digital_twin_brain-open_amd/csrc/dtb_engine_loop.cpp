@@ -207,12 +207,18 @@ void Engine::try_pull(Instance &p, hipStream_t s)
     }
 }
 
-StepCtl Engine::propagation_stats()
+CtlTotals Engine::propagation_stats()
 {
     StepCtl c{};
     d_ctl_.download(&c, 1, stream_);
     DTB_HIP(hipStreamSynchronize(stream_));
-    return c;
+    CtlTotals t{};
+    for (uint32_t k = 0; k < kCtlSlots; ++k) {
+        t.spilled += c.slot[k].spilled;
+        t.deferred += c.slot[k].deferred;
+        t.pulled += c.slot[k].pulled;
+    }
+    return t;
 }
 
 uint64_t Engine::queue_slots(const Instance &p) const

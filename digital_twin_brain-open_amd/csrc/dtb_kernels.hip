@@ -395,7 +395,7 @@ __device__ __forceinline__ void count_over(StepCtl *__restrict__ ctl, uint32_t o
     if (__ballot(over != 0u) != 0ull) {
         const unsigned long long s = wave_sum_u64((unsigned long long)over);
         if (lane == 0)
-            atomicAdd(&ctl->spilled, s);
+            atomicAdd(&ctl->slot[blockIdx.x & (kCtlSlots - 1u)].spilled, s);
     }
 }
 
@@ -696,7 +696,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
         accumulate_tile(acc, q_in + d.qoff, over ? 0u : cnt_raw, QSyn{}, 0u, tid);
         pull_tile<kFmt>(acc, I, par, d.begin >> kTileBits, over, tid);
         if (tid == 0)
-            atomicAdd(&ctl->pulled, 1ull);
+            atomicAdd(&ctl->slot[blockIdx.x & (kCtlSlots - 1u)].pulled, 1ull);
         // loaded again, so that nothing loaded above is live across the call
         J = ld4(I.J, lc);
         V = gp(I.V)[lc];
@@ -907,7 +907,7 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
             gp(I.dbits)[bw] = defer ? m : 0ull;
         if (defer && tid == 0) {
             gp(I.dstamp)[par ^ 1u] = step + 2;   // consumer's step + 1 (0: none)
-            atomicAdd(&ctl->deferred, 1ull);
+            atomicAdd(&ctl->slot[blockIdx.x & (kCtlSlots - 1u)].deferred, 1ull);
         }
     }
     const uint32_t nbin = defer ? 0u : nr;

@@ -309,11 +309,24 @@ struct alignas(16) SampleRecDev {
 constexpr size_t kScratchBytes = (size_t)64 << 20;
 
 // Shared per-engine diagnostics in device memory.
-struct alignas(16) StepCtl {
+// Diagnostic counters of the step launches.  Each is spread over kCtlSlots
+// cache lines, a workgroup adding into slot blockIdx.x % kCtlSlots: atomics
+// from every tile of a launch on one address would be serialised by the
+// memory and hold the launch's end (a volley step defers or pulls in all
+// 7,813 tiles of the 2M-neuron block).
+constexpr uint32_t kCtlSlots = 64;
+struct alignas(128) CtlSlot {
     unsigned long long spilled;   // synapses past a queue's capacity (spilled, or taken by the pull)
     unsigned long long deferred;  // tiles that left their emitted rows to the pull
     unsigned long long pulled;    // tiles that took input through their in-edges (step launches)
-    unsigned long long pad;
+    unsigned long long pad[13];
+};
+struct StepCtl {
+    CtlSlot slot[kCtlSlots];
+};
+// The counters summed over their slots.
+struct CtlTotals {
+    unsigned long long spilled, deferred, pulled;
 };
 
 }  // namespace dtb
