@@ -898,10 +898,10 @@ __device__ __forceinline__ void step_tile(const TileDesc &d, const TileDesc *__r
     // A tile with many emitted rows (a synchronous volley) leaves them to
     // the consumers' pull: binning them would cost one reservation per
     // (row, target tile) and overflow the queues.  A tile that pulls in
-    // this launch (the volley is under way) defers only from twice the
-    // count: the volley's tail, where a few tiles still emit that many,
+    // this launch (the volley is under way) defers only from four times
+    // the count: the volley's tail, where a few tiles still emit that many,
     // would otherwise cost the next step a pull of every in-edge.
-    const bool defer = pull_on && I.defer_min != 0u && nr >= (pull ? 2u * I.defer_min : I.defer_min);
+    const bool defer = pull_on && I.defer_min != 0u && nr >= (pull ? 4u * I.defer_min : I.defer_min);
     if (pull_on) {
         if (lane == 0 && wave_in)
             gp(I.dbits)[bw] = defer ? m : 0ull;

@@ -183,7 +183,7 @@ constexpr uint32_t kPullRowMask = (1u << kPullRowBits) - 1u;
 constexpr uint32_t kPullMaxNeurons = 1u << kPullRowBits;
 // Default deferral threshold: a tile whose step emitted at least this many
 // rows leaves them to the consumers' pull instead of binning them.
-constexpr uint32_t kDeferMinDefault = 32;
+constexpr uint32_t kDeferMinDefault = 16;
 
 // One tile = one workgroup of the step kernel: up to 256 consecutive neurons
 // of one instance, and the queue of the synapses delivered to them.

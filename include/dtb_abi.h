@@ -256,8 +256,8 @@ int dtb_engine_queue_capacity(dtb_engine_t h, uint32_t floor, uint32_t shift);
  * row through its in-edges instead of spill atomics, and a tile with at
  * least defer_min emitted rows in a step (a synchronous volley) leaves them
  * to its targets' pull instead of binning them (defer_min 0: never; a tile
- * that itself pulls in that step defers from 2 * defer_min rows, so the
- * volley's tail is binned).  Default on, defer_min 32.  Every setting gives
+ * that itself pulls in that step defers from 4 * defer_min rows, so the
+ * volley's tail is binned).  Default on, defer_min 16.  Every setting gives
  * bit-identical results. */
 int dtb_engine_pull(dtb_engine_t h, int32_t on, uint32_t defer_min);
 /* Split binning (the binning stage of src/stage/presynaptic_voltage.cpp:
